@@ -1,0 +1,425 @@
+// Steady lid-driven cavity operator: residual, stencil Jacobian and fused loss
+// adjoint (reference: ns_steady.py:142-394, defaults bc=0, rhiechow=1).
+//
+// Unknowns u, v, p on a cell-centred (nx, ny) grid, x along axis 0, y along
+// axis 1 (contiguous). Residuals:
+//   fu = uf u_x + vf u_y - mu lap(u) + p_x
+//   fv = uf v_x + vf v_y - mu lap(v) + p_y
+//   fp = Rhie-Chow face-flux divergence
+// with deferred-correction upwinding u_x = hi2(uf) + lo1(u) - lo1(uf)
+// (ns_steady.py:171-204,282-283,320-323). Frozen copies (uf, vf, pf) carry
+// no derivative (util_op.py:541-543), so the live Jacobian is
+//   fu <- u: lo1 upwind + Laplacian (5-point), fu <- p: central (3-point)
+//   fp <- u, v: face averages, fp <- p: Rhie-Chow pressure Laplacian
+// and every equation is linear in the live fields.
+#include "stencil2d.h"
+
+namespace odil {
+
+struct CavityParams {
+  double dx, dy, mu;
+  int bc;        // 0: no-slip + lid, 1: no-penetration, 2: extrapolation
+  int rhiechow;  // 1: Rhie-Chow continuity, 0: central divergence
+};
+
+// Halo rules per field and face (ns_steady.py:219-257).
+struct FaceRule {
+  int kx, ky;       // extrapolation kind on x-faces and y-faces
+  double wxm, wxp;  // wall values on x = 0 and x = 1
+  double wym, wyp;  // wall values on y = 0 and y = 1
+};
+
+__host__ __device__ inline FaceRule cavity_rule(int bc, int field) {
+  if (bc == 0) return FaceRule{QUADH, QUADH, 0, 0, 0, field == 0 ? 1. : 0.};
+  if (bc == 1)
+    return field == 0 ? FaceRule{QUADH, LINEAR, 0, 0, 0, 0}
+                      : FaceRule{LINEAR, QUADH, 0, 0, 0, 0};
+  return FaceRule{LINEAR, LINEAR, 0, 0, 0, 0};
+}
+
+// Halo-corrected neighbours of one field around cell (i, j): distance-1
+// values and the distance-2 values derived from them (stencil5,
+// ns_steady.py:269-280).
+template <class T>
+struct Nbr {
+  T c, xm, xp, ym, yp, xmm, xpp, ymm, ypp;
+};
+
+template <class T, class Acc>
+__device__ inline Nbr<T> neighbours(const Acc& q, int k, int i, int j, int nx,
+                                    int ny, const FaceRule& r) {
+  Nbr<T> s;
+  const T c = q(k, 0, 0);
+  const T rxm = q(k, -1, 0), rxp = q(k, 1, 0);
+  const T rym = q(k, 0, -1), ryp = q(k, 0, 1);
+  const T wxm = T(r.wxm), wxp = T(r.wxp), wym = T(r.wym), wyp = T(r.wyp);
+  s.c = c;
+  s.xm = (i == 0) ? ghost(r.kx, rxp, c, wxm) : rxm;
+  s.xp = (i == nx - 1) ? ghost(r.kx, rxm, c, wxp) : rxp;
+  s.ym = (j == 0) ? ghost(r.ky, ryp, c, wym) : rym;
+  s.yp = (j == ny - 1) ? ghost(r.ky, rym, c, wyp) : ryp;
+  s.xmm = (i == 0)   ? s.xm
+          : (i == 1) ? ghost(r.kx, c, rxm, wxm)
+                     : q(k, -2, 0);
+  s.xpp = (i == nx - 1)   ? s.xp
+          : (i == nx - 2) ? ghost(r.kx, c, rxp, wxp)
+                          : q(k, 2, 0);
+  s.ymm = (j == 0)   ? s.ym
+          : (j == 1) ? ghost(r.ky, c, rym, wym)
+                     : q(k, 0, -2);
+  s.ypp = (j == ny - 1)   ? s.yp
+          : (j == ny - 2) ? ghost(r.ky, c, ryp, wyp)
+                          : q(k, 0, 2);
+  return s;
+}
+
+// Effective-neighbour coefficients folded onto raw values: where a
+// neighbour is a wall halo, its coefficient moves to the far and near inner
+// values of the extrapolation.
+template <class T>
+__device__ inline void fold(T* raw, const T* eff, int i, int j, int nx, int ny,
+                            int kx, int ky) {
+  // raw/eff order: c, xm, xp, ym, yp
+  raw[0] = eff[0];
+  raw[1] = eff[1];
+  raw[2] = eff[2];
+  raw[3] = eff[3];
+  raw[4] = eff[4];
+  if (i == 0) {
+    raw[2] += eff[1] * ghost_dfar<T>(kx);
+    raw[0] += eff[1] * ghost_dnear<T>(kx);
+    raw[1] = 0;
+  }
+  if (i == nx - 1) {
+    raw[1] += eff[2] * ghost_dfar<T>(kx);
+    raw[0] += eff[2] * ghost_dnear<T>(kx);
+    raw[2] = 0;
+  }
+  if (j == 0) {
+    raw[4] += eff[3] * ghost_dfar<T>(ky);
+    raw[0] += eff[3] * ghost_dnear<T>(ky);
+    raw[3] = 0;
+  }
+  if (j == ny - 1) {
+    raw[3] += eff[4] * ghost_dfar<T>(ky);
+    raw[0] += eff[4] * ghost_dnear<T>(ky);
+    raw[4] = 0;
+  }
+}
+
+template <class T>
+struct CavityOp {
+  using Params = CavityParams;
+  static constexpr int NF = 3, NE = 3, NS = 5, R = 2, RJ = 1;
+  // Shift order: centre, x-1, x+1, y-1, y+1.
+  static constexpr int SX[NS] = {0, -1, 1, 0, 0};
+  static constexpr int SY[NS] = {0, 0, 0, -1, 1};
+  // Non-zero Jacobian entries (equation, field, shift): 27.
+  static constexpr int NJ = 27;
+  static constexpr int JE[NJ] = {0, 0, 0, 0, 0, 0, 0, 0,        //
+                                 1, 1, 1, 1, 1, 1, 1, 1,        //
+                                 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2};
+  static constexpr int JV[NJ] = {0, 0, 0, 0, 0, 2, 2, 2,        //
+                                 1, 1, 1, 1, 1, 2, 2, 2,        //
+                                 0, 0, 0, 1, 1, 1, 2, 2, 2, 2, 2};
+  static constexpr int JS[NJ] = {0, 1, 2, 3, 4, 0, 1, 2,        //
+                                 0, 1, 2, 3, 4, 0, 3, 4,        //
+                                 0, 1, 2, 0, 3, 4, 0, 1, 2, 3, 4};
+
+  template <class Acc>
+  __device__ static inline void compute(const Acc& q, int i, int j, int nx,
+                                        int ny, const Params& P, T* f,
+                                        T (*J)[NF][NS], bool want_jac) {
+    const T dx = T(P.dx), dy = T(P.dy), mu = T(P.mu);
+    const FaceRule ru = cavity_rule(P.bc, 0), rv = cavity_rule(P.bc, 1);
+    const FaceRule rp{LINEAR, LINEAR, 0, 0, 0, 0};
+    const Nbr<T> U = neighbours<T>(q, 0, i, j, nx, ny, ru);
+    const Nbr<T> V = neighbours<T>(q, 1, i, j, nx, ny, rv);
+    const Nbr<T> Pn = neighbours<T>(q, 2, i, j, nx, ny, rp);
+    const T a = U.c, b = V.c;  // frozen velocities at the cell
+    const T idx = T(1) / dx, idy = T(1) / dy;
+    const T h2x = T(1) / (T(2) * dx), h2y = T(1) / (T(2) * dy);
+
+    // Second-order upwind of the frozen field, first order at walls.
+    auto hi2x = [&](const Nbr<T>& s) {
+      const T m = (i == 0) ? (s.c - s.xm) * idx
+                           : (s.xmm - T(4) * s.xm + T(3) * s.c) * h2x;
+      const T p = (i == nx - 1) ? (s.xp - s.c) * idx
+                                : (-T(3) * s.c + T(4) * s.xp - s.xpp) * h2x;
+      return a > T(0) ? m : p;
+    };
+    auto hi2y = [&](const Nbr<T>& s) {
+      const T m = (j == 0) ? (s.c - s.ym) * idy
+                           : (s.ymm - T(4) * s.ym + T(3) * s.c) * h2y;
+      const T p = (j == ny - 1) ? (s.yp - s.c) * idy
+                                : (-T(3) * s.c + T(4) * s.yp - s.ypp) * h2y;
+      return b > T(0) ? m : p;
+    };
+    auto lo1x = [&](const Nbr<T>& s) {
+      return a > T(0) ? (s.c - s.xm) * idx : (s.xp - s.c) * idx;
+    };
+    auto lo1y = [&](const Nbr<T>& s) {
+      return b > T(0) ? (s.c - s.ym) * idy : (s.yp - s.c) * idy;
+    };
+    auto lap = [&](const Nbr<T>& s) {
+      return (s.xp - T(2) * s.c + s.xm) / (dx * dx) +
+             (s.yp - T(2) * s.c + s.ym) / (dy * dy);
+    };
+    {
+      T lx = lo1x(U), ly = lo1y(U);
+      T ux = (hi2x(U) + lx) - lx, uy = (hi2y(U) + ly) - ly;
+      f[0] = a * ux + b * uy - mu * lap(U) + (Pn.xp - Pn.xm) * h2x;
+      lx = lo1x(V);
+      ly = lo1y(V);
+      T vx = (hi2x(V) + lx) - lx, vy = (hi2y(V) + ly) - ly;
+      f[1] = a * vx + b * vy - mu * lap(V) + (Pn.yp - Pn.ym) * h2y;
+    }
+    const T gdiag = mu * (T(2) / (dx * dx) + T(2) / (dy * dy));
+    // Rhie-Chow coefficient g from frozen velocities (roll: periodic index,
+    // only used where the face flux is kept).
+    auto gat = [&](int di, int dj) {
+      return fabs(q(0, di, dj)) * idx + fabs(q(1, di, dj)) * idy + gdiag;
+    };
+    const T g = fabs(a) * idx + fabs(b) * idy + gdiag;
+    const bool wall_keep = (P.bc == 2);
+    T kxm = 0, kxp = 0, kym = 0, kyp = 0;
+    if (P.rhiechow) {
+      const T ig = T(1) / g;
+      T qxm = 0, qxp = 0, qym = 0, qyp = 0;
+      const T cx = ig * (Pn.xp - Pn.xm) * h2x * T(0.5);
+      const T cy = ig * (Pn.yp - Pn.ym) * h2y * T(0.5);
+      if (i != 0) {
+        const T igm = T(1) / gat(-1, 0);
+        kxm = (igm + ig) * T(0.5) * idx;
+        qxm = (U.c + U.xm) * T(0.5) + (igm * (Pn.c - Pn.xmm) * h2x) * T(0.5) +
+              cx - (igm + ig) * T(0.5) * (Pn.c - Pn.xm) * idx;
+      } else if (wall_keep) {
+        qxm = (U.c + U.xm) * T(0.5);
+      }
+      if (i != nx - 1) {
+        const T igp = T(1) / gat(1, 0);
+        kxp = (igp + ig) * T(0.5) * idx;
+        qxp = (U.c + U.xp) * T(0.5) + (igp * (Pn.xpp - Pn.c) * h2x) * T(0.5) +
+              cx - (igp + ig) * T(0.5) * (Pn.xp - Pn.c) * idx;
+      } else if (wall_keep) {
+        qxp = (U.c + U.xp) * T(0.5);
+      }
+      if (j != 0) {
+        const T igm = T(1) / gat(0, -1);
+        kym = (igm + ig) * T(0.5) * idy;
+        qym = (V.c + V.ym) * T(0.5) + (igm * (Pn.c - Pn.ymm) * h2y) * T(0.5) +
+              cy - (igm + ig) * T(0.5) * (Pn.c - Pn.ym) * idy;
+      } else if (wall_keep) {
+        qym = (V.c + V.ym) * T(0.5);
+      }
+      if (j != ny - 1) {
+        const T igp = T(1) / gat(0, 1);
+        kyp = (igp + ig) * T(0.5) * idy;
+        qyp = (V.c + V.yp) * T(0.5) + (igp * (Pn.ypp - Pn.c) * h2y) * T(0.5) +
+              cy - (igp + ig) * T(0.5) * (Pn.yp - Pn.c) * idy;
+      } else if (wall_keep) {
+        qyp = (V.c + V.yp) * T(0.5);
+      }
+      f[2] = (qxp - qxm) * idx + (qyp - qym) * idy;
+    } else {
+      f[2] = (U.xp - U.xm) * h2x + (V.yp - V.ym) * h2y;
+    }
+    if (!want_jac) return;
+
+    // Live Jacobian on effective neighbours, then folded onto raw values.
+    T e[5];
+    const T lxx = mu / (dx * dx), lyy = mu / (dy * dy);
+    // fu <- u and fv <- v share the operator uf d/dx + vf d/dy - mu lap.
+    e[0] = T(2) * lxx + T(2) * lyy;
+    e[1] = -lxx;
+    e[2] = -lxx;
+    e[3] = -lyy;
+    e[4] = -lyy;
+    if (a > T(0)) {
+      e[0] += a * idx;
+      e[1] -= a * idx;
+    } else {
+      e[2] += a * idx;
+      e[0] -= a * idx;
+    }
+    if (b > T(0)) {
+      e[0] += b * idy;
+      e[3] -= b * idy;
+    } else {
+      e[4] += b * idy;
+      e[0] -= b * idy;
+    }
+    for (int k = 0; k < 3; ++k)
+      for (int s = 0; s < 5; ++s) J[0][k][s] = J[1][k][s] = 0;
+    fold(J[0][0], e, i, j, nx, ny, ru.kx, ru.ky);
+    fold(J[1][1], e, i, j, nx, ny, rv.kx, rv.ky);
+    // fu <- p: (p_xp - p_xm) / (2 dx); fv <- p along y.
+    {
+      T ep[5] = {0, -h2x, h2x, 0, 0};
+      fold(J[0][2], ep, i, j, nx, ny, LINEAR, LINEAR);
+      T eq[5] = {0, 0, 0, -h2y, h2y};
+      fold(J[1][2], eq, i, j, nx, ny, LINEAR, LINEAR);
+    }
+    // fp <- u, v, p.
+    {
+      T eu[5] = {0, 0, 0, 0, 0}, ev[5] = {0, 0, 0, 0, 0},
+        ep[5] = {0, 0, 0, 0, 0};
+      const T hx = T(0.5) * idx, hy = T(0.5) * idy;
+      if (P.rhiechow) {
+        if (i != 0 || wall_keep) {
+          eu[0] -= hx;
+          eu[1] -= hx;
+        }
+        if (i != nx - 1 || wall_keep) {
+          eu[0] += hx;
+          eu[2] += hx;
+        }
+        if (j != 0 || wall_keep) {
+          ev[0] -= hy;
+          ev[3] -= hy;
+        }
+        if (j != ny - 1 || wall_keep) {
+          ev[0] += hy;
+          ev[4] += hy;
+        }
+        // Pressure part (inner faces only; kxm etc. are zero at walls).
+        ep[0] = (kxm + kxp) * idx + (kym + kyp) * idy;
+        ep[1] = -kxm * idx;
+        ep[2] = -kxp * idx;
+        ep[3] = -kym * idy;
+        ep[4] = -kyp * idy;
+      } else {
+        eu[1] = -h2x;
+        eu[2] = h2x;
+        ev[3] = -h2y;
+        ev[4] = h2y;
+      }
+      fold(J[2][0], eu, i, j, nx, ny, ru.kx, ru.ky);
+      fold(J[2][1], ev, i, j, nx, ny, rv.kx, rv.ky);
+      fold(J[2][2], ep, i, j, nx, ny, LINEAR, LINEAR);
+    }
+  }
+
+  template <class Acc>
+  __device__ static inline void residual(const Acc& q, int i, int j, int nx,
+                                         int ny, const Params& P, T* f) {
+    compute(q, i, j, nx, ny, P, f, nullptr, false);
+  }
+  template <class Acc>
+  __device__ static inline void eval(const Acc& q, int i, int j, int nx,
+                                     int ny, const Params& P, T* f,
+                                     T (*J)[NF][NS]) {
+    compute(q, i, j, nx, ny, P, f, J, true);
+  }
+};
+
+}  // namespace odil
+
+using namespace odil;
+
+namespace {
+template <class T>
+int cavity_dispatch_grad(const void* const* fields, void* const* grad,
+                         double* partial, int nx, int ny,
+                         const CavityParams& P, hipStream_t s, int* nb) {
+  const T scale = T(2.0 / ((double)nx * ny));
+  if constexpr (sizeof(T) == 4)
+    return launch_loss_grad_t<T, CavityOp<T>, 16, 64>(
+        fields, grad, partial, nx, ny, scale, P, s, nb);
+  else
+    return launch_loss_grad_t<T, CavityOp<T>, 8, 64>(
+        fields, grad, partial, nx, ny, scale, P, s, nb);
+}
+}  // namespace
+
+extern "C" {
+
+// Residuals fu, fv, fp of the cavity at state (u, v, p); `res` may be NULL
+// (sums of squares only). `partial` receives per-block sums of squares
+// (3 x nblocks doubles, see odil_cavity_nblocks).
+int odil_cavity_residual(int dtype, int nx, int ny, double dx, double dy,
+                         double mu, int bc, int rhiechow,
+                         const void* const* fields, void* const* res,
+                         double* partial, void* stream) {
+  if (nx < 3 || ny < 3 || !fields) {
+    set_error("cavity_residual: need nx, ny >= 3 and fields");
+    return kErrArg;
+  }
+  CavityParams P{dx, dy, mu, bc, rhiechow};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    return launch_residual<double, CavityOp<double>>(fields, res, partial, nx,
+                                                     ny, P, s, nullptr);
+  if (dtype == kF32)
+    return launch_residual<float, CavityOp<float>>(fields, res, partial, nx,
+                                                   ny, P, s, nullptr);
+  set_error("cavity_residual: bad dtype %d", dtype);
+  return kErrArg;
+}
+
+// Jacobian coefficient fields, 27 entries in the order of the
+// odil_cavity_jac_entries table, each a full (nx, ny) field.
+int odil_cavity_jac(int dtype, int nx, int ny, double dx, double dy,
+                    double mu, int bc, int rhiechow, const void* const* fields,
+                    void* coef, void* const* res, void* stream) {
+  if (nx < 3 || ny < 3 || !fields || !coef) {
+    set_error("cavity_jac: need nx, ny >= 3, fields and coef");
+    return kErrArg;
+  }
+  CavityParams P{dx, dy, mu, bc, rhiechow};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    return launch_jac<double, CavityOp<double>>(fields, coef, res, nx, ny, P,
+                                                s);
+  if (dtype == kF32)
+    return launch_jac<float, CavityOp<float>>(fields, coef, res, nx, ny, P, s);
+  set_error("cavity_jac: bad dtype %d", dtype);
+  return kErrArg;
+}
+
+// Table of the Jacobian entries: for q < 27, ent[3q..3q+2] =
+// (equation, field, shift index); shifts[2s..2s+1] = (dx, dy).
+int odil_cavity_jac_entries(int* ent, int* shifts) {
+  for (int q = 0; q < CavityOp<double>::NJ; ++q) {
+    ent[3 * q] = CavityOp<double>::JE[q];
+    ent[3 * q + 1] = CavityOp<double>::JV[q];
+    ent[3 * q + 2] = CavityOp<double>::JS[q];
+  }
+  for (int s = 0; s < CavityOp<double>::NS; ++s) {
+    shifts[2 * s] = CavityOp<double>::SX[s];
+    shifts[2 * s + 1] = CavityOp<double>::SY[s];
+  }
+  return CavityOp<double>::NJ;
+}
+
+// Fused loss and adjoint: grad_k = (2/N) sum_e J_e,k^T f_e for (u, v, p);
+// `partial` receives per-block sums of f_e^2 (3 x nblocks doubles).
+int odil_cavity_loss_grad(int dtype, int nx, int ny, double dx, double dy,
+                          double mu, int bc, int rhiechow,
+                          const void* const* fields, void* const* grad,
+                          double* partial, int* nblocks, void* stream) {
+  if (nx < 3 || ny < 3 || !fields || !grad) {
+    set_error("cavity_loss_grad: need nx, ny >= 3, fields and grad");
+    return kErrArg;
+  }
+  CavityParams P{dx, dy, mu, bc, rhiechow};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    return cavity_dispatch_grad<double>(fields, grad, partial, nx, ny, P, s,
+                                        nblocks);
+  if (dtype == kF32)
+    return cavity_dispatch_grad<float>(fields, grad, partial, nx, ny, P, s,
+                                       nblocks);
+  set_error("cavity_loss_grad: bad dtype %d", dtype);
+  return kErrArg;
+}
+
+// Number of per-block partials written by the residual (kind 0) and
+// loss-grad (kind 1) kernels of a 2D operator for the given dtype.
+int odil_cavity_nblocks(int kind, int dtype, int nx, int ny) {
+  if (kind == 0) return ceil_div(ny, 64) * ceil_div(nx, 4);
+  const int tx = dtype == kF32 ? 16 : 8;
+  return ceil_div(ny, 64) * ceil_div(nx, tx);
+}
+
+}  // extern "C"

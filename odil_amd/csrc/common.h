@@ -1,0 +1,78 @@
+// Shared definitions for the ODIL HIP kernels (gfx950).
+//
+// Grids are cell-centred and stored C-order: a 2D field of shape (n0, n1) has
+// element (i, j) at i*n1 + j, so the last axis is contiguous in HBM. Fields of
+// one problem are separate arrays (the packed state is their concatenation,
+// field-major), matching the flat layout of the reference's pack_state.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+namespace odil {
+
+enum Dtype : int { kF32 = 0, kF64 = 1 };
+
+// Status codes returned across the C ABI. Nothing throws across it.
+enum Status : int {
+  kOk = 0,
+  kErrArg = 1,      // invalid argument (shape, dtype, null pointer)
+  kErrLaunch = 2,   // kernel launch failed
+  kErrCapacity = 3  // size above a documented maximum
+};
+
+void set_error(const char* fmt, ...);
+int check_launch(const char* what);
+
+// Wave-level sum over 64 lanes (CDNA wavefront) using DPP-capable shuffles.
+template <class T>
+__device__ inline T wave_sum(T v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-level sum: wavefront reduction, then across waves through LDS.
+// `buf` must hold one element per wave. Result valid in thread 0.
+template <class T>
+__device__ inline T block_sum(T v, T* buf) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (lane == 0) buf[wid] = v;
+  __syncthreads();
+  T s = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < nw; ++w) s += buf[w];
+  return s;
+}
+
+__host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// Halo rule at a wall: the value outside the domain is extrapolated from the
+// nearest inner value `near`, the next one `far` and the wall value `w`.
+//   QUADH:  quadratic through far, near and the wall half-point,
+//           (far - 6 near + 8 w) / 3                       (util_op.py:1017)
+//   LINEAR: 2 near - far                                    (util_op.py:1032)
+enum Extrap : int { QUADH = 0, LINEAR = 1 };
+
+template <class T>
+__host__ __device__ inline T ghost(int kind, T far, T near, T w) {
+  return kind == QUADH ? (far - T(6) * near + T(8) * w) / T(3)
+                       : T(2) * near - far;
+}
+// Derivatives of ghost() with respect to `far` and `near`.
+template <class T>
+__host__ __device__ inline T ghost_dfar(int kind) {
+  return kind == QUADH ? T(1) / T(3) : T(-1);
+}
+template <class T>
+__host__ __device__ inline T ghost_dnear(int kind) {
+  return kind == QUADH ? T(-2) : T(2);
+}
+
+}  // namespace odil

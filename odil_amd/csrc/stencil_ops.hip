@@ -1,0 +1,611 @@
+// Block stencil operators on structured grids and the geometric-multigrid
+// building blocks (reference: SparseOperator / MultigridOp, multigrid.py:
+// 342-602, 877-1214).
+//
+// A block stencil operator A maps nvar_in input fields on a grid of nw points
+// (2D or 3D, C-order) to nvar_out output fields on nwh points
+// (nwh = (nw - 1) / stride + 1):
+//     (A x)_o[I] = sum_{e : out(e) = o} coef_e[I] * x_{in(e)}[stride*I + s_e]
+// with zero fill where stride*I + s_e leaves the grid (non-circular shift,
+// multigrid.py:941-947). Coefficients are one contiguous array
+// coef[nent][nwh]; the entry table is int32 [nent][5] = (out, in, s0, s1, s2)
+// sorted by `out`, with ostart[nvar_out + 1] delimiting each output field.
+#include "common.h"
+
+namespace odil {
+
+constexpr int kMaxVar = 4;
+
+template <class T>
+struct VecPtrs {
+  T* p[kMaxVar];
+};
+template <class T>
+struct CVecPtrs {
+  const T* p[kMaxVar];
+};
+
+struct Grid3 {
+  int n0, n1, n2;
+  __host__ __device__ size_t size() const { return (size_t)n0 * n1 * n2; }
+};
+
+__device__ inline void unflat(size_t c, const Grid3& g, int& i0, int& i1,
+                              int& i2) {
+  i2 = (int)(c % g.n2);
+  size_t r = c / g.n2;
+  i1 = (int)(r % g.n1);
+  i0 = (int)(r / g.n1);
+}
+
+__device__ inline bool inside(int a, int b, int c, const Grid3& g) {
+  return a >= 0 && a < g.n0 && b >= 0 && b < g.n1 && c >= 0 && c < g.n2;
+}
+
+// mode 0: y = A x; mode 1: y = f - A x;
+// mode 2: weighted Jacobi y = x + omega (f - A x) / diag(A)
+//         (smoother_jacobi, multigrid.py:543-552; diag = entry dent[o]).
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_apply(const int* __restrict__ ent, const int* __restrict__ ostart,
+            const int* __restrict__ dent, int nout, const T* __restrict__ coef,
+            CVecPtrs<T> x, VecPtrs<T> y, CVecPtrs<T> f, Grid3 nw, Grid3 nwh,
+            int stride, int mode, T omega) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t Nh = nwh.size();
+  if (c >= Nh) return;
+  int i0, i1, i2;
+  unflat(c, nwh, i0, i1, i2);
+  const int b0 = stride * i0, b1 = stride * i1, b2 = stride * i2;
+  for (int o = 0; o < nout; ++o) {
+    T acc = 0;
+    const int e1 = ostart[o + 1];
+    for (int e = ostart[o]; e < e1; ++e) {
+      const int k = ent[5 * e + 1];
+      const int p0 = b0 + ent[5 * e + 2], p1 = b1 + ent[5 * e + 3],
+                p2 = b2 + ent[5 * e + 4];
+      if (inside(p0, p1, p2, nw))
+        acc += coef[(size_t)e * Nh + c] *
+               x.p[k][((size_t)p0 * nw.n1 + p1) * nw.n2 + p2];
+    }
+    if (mode == 0) {
+      y.p[o][c] = acc;
+    } else if (mode == 1) {
+      y.p[o][c] = f.p[o][c] - acc;
+    } else {
+      const T d = coef[(size_t)dent[o] * Nh + c];
+      y.p[o][c] = x.p[o][c] + omega * (f.p[o][c] - acc) / d;
+    }
+  }
+}
+
+// Transpose: y_k[p] = sum_{e: in(e)=k} sum_{I: stride*I + s_e = p}
+//                      coef_e[I] x_{out(e)}[I].
+// `tent` lists entry indices grouped by input field, delimited by istart.
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_apply_t(const int* __restrict__ ent, const int* __restrict__ tent,
+              const int* __restrict__ istart, int nin,
+              const T* __restrict__ coef, CVecPtrs<T> x, VecPtrs<T> y,
+              Grid3 nw, Grid3 nwh, int stride, int accumulate) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nw.size()) return;
+  int p0, p1, p2;
+  unflat(p, nw, p0, p1, p2);
+  const size_t Nh = nwh.size();
+  for (int k = 0; k < nin; ++k) {
+    T acc = 0;
+    for (int t = istart[k]; t < istart[k + 1]; ++t) {
+      const int e = tent[t];
+      const int o = ent[5 * e];
+      int q0 = p0 - ent[5 * e + 2], q1 = p1 - ent[5 * e + 3],
+          q2 = p2 - ent[5 * e + 4];
+      if (stride == 2) {
+        if ((q0 | q1 | q2) & 1) continue;
+        q0 >>= 1;
+        q1 >>= 1;
+        q2 >>= 1;
+      }
+      if (!inside(q0, q1, q2, nwh)) continue;
+      const size_t I = ((size_t)q0 * nwh.n1 + q1) * nwh.n2 + q2;
+      acc += coef[(size_t)e * Nh + I] * x.p[o][I];
+    }
+    if (accumulate)
+      y.p[k][p] += acc;
+    else
+      y.p[k][p] = acc;
+  }
+}
+
+// Normal operator N = sum_e J_e^T J_e in stencil form
+// (mul_transpose_op + add_elementwise, multigrid.py:1035-1071,1138-1184):
+//   N_o[m] = sum_{terms (a, b, sa)} J_a[m - sa] * J_b[m - sa]
+// where the output entry o couples (var(a), var(b)) at shift s_b - s_a.
+// Terms are int32 [nterm][5] = (a, b, sa0, sa1, sa2) grouped by tstart.
+template <class Tin, class Tout>
+__global__ void __launch_bounds__(256)
+    k_normal(const int* __restrict__ terms, const int* __restrict__ tstart,
+             int nout, const Tin* __restrict__ jc, Tout* __restrict__ out,
+             Grid3 g) {
+  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = g.size();
+  if (m >= N) return;
+  int m0, m1, m2;
+  unflat(m, g, m0, m1, m2);
+  for (int o = 0; o < nout; ++o) {
+    double acc = 0;
+    for (int t = tstart[o]; t < tstart[o + 1]; ++t) {
+      const int q0 = m0 - terms[5 * t + 2], q1 = m1 - terms[5 * t + 3],
+                q2 = m2 - terms[5 * t + 4];
+      if (!inside(q0, q1, q2, g)) continue;
+      const size_t n = ((size_t)q0 * g.n1 + q1) * g.n2 + q2;
+      acc += (double)jc[(size_t)terms[5 * t] * N + n] *
+             (double)jc[(size_t)terms[5 * t + 1] * N + n];
+    }
+    out[(size_t)o * N + m] = (Tout)acc;
+  }
+}
+
+// Galerkin coarse operator on node-based grids nw = 2 nwh - 1.
+// sigma = +1: A_c = R A T. sigma = -1: the arithmetic of the reference's
+// coarsen_sparse_operator (multigrid.py:390-427), which couples fine node a
+// to a - s_e and coarse node I to I - dc, i.e. reflect(R reflect(A) T).
+// Output entry o = (i, j, dc); terms int32 [nterm][5] = (e, r0, r1, r2, wt_id)
+// with fine entry e of block (i, j), restriction offset r, interpolation
+// weight wtab[wt_id] = 0.5^|t| for t = r + sigma (s_e - 2 dc).
+// Restriction weights: inner coarse nodes use rw[(r0+1)*9+(r1+1)*3+(r2+1)],
+// coarse boundary nodes inject (r = 0, weight 1) (multigrid.py:525-532).
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_galerkin(const int* __restrict__ fent, const int* __restrict__ oent,
+               const int* __restrict__ terms, const int* __restrict__ tstart,
+               int nout, const double* __restrict__ rw,
+               const double* __restrict__ wtab, const T* __restrict__ fc,
+               T* __restrict__ cc, Grid3 nw, Grid3 nwh, int dim, int sigma) {
+  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t Nc = nwh.size();
+  if (I >= Nc) return;
+  int I0, I1, I2;
+  unflat(I, nwh, I0, I1, I2);
+  const bool inner = (I0 > 0 && I0 < nwh.n0 - 1) &&
+                     (I1 > 0 && I1 < nwh.n1 - 1) &&
+                     (dim < 3 || (I2 > 0 && I2 < nwh.n2 - 1));
+  const size_t Nf = nw.size();
+  for (int o = 0; o < nout; ++o) {
+    const int J0 = I0 + sigma * oent[5 * o + 2],
+              J1 = I1 + sigma * oent[5 * o + 3],
+              J2 = I2 + sigma * oent[5 * o + 4];
+    double acc = 0;
+    if (inside(J0, J1, J2, nwh)) {
+      for (int t = tstart[o]; t < tstart[o + 1]; ++t) {
+        const int e = terms[5 * t];
+        const int r0 = terms[5 * t + 1], r1 = terms[5 * t + 2],
+                  r2 = terms[5 * t + 3];
+        double w;
+        if (inner)
+          w = rw[(r0 + 1) * 9 + (r1 + 1) * 3 + (r2 + 1)];
+        else
+          w = (r0 == 0 && r1 == 0 && r2 == 0) ? 1.0 : 0.0;
+        if (w == 0.0) continue;
+        const int a0 = 2 * I0 + r0, a1 = 2 * I1 + r1, a2 = 2 * I2 + r2;
+        if (!inside(a0, a1, a2, nw)) continue;
+        const int b0 = a0 + sigma * fent[5 * e + 2],
+                  b1 = a1 + sigma * fent[5 * e + 3],
+                  b2 = a2 + sigma * fent[5 * e + 4];
+        if (!inside(b0, b1, b2, nw)) continue;
+        const size_t a = ((size_t)a0 * nw.n1 + a1) * nw.n2 + a2;
+        acc += w * wtab[terms[5 * t + 4]] * (double)fc[(size_t)e * Nf + a];
+      }
+    }
+    cc[(size_t)o * Nc + I] = (T)acc;
+  }
+}
+
+// Restriction of nvar fields (R.mul_field, multigrid.py:496-534).
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_restrict(CVecPtrs<T> fine, VecPtrs<T> coarse, int nvar,
+               const double* __restrict__ rw, Grid3 nw, Grid3 nwh, int dim) {
+  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nwh.size()) return;
+  int I0, I1, I2;
+  unflat(I, nwh, I0, I1, I2);
+  const bool inner = (I0 > 0 && I0 < nwh.n0 - 1) &&
+                     (I1 > 0 && I1 < nwh.n1 - 1) &&
+                     (dim < 3 || (I2 > 0 && I2 < nwh.n2 - 1));
+  const int c0 = 2 * I0, c1 = 2 * I1, c2 = 2 * I2;
+  for (int k = 0; k < nvar; ++k) {
+    const T* f = fine.p[k];
+    T acc;
+    if (!inner) {
+      acc = f[((size_t)c0 * nw.n1 + c1) * nw.n2 + c2];
+    } else {
+      acc = 0;
+      const int r2lo = dim < 3 ? 0 : -1, r2hi = dim < 3 ? 0 : 1;
+      for (int r0 = -1; r0 <= 1; ++r0)
+        for (int r1 = -1; r1 <= 1; ++r1)
+          for (int r2 = r2lo; r2 <= r2hi; ++r2) {
+            const double w = rw[(r0 + 1) * 9 + (r1 + 1) * 3 + (r2 + 1)];
+            if (w != 0.0)
+              acc += T(w) *
+                     f[((size_t)(c0 + r0) * nw.n1 + (c1 + r1)) * nw.n2 +
+                       (c2 + r2)];
+          }
+    }
+    coarse.p[k][I] = acc;
+  }
+}
+
+// Prolongation with add: u_fine += T e_coarse, multilinear interpolation on
+// node-based grids (TT.mul_transpose_field, multigrid.py:471-494).
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_prolong_add(CVecPtrs<T> coarse, VecPtrs<T> fine, int nvar, Grid3 nw,
+                  Grid3 nwh) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nw.size()) return;
+  int p0, p1, p2;
+  unflat(p, nw, p0, p1, p2);
+  const int a0 = p0 >> 1, a1 = p1 >> 1, a2 = p2 >> 1;
+  const int o0 = p0 & 1, o1 = p1 & 1, o2 = p2 & 1;
+  const T w = T(1) / T(1 << (o0 + o1 + o2));
+  for (int k = 0; k < nvar; ++k) {
+    const T* e = coarse.p[k];
+    T acc = 0;
+    for (int d0 = 0; d0 <= o0; ++d0)
+      for (int d1 = 0; d1 <= o1; ++d1)
+        for (int d2 = 0; d2 <= o2; ++d2)
+          acc += e[((size_t)(a0 + d0) * nwh.n1 + (a1 + d1)) * nwh.n2 +
+                   (a2 + d2)];
+    fine.p[k][p] += w * acc;
+  }
+}
+
+// Dense matrix of a square block stencil operator (stride 1), row-major,
+// n = nvar * N; used for the coarsest-level direct solve.
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_to_dense(const int* __restrict__ ent, int nent, int nvar,
+               const T* __restrict__ coef, double* __restrict__ M, Grid3 g) {
+  const size_t N = g.size();
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * nent) return;
+  const int e = (int)(t / N);
+  const size_t c = t - (size_t)e * N;
+  int c0, c1, c2;
+  unflat(c, g, c0, c1, c2);
+  const int q0 = c0 + ent[5 * e + 2], q1 = c1 + ent[5 * e + 3],
+            q2 = c2 + ent[5 * e + 4];
+  if (!inside(q0, q1, q2, g)) return;
+  const size_t col = (size_t)ent[5 * e + 1] * N +
+                     ((size_t)q0 * g.n1 + q1) * g.n2 + q2;
+  const size_t row = (size_t)ent[5 * e] * N + c;
+  // Entries of one operator have distinct (out, in, shift), so every matrix
+  // element is written at most once.
+  M[row * (nvar * N) + col] = (double)coef[(size_t)e * N + c];
+}
+
+// y = M x for a dense row-major FP64 matrix, one wavefront per row; x and y
+// may be FP32 or FP64 (the coarse solve stays FP64 inside).
+template <class Tx, class Ty>
+__global__ void __launch_bounds__(256)
+    k_gemv(const double* __restrict__ M, CVecPtrs<Tx> x, VecPtrs<Ty> y,
+           int nvar, int N) {
+  const int n = nvar * N;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double acc = 0;
+  const double* m = M + (size_t)row * n;
+  for (int c = lane; c < n; c += 64) {
+    const int k = c / N;
+    acc += m[c] * (double)x.p[k][c - k * N];
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) {
+    const int k = row / N;
+    y.p[k][row - k * N] = (Ty)acc;
+  }
+}
+
+}  // namespace odil
+
+using namespace odil;
+
+namespace {
+
+Grid3 mk_grid(int ndim, const int* n) {
+  Grid3 g{1, 1, 1};
+  if (ndim == 2) {
+    g.n0 = n[0];
+    g.n1 = n[1];
+  } else {
+    g.n0 = n[0];
+    g.n1 = n[1];
+    g.n2 = n[2];
+  }
+  return g;
+}
+
+template <class T>
+VecPtrs<T> vp(void* const* a, int n) {
+  VecPtrs<T> v{};
+  for (int k = 0; k < n && a; ++k) v.p[k] = static_cast<T*>(a[k]);
+  return v;
+}
+template <class T>
+CVecPtrs<T> cvp(const void* const* a, int n) {
+  CVecPtrs<T> v{};
+  for (int k = 0; k < n && a; ++k) v.p[k] = static_cast<const T*>(a[k]);
+  return v;
+}
+
+bool bad_var(int n, const char* what) {
+  if (n < 1 || n > kMaxVar) {
+    set_error("%s: number of fields %d outside [1, %d]", what, n, kMaxVar);
+    return true;
+  }
+  return false;
+}
+
+bool bad_dim(int ndim, const char* what) {
+  if (ndim != 2 && ndim != 3) {
+    set_error("%s: ndim must be 2 or 3, got %d", what, ndim);
+    return true;
+  }
+  return false;
+}
+
+unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+extern "C" {
+
+// y = A x (mode 0), y = f - A x (mode 1), or one weighted-Jacobi sweep
+// y = x + omega (f - A x) / diag (mode 2, stride 1, nvar_in == nvar_out).
+// `ent`, `ostart`, `dent` are device int32 arrays (see file comment).
+int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
+                       int nvar_in, int nvar_out, const int* ent,
+                       const int* ostart, const int* dent, const void* coef,
+                       const void* const* x, void* const* y,
+                       const void* const* f, int mode, double omega,
+                       void* stream) {
+  if (bad_dim(ndim, "stencil_apply") || bad_var(nvar_in, "stencil_apply") ||
+      bad_var(nvar_out, "stencil_apply"))
+    return kErrArg;
+  if (stride != 1 && stride != 2) {
+    set_error("stencil_apply: stride must be 1 or 2");
+    return kErrArg;
+  }
+  if (mode == 2 && (stride != 1 || nvar_in != nvar_out || !dent)) {
+    set_error("stencil_apply: Jacobi needs stride 1, square blocks, diag");
+    return kErrArg;
+  }
+  if ((mode == 1 || mode == 2) && !f) {
+    set_error("stencil_apply: mode %d needs f", mode);
+    return kErrArg;
+  }
+  Grid3 g = mk_grid(ndim, nw);
+  Grid3 gh{(g.n0 - 1) / stride + 1, (g.n1 - 1) / stride + 1,
+           ndim == 3 ? (g.n2 - 1) / stride + 1 : 1};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_apply<double>, dim3(nblk(gh.size())), dim3(256), 0,
+                       s, ent, ostart, dent, nvar_out,
+                       static_cast<const double*>(coef),
+                       cvp<double>(x, nvar_in), vp<double>(y, nvar_out),
+                       cvp<double>(f, nvar_out), g, gh, stride, mode, omega);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_apply<float>, dim3(nblk(gh.size())), dim3(256), 0,
+                       s, ent, ostart, dent, nvar_out,
+                       static_cast<const float*>(coef), cvp<float>(x, nvar_in),
+                       vp<float>(y, nvar_out), cvp<float>(f, nvar_out), g, gh,
+                       stride, mode, (float)omega);
+  else {
+    set_error("stencil_apply: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  return check_launch("stencil_apply");
+}
+
+// y = A^T x (accumulate != 0: y += A^T x). `tent`/`istart` group entry
+// indices by input field.
+int odil_stencil_apply_t(int dtype, int ndim, const int* nw, int stride,
+                         int nvar_in, int nvar_out, const int* ent,
+                         const int* tent, const int* istart, const void* coef,
+                         const void* const* x, void* const* y, int accumulate,
+                         void* stream) {
+  if (bad_dim(ndim, "stencil_apply_t") ||
+      bad_var(nvar_in, "stencil_apply_t") ||
+      bad_var(nvar_out, "stencil_apply_t"))
+    return kErrArg;
+  if (stride != 1 && stride != 2) {
+    set_error("stencil_apply_t: stride must be 1 or 2");
+    return kErrArg;
+  }
+  Grid3 g = mk_grid(ndim, nw);
+  Grid3 gh{(g.n0 - 1) / stride + 1, (g.n1 - 1) / stride + 1,
+           ndim == 3 ? (g.n2 - 1) / stride + 1 : 1};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_apply_t<double>, dim3(nblk(g.size())), dim3(256), 0,
+                       s, ent, tent, istart, nvar_in,
+                       static_cast<const double*>(coef),
+                       cvp<double>(x, nvar_out), vp<double>(y, nvar_in), g, gh,
+                       stride, accumulate);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_apply_t<float>, dim3(nblk(g.size())), dim3(256), 0, s,
+                       ent, tent, istart, nvar_in,
+                       static_cast<const float*>(coef),
+                       cvp<float>(x, nvar_out), vp<float>(y, nvar_in), g, gh,
+                       stride, accumulate);
+  else {
+    set_error("stencil_apply_t: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  return check_launch("stencil_apply_t");
+}
+
+// Normal operator sum_e J_e^T J_e from Jacobian coefficients `jc`
+// (dtype_in) into `out` (dtype_out), accumulated in FP64.
+int odil_stencil_normal(int dtype_in, int dtype_out, int ndim, const int* nw,
+                        const int* terms, const int* tstart, int nout,
+                        const void* jc, void* out, void* stream) {
+  if (bad_dim(ndim, "stencil_normal")) return kErrArg;
+  Grid3 g = mk_grid(ndim, nw);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  dim3 grid(nblk(g.size()));
+  if (dtype_in == kF64 && dtype_out == kF64)
+    hipLaunchKernelGGL((k_normal<double, double>), grid, dim3(256), 0, s,
+                       terms, tstart, nout, static_cast<const double*>(jc),
+                       static_cast<double*>(out), g);
+  else if (dtype_in == kF64 && dtype_out == kF32)
+    hipLaunchKernelGGL((k_normal<double, float>), grid, dim3(256), 0, s, terms,
+                       tstart, nout, static_cast<const double*>(jc),
+                       static_cast<float*>(out), g);
+  else if (dtype_in == kF32 && dtype_out == kF32)
+    hipLaunchKernelGGL((k_normal<float, float>), grid, dim3(256), 0, s, terms,
+                       tstart, nout, static_cast<const float*>(jc),
+                       static_cast<float*>(out), g);
+  else {
+    set_error("stencil_normal: unsupported dtype pair %d -> %d", dtype_in,
+              dtype_out);
+    return kErrArg;
+  }
+  return check_launch("stencil_normal");
+}
+
+// Galerkin coarse operator; `rw` (27 doubles) inner restriction weights,
+// `wtab` interpolation weights indexed by the terms; sigma = +1 exact
+// R A T, -1 the reference's stencil coarsening.
+int odil_stencil_galerkin(int dtype, int ndim, const int* nw, const int* fent,
+                          const int* oent, const int* terms,
+                          const int* tstart, int nout, const double* rw,
+                          const double* wtab, const void* fc, void* cc,
+                          int sigma, void* stream) {
+  if (sigma != 1 && sigma != -1) {
+    set_error("stencil_galerkin: sigma must be +1 or -1");
+    return kErrArg;
+  }
+  if (bad_dim(ndim, "stencil_galerkin")) return kErrArg;
+  Grid3 g = mk_grid(ndim, nw);
+  if (g.n0 % 2 == 0 || g.n1 % 2 == 0 || (ndim == 3 && g.n2 % 2 == 0)) {
+    set_error("stencil_galerkin: node grids need odd sizes");
+    return kErrArg;
+  }
+  Grid3 gh{g.n0 / 2 + 1, g.n1 / 2 + 1, ndim == 3 ? g.n2 / 2 + 1 : 1};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  dim3 grid(nblk(gh.size()));
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_galerkin<double>, grid, dim3(256), 0, s, fent, oent,
+                       terms, tstart, nout, rw, wtab,
+                       static_cast<const double*>(fc), static_cast<double*>(cc),
+                       g, gh, ndim, sigma);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_galerkin<float>, grid, dim3(256), 0, s, fent, oent,
+                       terms, tstart, nout, rw, wtab,
+                       static_cast<const float*>(fc), static_cast<float*>(cc),
+                       g, gh, ndim, sigma);
+  else {
+    set_error("stencil_galerkin: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  return check_launch("stencil_galerkin");
+}
+
+int odil_mg_restrict(int dtype, int ndim, const int* nw, int nvar,
+                     const double* rw, const void* const* fine,
+                     void* const* coarse, void* stream) {
+  if (bad_dim(ndim, "mg_restrict") || bad_var(nvar, "mg_restrict"))
+    return kErrArg;
+  Grid3 g = mk_grid(ndim, nw);
+  Grid3 gh{g.n0 / 2 + 1, g.n1 / 2 + 1, ndim == 3 ? g.n2 / 2 + 1 : 1};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_restrict<double>, dim3(nblk(gh.size())), dim3(256), 0,
+                       s, cvp<double>(fine, nvar), vp<double>(coarse, nvar),
+                       nvar, rw, g, gh, ndim);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_restrict<float>, dim3(nblk(gh.size())), dim3(256), 0,
+                       s, cvp<float>(fine, nvar), vp<float>(coarse, nvar),
+                       nvar, rw, g, gh, ndim);
+  else {
+    set_error("mg_restrict: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  return check_launch("mg_restrict");
+}
+
+int odil_mg_prolong_add(int dtype, int ndim, const int* nw, int nvar,
+                        const void* const* coarse, void* const* fine,
+                        void* stream) {
+  if (bad_dim(ndim, "mg_prolong_add") || bad_var(nvar, "mg_prolong_add"))
+    return kErrArg;
+  Grid3 g = mk_grid(ndim, nw);
+  if (g.n0 % 2 == 0 || g.n1 % 2 == 0 || (ndim == 3 && g.n2 % 2 == 0)) {
+    set_error("mg_prolong_add: node grids need odd sizes");
+    return kErrArg;
+  }
+  Grid3 gh{g.n0 / 2 + 1, g.n1 / 2 + 1, ndim == 3 ? g.n2 / 2 + 1 : 1};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_prolong_add<double>, dim3(nblk(g.size())), dim3(256),
+                       0, s, cvp<double>(coarse, nvar), vp<double>(fine, nvar),
+                       nvar, g, gh);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_prolong_add<float>, dim3(nblk(g.size())), dim3(256),
+                       0, s, cvp<float>(coarse, nvar), vp<float>(fine, nvar),
+                       nvar, g, gh);
+  else {
+    set_error("mg_prolong_add: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  return check_launch("mg_prolong_add");
+}
+
+// Dense FP64 matrix (zero-initialised by the caller) of a square stride-1
+// block stencil operator.
+int odil_stencil_to_dense(int dtype, int ndim, const int* nw, int nvar,
+                          const int* ent, int nent, const void* coef,
+                          double* M, void* stream) {
+  if (bad_dim(ndim, "stencil_to_dense") || bad_var(nvar, "stencil_to_dense"))
+    return kErrArg;
+  Grid3 g = mk_grid(ndim, nw);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  dim3 grid(nblk(g.size() * nent));
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_to_dense<double>, grid, dim3(256), 0, s, ent, nent,
+                       nvar, static_cast<const double*>(coef), M, g);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_to_dense<float>, grid, dim3(256), 0, s, ent, nent,
+                       nvar, static_cast<const float*>(coef), M, g);
+  else {
+    set_error("stencil_to_dense: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  return check_launch("stencil_to_dense");
+}
+
+// y = M x with M dense FP64 (nvar*N square); x, y split into nvar fields of
+// N values each, in dtype_x / dtype_y.
+int odil_dense_gemv(int dtype_x, int dtype_y, int nvar, int N,
+                    const double* M, const void* const* x, void* const* y,
+                    void* stream) {
+  if (bad_var(nvar, "dense_gemv")) return kErrArg;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  dim3 grid(ceil_div(nvar * N, 4));
+  if (dtype_x == kF64 && dtype_y == kF64)
+    hipLaunchKernelGGL((k_gemv<double, double>), grid, dim3(256), 0, s, M,
+                       cvp<double>(x, nvar), vp<double>(y, nvar), nvar, N);
+  else if (dtype_x == kF32 && dtype_y == kF32)
+    hipLaunchKernelGGL((k_gemv<float, float>), grid, dim3(256), 0, s, M,
+                       cvp<float>(x, nvar), vp<float>(y, nvar), nvar, N);
+  else {
+    set_error("dense_gemv: unsupported dtypes");
+    return kErrArg;
+  }
+  return check_launch("dense_gemv");
+}
+
+}  // extern "C"

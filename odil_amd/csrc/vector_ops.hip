@@ -1,0 +1,187 @@
+// Vector kernels for the Krylov and Newton drivers: deterministic dot
+// products and axpy forms whose scalar is a ratio of device values, so a
+// whole solver iteration can run without host round trips (and be captured
+// in a HIP graph).
+#include "common.h"
+
+namespace odil {
+
+constexpr int kMaxVec = 4;
+
+template <class T>
+struct VP {
+  T* p[kMaxVec];
+};
+template <class T>
+struct CVP {
+  const T* p[kMaxVec];
+};
+
+constexpr int kDotBlocks = 1024;
+
+// Stage 1: per-block partial sums of sum_k x_k . y_k in FP64.
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_dot_partial(CVP<T> x, CVP<T> y, int nvar, size_t N,
+                  double* __restrict__ partial) {
+  __shared__ double red[4];
+  double s = 0;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (int k = 0; k < nvar; ++k) {
+    const T* a = x.p[k];
+    const T* b = y.p[k];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += stride)
+      s += (double)a[i] * (double)b[i];
+  }
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Stage 2: out[0] = sum of partials (fixed order), optionally also copies
+// the previous value out[0] into out[1] first.
+__global__ void __launch_bounds__(256)
+    k_dot_final(const double* __restrict__ partial, int nb,
+                double* __restrict__ out, int keep_prev) {
+  __shared__ double red[4];
+  double s = 0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += partial[b];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) {
+    if (keep_prev) out[1] = out[0];
+    out[0] = s;
+  }
+}
+
+// y_k += sign * (num / den) * x_k  (den == 0 -> no update).
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_axpy_ratio(VP<T> y, CVP<T> x, int nvar, size_t N,
+                 const double* __restrict__ num,
+                 const double* __restrict__ den, double sign) {
+  const double d = *den;
+  const T a = d != 0.0 ? T(sign * (*num) / d) : T(0);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (int k = 0; k < nvar; ++k)
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += stride)
+      y.p[k][i] += a * x.p[k][i];
+}
+
+// p_k = z_k + (num / den) * p_k  (den == 0 -> p = z).
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_xpay_ratio(VP<T> p, CVP<T> z, int nvar, size_t N,
+                 const double* __restrict__ num,
+                 const double* __restrict__ den) {
+  const double d = *den;
+  const T b = d != 0.0 ? T((*num) / d) : T(0);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (int k = 0; k < nvar; ++k)
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += stride)
+      p.p[k][i] = z.p[k][i] + b * p.p[k][i];
+}
+
+}  // namespace odil
+
+using namespace odil;
+
+namespace {
+template <class T>
+VP<T> mkv(void* const* a, int n) {
+  VP<T> v{};
+  for (int k = 0; k < n; ++k) v.p[k] = static_cast<T*>(a[k]);
+  return v;
+}
+template <class T>
+CVP<T> mkc(const void* const* a, int n) {
+  CVP<T> v{};
+  for (int k = 0; k < n; ++k) v.p[k] = static_cast<const T*>(a[k]);
+  return v;
+}
+unsigned grid_for(size_t N) {
+  size_t b = (N + 255) / 256;
+  return (unsigned)(b < 2048 ? (b ? b : 1) : 2048);
+}
+}  // namespace
+
+extern "C" {
+
+// Number of FP64 partials `partial` must hold for odil_dot.
+int odil_dot_partials(void) { return kDotBlocks; }
+
+// out[0] = sum_k x_k . y_k (FP64). keep_prev != 0 moves the old out[0] to
+// out[1] first (the previous r.z of conjugate gradients).
+int odil_dot(int dtype, int nvar, long long N, const void* const* x,
+             const void* const* y, double* partial, double* out,
+             int keep_prev, void* stream) {
+  if (nvar < 1 || nvar > kMaxVec || N < 0) {
+    set_error("dot: bad nvar/N");
+    return kErrArg;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_dot_partial<double>, dim3(kDotBlocks), dim3(256), 0,
+                       s, mkc<double>(x, nvar), mkc<double>(y, nvar), nvar,
+                       (size_t)N, partial);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_dot_partial<float>, dim3(kDotBlocks), dim3(256), 0, s,
+                       mkc<float>(x, nvar), mkc<float>(y, nvar), nvar,
+                       (size_t)N, partial);
+  else {
+    set_error("dot: bad dtype");
+    return kErrArg;
+  }
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, s, partial,
+                     kDotBlocks, out, keep_prev);
+  return check_launch("dot");
+}
+
+int odil_axpy_ratio(int dtype, int nvar, long long N, void* const* y,
+                    const void* const* x, const double* num,
+                    const double* den, double sign, void* stream) {
+  if (nvar < 1 || nvar > kMaxVec) {
+    set_error("axpy_ratio: bad nvar");
+    return kErrArg;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_axpy_ratio<double>, dim3(grid_for(N)), dim3(256), 0,
+                       s, mkv<double>(y, nvar), mkc<double>(x, nvar), nvar,
+                       (size_t)N, num, den, sign);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_axpy_ratio<float>, dim3(grid_for(N)), dim3(256), 0,
+                       s, mkv<float>(y, nvar), mkc<float>(x, nvar), nvar,
+                       (size_t)N, num, den, sign);
+  else {
+    set_error("axpy_ratio: bad dtype");
+    return kErrArg;
+  }
+  return check_launch("axpy_ratio");
+}
+
+int odil_xpay_ratio(int dtype, int nvar, long long N, void* const* p,
+                    const void* const* z, const double* num,
+                    const double* den, void* stream) {
+  if (nvar < 1 || nvar > kMaxVec) {
+    set_error("xpay_ratio: bad nvar");
+    return kErrArg;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_xpay_ratio<double>, dim3(grid_for(N)), dim3(256), 0,
+                       s, mkv<double>(p, nvar), mkc<double>(z, nvar), nvar,
+                       (size_t)N, num, den);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_xpay_ratio<float>, dim3(grid_for(N)), dim3(256), 0,
+                       s, mkv<float>(p, nvar), mkc<float>(z, nvar), nvar,
+                       (size_t)N, num, den);
+  else {
+    set_error("xpay_ratio: bad dtype");
+    return kErrArg;
+  }
+  return check_launch("xpay_ratio");
+}
+
+}  // extern "C"

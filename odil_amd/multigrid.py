@@ -1,0 +1,535 @@
+"""Stencil operators and geometric multigrid on device (reference:
+`SparseOperator`, `MultigridOp`, multigrid.py:342-602,877-1214).
+
+`BlockStencil` is the device form of an nvar_out x nvar_in block of
+`SparseOperator`s: coefficient fields `coef[nent, *nwh]` and an entry table
+(out, in, shift). All arithmetic runs in the HIP kernels of
+`libodil_amd.so`; torch only owns the memory.
+"""
+
+import ctypes
+import itertools
+
+import numpy as np
+import torch
+
+from odil_amd import _lib
+from odil_amd._lib import c_double, c_int, check, ptr, ptrs, stream
+
+
+def _grid_arr(nw):
+    return (ctypes.c_int * 3)(*(list(nw) + [1] * (3 - len(nw))))
+
+
+def _shift3(s):
+    s = tuple(int(x) for x in s)
+    return s + (0, ) * (3 - len(s))
+
+
+class BlockStencil:
+    """Block stencil operator from nvar_in fields on grid `nw` to nvar_out
+    fields on grid `nwh = (nw - 1) // stride + 1`:
+    (A x)_o[I] = sum_{e: out_e = o} coef_e[I] x_{in_e}[stride*I + s_e],
+    zero outside the grid (non-circular, multigrid.py:941-969)."""
+
+    def __init__(self, coef, entries, nw, stride=1, nvar_in=None,
+                 nvar_out=None):
+        self.nw = tuple(int(n) for n in nw)
+        self.ndim = len(self.nw)
+        self.stride = int(stride)
+        self.nwh = tuple((n - 1) // self.stride + 1 for n in self.nw)
+        entries = [(int(o), int(i), tuple(int(x) for x in s))
+                   for o, i, s in entries]
+        if len(entries) != coef.shape[0]:
+            raise ValueError('entries and coefficient count differ')
+        if tuple(coef.shape[1:]) != self.nwh:
+            raise ValueError('coefficient shape {} != {}'.format(
+                tuple(coef.shape[1:]), self.nwh))
+        order = sorted(range(len(entries)), key=lambda q: entries[q])
+        if order != list(range(len(entries))):
+            idx = torch.tensor(order, device=coef.device)
+            coef = coef.index_select(0, idx)
+            entries = [entries[q] for q in order]
+        if len(set(entries)) != len(entries):
+            raise ValueError('duplicate stencil entries')
+        self.coef = coef.contiguous()
+        self.entries = entries
+        self.nvar_out = nvar_out if nvar_out is not None else (
+            max(e[0] for e in entries) + 1 if entries else 1)
+        self.nvar_in = nvar_in if nvar_in is not None else (
+            max(e[1] for e in entries) + 1 if entries else 1)
+        self._tables()
+
+    @property
+    def dtype(self):
+        return self.coef.dtype
+
+    @property
+    def device(self):
+        return self.coef.device
+
+    def _tables(self):
+        dev = self.coef.device
+        ent = np.array([[o, i] + list(_shift3(s)) for o, i, s in self.entries],
+                       dtype=np.int32).reshape(-1, 5)
+        ostart = np.zeros(self.nvar_out + 1, dtype=np.int32)
+        for o, _, _ in self.entries:
+            ostart[o + 1] += 1
+        ostart = np.cumsum(ostart).astype(np.int32)
+        zero = (0, ) * self.ndim
+        dent = np.full(self.nvar_out, -1, dtype=np.int32)
+        for q, (o, i, s) in enumerate(self.entries):
+            if o == i and s == zero:
+                dent[o] = q
+        by_in = sorted(range(len(self.entries)),
+                       key=lambda q: (self.entries[q][1], q))
+        tent = np.array(by_in, dtype=np.int32)
+        istart = np.zeros(self.nvar_in + 1, dtype=np.int32)
+        for _, i, _ in self.entries:
+            istart[i + 1] += 1
+        istart = np.cumsum(istart).astype(np.int32)
+        self._dent_host = dent
+        t = lambda a: torch.tensor(a, dtype=torch.int32, device=dev)
+        self.t_ent = t(ent)
+        self.t_ostart = t(ostart)
+        self.t_dent = t(dent)
+        self.t_tent = t(tent if len(tent) else np.zeros(1, np.int32))
+        self.t_istart = t(istart)
+
+    # -- products --------------------------------------------------------
+    def _fields(self, x, n, shape):
+        if isinstance(x, torch.Tensor):
+            x = [x[k] for k in range(n)] if x.dim() == len(shape) + 1 else [x]
+        x = list(x)
+        if len(x) != n:
+            raise ValueError('expected {} fields, got {}'.format(n, len(x)))
+        for a in x:
+            _lib.require_device(a)
+            if a.dtype != self.dtype:
+                raise ValueError('dtype mismatch {} vs {}'.format(
+                    a.dtype, self.dtype))
+            if a.numel() != int(np.prod(shape)):
+                raise ValueError('field size {} != grid {}'.format(
+                    a.numel(), shape))
+        return x
+
+    def _new(self, n, shape):
+        return [torch.empty(shape, dtype=self.dtype, device=self.device)
+                for _ in range(n)]
+
+    def _call_apply(self, x, y, f, mode, omega):
+        st = _lib.lib().odil_stencil_apply(
+            c_int(_lib.dtype_tag(self.dtype)), c_int(self.ndim),
+            _grid_arr(self.nw), c_int(self.stride), c_int(self.nvar_in),
+            c_int(self.nvar_out), ptr(self.t_ent), ptr(self.t_ostart),
+            ptr(self.t_dent), ptr(self.coef), ptrs(x), ptrs(y),
+            ptrs(f) if f is not None else None, c_int(mode),
+            c_double(omega), stream())
+        check(st, 'stencil_apply')
+
+    def apply(self, x, y=None):
+        """y = A x (mul_field)."""
+        x = self._fields(x, self.nvar_in, self.nw)
+        y = self._new(self.nvar_out, self.nwh) if y is None else y
+        self._call_apply(x, y, None, 0, 0.)
+        return y
+
+    def residual(self, x, f, y=None):
+        """y = f - A x."""
+        x = self._fields(x, self.nvar_in, self.nw)
+        f = self._fields(f, self.nvar_out, self.nwh)
+        y = self._new(self.nvar_out, self.nwh) if y is None else y
+        self._call_apply(x, y, f, 1, 0.)
+        return y
+
+    def jacobi(self, x, f, y, omega):
+        """One weighted-Jacobi sweep y = x + omega (f - A x) / diag(A)
+        (smoother_jacobi, multigrid.py:543-552). y must not alias x."""
+        if np.any(self._dent_host < 0):
+            raise ValueError('Jacobi needs a diagonal entry in every block')
+        self._call_apply(x, y, f, 2, omega)
+        return y
+
+    def apply_t(self, x, y=None, accumulate=False):
+        """y = A^T x (mul_transpose_field); accumulate adds into y."""
+        x = self._fields(x, self.nvar_out, self.nwh)
+        if y is None:
+            y = self._new(self.nvar_in, self.nw)
+            accumulate = False
+        st = _lib.lib().odil_stencil_apply_t(
+            c_int(_lib.dtype_tag(self.dtype)), c_int(self.ndim),
+            _grid_arr(self.nw), c_int(self.stride), c_int(self.nvar_in),
+            c_int(self.nvar_out), ptr(self.t_ent), ptr(self.t_tent),
+            ptr(self.t_istart), ptr(self.coef), ptrs(x), ptrs(y),
+            c_int(1 if accumulate else 0), stream())
+        check(st, 'stencil_apply_t')
+        return y
+
+    def to_dense(self):
+        """Dense FP64 matrix (nvar*N)^2 of a square stride-1 operator."""
+        if self.stride != 1 or self.nvar_in != self.nvar_out:
+            raise ValueError('to_dense needs a square stride-1 operator')
+        n = self.nvar_out * int(np.prod(self.nw))
+        M = torch.zeros((n, n), dtype=torch.float64, device=self.device)
+        st = _lib.lib().odil_stencil_to_dense(
+            c_int(_lib.dtype_tag(self.dtype)), c_int(self.ndim),
+            _grid_arr(self.nw), c_int(self.nvar_out), ptr(self.t_ent),
+            c_int(len(self.entries)), ptr(self.coef), ptr(M), stream())
+        check(st, 'stencil_to_dense')
+        return M
+
+    def astype(self, dtype):
+        if dtype == self.dtype:
+            return self
+        return BlockStencil(self.coef.to(dtype), self.entries, self.nw,
+                            self.stride, self.nvar_in, self.nvar_out)
+
+    def tocsr(self):
+        """SciPy CSR copy on the host (tests and the CSR export)."""
+        import scipy.sparse
+        coef = self.coef.double().cpu().numpy()
+        n = int(np.prod(self.nw))
+        nh = int(np.prod(self.nwh))
+        grids = np.meshgrid(*[np.arange(m) for m in self.nwh], indexing='ij')
+        rows, cols, vals = [], [], []
+        for q, (o, i, s) in enumerate(self.entries):
+            valid = np.ones(self.nwh, dtype=bool)
+            kk = []
+            for d in range(self.ndim):
+                k = self.stride * grids[d] + s[d]
+                valid &= (k >= 0) & (k < self.nw[d])
+                kk.append(np.where(valid, k, 0))
+            col = np.ravel_multi_index(tuple(np.where(valid, k, 0)
+                                             for k in kk), self.nw)
+            r = np.arange(nh).reshape(self.nwh)
+            rows.append((r + o * nh)[valid])
+            cols.append((col + i * n)[valid])
+            vals.append(coef[q][valid])
+        shape = (self.nvar_out * nh, self.nvar_in * n)
+        if not vals:
+            return scipy.sparse.csr_matrix(shape)
+        m = scipy.sparse.csr_matrix(
+            (np.concatenate(vals), (np.concatenate(rows),
+                                    np.concatenate(cols))), shape=shape)
+        m.sum_duplicates()
+        return m
+
+    @staticmethod
+    def from_jacobian(jac):
+        """J (equations x fields) as a block stencil."""
+        return BlockStencil(jac.coef, jac.entries, jac.shape,
+                            nvar_in=jac.nvar, nvar_out=jac.neq)
+
+
+def normal_operator(J, dtype=None):
+    """N = J^T J in stencil form (the sum over equations of
+    `mul_transpose_op`, util.py:577-583 / multigrid.py:1035-1071):
+    N_o[m] = sum_{(a,b,sa)} J_a[m - sa] J_b[m - sa], out var = field of a,
+    in var = field of b, shift = s_b - s_a. Accumulated in FP64."""
+    if J.stride != 1:
+        raise ValueError('normal_operator needs a stride-1 Jacobian')
+    dtype = dtype or J.dtype
+    out = {}
+    for a, (ea, ka, sa) in enumerate(J.entries):
+        for b, (eb, kb, sb) in enumerate(J.entries):
+            if ea != eb:
+                continue
+            d = tuple(y - x for x, y in zip(sa, sb))
+            out.setdefault((ka, kb, d), []).append((a, b) + _shift3(sa))
+    keys = sorted(out)
+    terms = []
+    tstart = [0]
+    for k in keys:
+        terms.extend(out[k])
+        tstart.append(len(terms))
+    dev = J.device
+    t_terms = torch.tensor(np.array(terms, dtype=np.int32).reshape(-1, 5),
+                           device=dev)
+    t_start = torch.tensor(np.array(tstart, dtype=np.int32), device=dev)
+    coef = torch.empty((len(keys), ) + J.nw, dtype=dtype, device=dev)
+    st = _lib.lib().odil_stencil_normal(
+        c_int(_lib.dtype_tag(J.dtype)), c_int(_lib.dtype_tag(dtype)),
+        c_int(J.ndim), _grid_arr(J.nw), ptr(t_terms), ptr(t_start),
+        c_int(len(keys)), ptr(J.coef), ptr(coef), stream())
+    check(st, 'stencil_normal')
+    return BlockStencil(coef, keys, J.nw, nvar_in=J.nvar_in,
+                        nvar_out=J.nvar_in)
+
+
+def restriction_weights(restriction, dim):
+    """Inner-node restriction weights indexed by offset r in {-1,0,1}^3
+    (get_R, multigrid.py:496-534)."""
+    rw = np.zeros(27)
+    for r in itertools.product([-1, 0, 1], repeat=3):
+        if dim == 2 and r[2] != 0:
+            continue
+        a = sum(abs(x) for x in r)
+        if restriction == 'full':
+            w = 0.5**(a + dim)
+        elif restriction == 'half':
+            w = 0.5 if a == 0 else (0.25 / dim if a == 1 else 0.)
+        elif restriction == 'injection':
+            w = 1. if a == 0 else 0.
+        else:
+            raise ValueError('Unknown restriction=' + restriction)
+        rw[(r[0] + 1) * 9 + (r[1] + 1) * 3 + (r[2] + 1)] = w
+    return rw
+
+
+def galerkin(A, restriction='full', rw_dev=None, mode='reference'):
+    """Coarse operator on the node grid nw//2+1.
+
+    mode 'exact': R A T (the CSR `Multigrid.update_A`, multigrid.py:131).
+    mode 'reference': the arithmetic of the stencil coarsening
+    `coarsen_sparse_operator` (multigrid.py:390-427), which pairs the fine
+    coefficient c_s(a) with node a - s; it equals reflect(R reflect(A) T)
+    with reflect: c_s(x) -> c_{-s}(x), and R A T when the stencil is
+    reflection-symmetric."""
+    if mode not in ('reference', 'exact'):
+        raise ValueError('Unknown galerkin mode ' + str(mode))
+    sigma = 1 if mode == 'exact' else -1
+    if A.stride != 1 or A.nvar_in != A.nvar_out:
+        raise ValueError('galerkin needs a square stride-1 operator')
+    dim = A.ndim
+    ofs = list(itertools.product([-1, 0, 1], repeat=dim))
+    dcs = list(itertools.product([-2, -1, 0, 1, 2], repeat=dim))
+    blocks = {}
+    for q, (o, i, s) in enumerate(A.entries):
+        blocks.setdefault((o, i), []).append((q, s))
+    keys, terms, tstart = [], [], [0]
+    for (o, i) in sorted(blocks):
+        for dc in dcs:
+            tl = []
+            for q, s in blocks[(o, i)]:
+                for r in ofs:
+                    t = [r[d] + sigma * (s[d] - 2 * dc[d])
+                         for d in range(dim)]
+                    if all(abs(x) <= 1 for x in t):
+                        tl.append((q, ) + _shift3(r) +
+                                  (sum(abs(x) for x in t), ))
+            if tl:
+                keys.append((o, i, dc))
+                terms.extend(tl)
+                tstart.append(len(terms))
+    dev = A.device
+    nwh = tuple(n // 2 + 1 for n in A.nw)
+    t = lambda a: torch.tensor(np.array(a, dtype=np.int32), device=dev)
+    oent = t([[o, i] + list(_shift3(dc)) for o, i, dc in keys])
+    wtab = torch.tensor([1., 0.5, 0.25, 0.125], dtype=torch.float64,
+                        device=dev)
+    if rw_dev is None:
+        rw_dev = torch.tensor(restriction_weights(restriction, dim),
+                              device=dev)
+    coef = torch.empty((len(keys), ) + nwh, dtype=A.dtype, device=dev)
+    st = _lib.lib().odil_stencil_galerkin(
+        c_int(_lib.dtype_tag(A.dtype)), c_int(dim), _grid_arr(A.nw),
+        ptr(A.t_ent), ptr(oent), ptr(t(terms).reshape(-1, 5)),
+        ptr(t(tstart)), c_int(len(keys)), ptr(rw_dev), ptr(wtab), ptr(A.coef),
+        ptr(coef), c_int(sigma), stream())
+    check(st, 'stencil_galerkin')
+    return BlockStencil(coef, keys, nwh, nvar_in=A.nvar_in,
+                        nvar_out=A.nvar_out)
+
+
+def restrict(fine, nw, rw_dev, coarse=None):
+    """Restriction R of a list of fields from node grid `nw` to nw//2+1."""
+    nwh = tuple(n // 2 + 1 for n in nw)
+    if coarse is None:
+        coarse = [torch.empty(nwh, dtype=f.dtype, device=f.device)
+                  for f in fine]
+    st = _lib.lib().odil_mg_restrict(
+        c_int(_lib.dtype_tag(fine[0].dtype)), c_int(len(nw)), _grid_arr(nw),
+        c_int(len(fine)), ptr(rw_dev), ptrs(fine), ptrs(coarse), stream())
+    check(st, 'mg_restrict')
+    return coarse
+
+
+def prolong_add(coarse, fine, nw):
+    """fine += T coarse (multilinear interpolation, node grids)."""
+    st = _lib.lib().odil_mg_prolong_add(
+        c_int(_lib.dtype_tag(fine[0].dtype)), c_int(len(nw)), _grid_arr(nw),
+        c_int(len(fine)), ptrs(coarse), ptrs(fine), stream())
+    check(st, 'mg_prolong_add')
+    return fine
+
+
+def level_sizes(nw, nlevels=None):
+    """Grid sizes per level, (n - 1)//2 + 1 (multigrid.py:362-370)."""
+    nw = np.array(nw)
+    max_nlevels = 0
+    while 2**(max_nlevels + 1) < min(nw):
+        max_nlevels += 1
+    nlevels = max_nlevels if nlevels is None else min(nlevels, max_nlevels)
+    nnw = [tuple(int(x) for x in nw)]
+    for _ in range(1, nlevels):
+        nnw.append(tuple((n - 1) // 2 + 1 for n in nnw[-1]))
+    return nnw
+
+
+class MultigridOp:
+    """Geometric multigrid on node-based grids of odd size (2^k+1 for full
+    depth), mirroring the reference `MultigridOp` (multigrid.py:342-602):
+    Galerkin coarse operators R A T, weighted-Jacobi smoothing, and a
+    direct solve on the coarsest level, here an FP64 dense inverse applied by
+    a GEMV kernel.
+
+    `coarse_reg` adds coarse_reg * max|diag| to the coarsest diagonal before
+    inversion; use a tiny value for operators with a null space (the cavity's
+    constant pressure), 0 otherwise.
+    """
+
+    def __init__(self, nw, nvar=1, nlevels=None, restriction='full',
+                 dtype=torch.float64, device=None, coarse_reg=0.,
+                 galerkin_mode='reference'):
+        self.nnw = level_sizes(nw, nlevels)
+        self.nlevels = len(self.nnw)
+        for n in self.nnw[:-1]:
+            if any(x % 2 == 0 for x in n):
+                raise ValueError('grid sizes {} must be odd on every level '
+                                 'but the coarsest'.format(self.nnw))
+        self.nvar = nvar
+        self.dim = len(self.nnw[0])
+        self.restriction = restriction
+        self.dtype = dtype
+        self.device = device or torch.device('cuda')
+        self.coarse_reg = float(coarse_reg)
+        self.galerkin_mode = galerkin_mode
+        self.rw = torch.tensor(restriction_weights(restriction, self.dim),
+                               device=self.device)
+        self.AA = None
+        self._buf = None
+
+    # -- setup -----------------------------------------------------------
+    def update_A(self, A0):
+        """A0: BlockStencil of the finest operator (cast to self.dtype)."""
+        if A0.nw != self.nnw[0] or A0.nvar_in != self.nvar:
+            raise ValueError('operator grid/fields do not match the levels')
+        AA = [A0.astype(self.dtype)]
+        for level in range(1, self.nlevels):
+            AA.append(galerkin(AA[-1], self.restriction, self.rw,
+                               self.galerkin_mode))
+        self.AA = AA
+        self._coarse_inv = {}
+        return self
+
+    def _coarse(self, level):
+        if level not in self._coarse_inv:
+            M = self.AA[level].to_dense()
+            if self.coarse_reg:
+                s = M.diagonal().abs().max()
+                M = M + torch.eye(M.shape[0], dtype=M.dtype,
+                                  device=M.device) * (self.coarse_reg * s)
+            self._coarse_inv[level] = torch.linalg.inv(M)
+        return self._coarse_inv[level]
+
+    def prepare(self, ndirect=5):
+        """Precompute the coarse inverse used by step() (outside any graph
+        capture)."""
+        lv = self._direct_level(ndirect)
+        self._coarse(lv)
+        return lv
+
+    def _direct_level(self, ndirect):
+        for level in range(self.nlevels):
+            if level + 1 >= self.nlevels or min(self.nnw[level]) <= ndirect:
+                return level
+        return self.nlevels - 1
+
+    # -- reference-shaped operations -------------------------------------
+    def residual(self, A, u, f):
+        return A.residual(u, f)
+
+    def smoother_jacobi(self, level, u, f, omega=1.):
+        """In-place weighted Jacobi (u is modified and returned)."""
+        tmp = [torch.empty_like(x) for x in u]
+        self.AA[level].jacobi(u, f, tmp, omega)
+        for a, b in zip(u, tmp):
+            a.copy_(b)
+        return u
+
+    def smoother_direct(self, level, u, f):
+        Minv = self._coarse(level)
+        out = [torch.empty_like(x) for x in f]
+        N = int(np.prod(self.nnw[level]))
+        dt = _lib.dtype_tag(f[0].dtype)
+        st = _lib.lib().odil_dense_gemv(c_int(dt), c_int(dt),
+                                        c_int(self.nvar), c_int(N), ptr(Minv),
+                                        ptrs(f), ptrs(out), stream())
+        check(st, 'dense_gemv')
+        return out
+
+    def step(self, u, f, omega=0.6, level=0, ndirect=5, pre=2, post=2):
+        """One V-cycle. Like the reference (multigrid.py:589-593) the
+        recursive call does not pass `ndirect`, `pre`, `post`: coarser levels
+        use 5, 2, 2."""
+        if level + 1 >= self.nlevels or min(self.nnw[level]) <= ndirect:
+            return self.smoother_direct(level, u, f)
+        for _ in range(pre):
+            u = self.smoother_jacobi(level, u, f, omega)
+        A = self.AA[level]
+        r = A.residual(u, f)
+        rc = restrict(r, self.nnw[level], self.rw)
+        du = self.step([torch.zeros_like(x) for x in rc], rc, omega,
+                       level=level + 1)
+        u = [x.clone() for x in u]
+        prolong_add(du, u, self.nnw[level])
+        for _ in range(post):
+            u = self.smoother_jacobi(level, u, f, omega)
+        return u
+
+    # -- allocation-free V-cycle on preallocated buffers ----------------
+    def _buffers(self):
+        if self._buf is None:
+            b = []
+            for n in self.nnw:
+                mk = lambda: [torch.zeros(n, dtype=self.dtype,
+                                          device=self.device)
+                              for _ in range(self.nvar)]
+                b.append({'u': mk(), 'v': mk(), 'f': mk(), 'r': mk()})
+            self._buf = b
+        return self._buf
+
+    def vcycle(self, u, f, omega=0.6, ndirect=5, pre=2, post=2):
+        """V-cycle with the same arithmetic as step() but on preallocated
+        level buffers (no allocation, capturable in a HIP graph). `u` is
+        updated in place; `f` is read only."""
+        buf = self._buffers()
+        self._vc(0, u, f, buf[0]['v'], omega, ndirect, pre, post)
+        return u
+
+    def _vc(self, level, u, f, spare, omega, ndirect, pre, post):
+        # `u` holds the iterate; `spare` is a same-shape scratch. Returns the
+        # list holding the result (u or spare).
+        if level + 1 >= self.nlevels or min(self.nnw[level]) <= ndirect:
+            Minv = self._coarse(level)
+            N = int(np.prod(self.nnw[level]))
+            dt = _lib.dtype_tag(f[0].dtype)
+            check(_lib.lib().odil_dense_gemv(c_int(dt), c_int(dt),
+                                             c_int(self.nvar), c_int(N),
+                                             ptr(Minv), ptrs(f), ptrs(spare),
+                                             stream()), 'dense_gemv')
+            for a, b in zip(u, spare):
+                a.copy_(b)
+            return u
+        A = self.AA[level]
+        buf = self._buffers()
+        cur, oth = u, spare
+        for _ in range(pre):
+            A.jacobi(cur, f, oth, omega)
+            cur, oth = oth, cur
+        r = buf[level]['r']
+        A.residual(cur, f, r)
+        nxt = buf[level + 1]
+        restrict(r, self.nnw[level], self.rw, nxt['f'])
+        for x in nxt['u']:
+            x.zero_()
+        res = self._vc(level + 1, nxt['u'], nxt['f'], nxt['v'], omega, 5, 2,
+                       2)
+        prolong_add(res, cur, self.nnw[level])
+        for _ in range(post):
+            A.jacobi(cur, f, oth, omega)
+            cur, oth = oth, cur
+        if cur is not u:
+            for a, b in zip(u, cur):
+                a.copy_(b)
+        return u

@@ -1,0 +1,112 @@
+"""Parity of the HIP cavity kernels with the CPU oracle (float64 tape)."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import operators as ops
+from oracle import tape
+
+pytestmark = pytest.mark.gpu
+
+# FP64 kernels must agree with the oracle to round-off: terms are O(mu/h^2),
+# so the bound is relative to the largest term magnitude of each field.
+RTOL64 = 1e-12
+RTOL32 = 2e-5
+
+
+def _state(n0, n1, seed):
+    rng = np.random.default_rng(seed)
+    return {k: rng.uniform(-1, 1, (n0, n1)) for k in 'uvp'}
+
+
+def _dev(fields, dev, dtype):
+    return [torch.tensor(fields[k], dtype=dtype, device=dev) for k in 'uvp']
+
+
+def _close(a, b, rtol, scale=None):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    s = scale if scale is not None else max(np.abs(b).max(), 1e-300)
+    err = np.abs(a - b).max() / s
+    assert err <= rtol, 'relative error {:.3e} > {:.1e}'.format(err, rtol)
+
+
+CASES = [(16, 16, 100., 0, 1), (17, 23, 400., 0, 1), (9, 12, 10., 1, 1),
+         (11, 10, 50., 2, 1), (12, 9, 30., 0, 0)]
+
+
+@pytest.mark.parametrize('n0,n1,re,bc,rc', CASES)
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_residual_and_jacobian(dev, n0, n1, re, bc, rc, dtype):
+    from odil_amd.operators import Cavity
+    st = _state(n0, n1, seed=n0 * 7 + n1)
+    dom = ops.cavity_domain(n0, n1)
+    op_ref = ops.make_cavity_operator(1 / re, bc=bc, rhiechow=rc)
+    ff, grads, desc = tape.eval_grad(op_ref, dom, st)
+    op = Cavity((n0, n1), 1 / re, bc=bc, rhiechow=rc)
+    rtol = RTOL64 if dtype == torch.float64 else RTOL32
+    res, ssq = op.residual(_dev(st, dev, dtype))
+    for e in range(3):
+        _close(res[e].cpu(), ff[e], rtol, scale=np.abs(ff[e]).max() + 1 / re)
+        np.testing.assert_allclose(ssq[e].item(), (ff[e]**2).sum(),
+                                   rtol=1e-10 if dtype == torch.float64 else
+                                   1e-4)
+    jac, _ = op.jacobian(_dev(st, dev, dtype))
+    coef = jac.coef.cpu().numpy()
+    seen = set()
+    for q, (e, k, s) in enumerate(jac.entries):
+        key = (k, tuple(s))
+        g = None
+        if key in desc:
+            g = grads[e][desc.index(key)]
+        if g is None:
+            assert np.abs(coef[q]).max() == 0, (e, k, s)
+        else:
+            _close(coef[q], g, rtol, scale=np.abs(g).max() + 1)
+            seen.add((e, key))
+    # Every non-zero tape coefficient is represented by a kernel entry.
+    for e in range(3):
+        for j, key in enumerate(desc):
+            g = grads[e][j]
+            if g is not None and np.any(g):
+                assert (e, key) in seen, (e, key)
+
+
+@pytest.mark.parametrize('n0,n1,re,bc,rc', CASES + [(40, 70, 100., 0, 1)])
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_loss_grad(dev, n0, n1, re, bc, rc, dtype):
+    from odil_amd.operators import Cavity
+    st = _state(n0, n1, seed=3 * n0 + n1)
+    dom = ops.cavity_domain(n0, n1)
+    op_ref = ops.make_cavity_operator(1 / re, bc=bc, rhiechow=rc)
+    loss, grads, split = tape.eval_loss_grad(op_ref, dom, st)
+    op = Cavity((n0, n1), 1 / re, bc=bc, rhiechow=rc)
+    l, g, ls = op.loss_grad(_dev(st, dev, dtype))
+    rtol = RTOL64 if dtype == torch.float64 else RTOL32
+    np.testing.assert_allclose(l.item(), loss,
+                               rtol=1e-11 if dtype == torch.float64 else 1e-4)
+    np.testing.assert_allclose(ls.cpu().numpy(), split,
+                               rtol=1e-11 if dtype == torch.float64 else 1e-4)
+    for k, name in enumerate('uvp'):
+        _close(g[k].cpu(), grads[name], rtol,
+               scale=np.abs(grads[name]).max())
+
+
+def test_fixture_state_residual(dev):
+    """Converged Re=100 state shipped by the reference (128x128): the FP64
+    residual kernel agrees with the oracle and is small."""
+    import os
+    from odil_amd.operators import Cavity
+    path = os.path.join(os.path.dirname(__file__), 'golden',
+                        'cavity_N128_Re100.npz')
+    d = np.load(path)
+    st = {k: d[k] for k in 'uvp'}
+    ff = tape.eval_operator(ops.make_cavity_operator(0.01),
+                            ops.cavity_domain(128), st)
+    res, ssq = Cavity((128, 128), 0.01).residual(_dev(st, dev,
+                                                      torch.float64))
+    # Round-off is relative to the largest term, mu/h^2 * |u| ~ 164.
+    for e in range(3):
+        _close(res[e].cpu(), ff[e], 1e-12, scale=0.01 * 128**2)
+        assert np.sqrt(ssq[e].item() / 128**2) < 1e-5

@@ -1,0 +1,246 @@
+"""Pins of the CPU oracle against the reference's golden data and against
+closed forms (CPU only, no GPU)."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse
+
+from oracle import multigrid as omg
+from oracle import newton
+from oracle import operators as ops
+from oracle import tape
+
+GOLD = os.path.join(os.path.dirname(__file__), 'golden')
+
+
+def _load(name):
+    with open(os.path.join(GOLD, name)) as f:
+        return json.load(f)
+
+
+# -- wave: 2 Newton steps reproduce the reference's discretisation error ----
+@pytest.mark.parametrize('nx', [3, 4, 5, 8, 10, 16, 21, 33, 40, 64])
+def test_wave_newton_matches_reference_error(nx):
+    g = _load('wave_fd_newton.json')
+    ref = g['error'][g['nx'].index(nx)]
+    dom, op, exact = ops.wave_reference_setup(nx, nx)
+    packed = newton.optimize_newton(op, dom, {'u': np.zeros((nx, nx))}, 2)
+    u = tape.unpack_fields(dom, packed)['u']
+    err = np.mean((exact - u)**2)**0.5
+    assert abs(err - ref) <= 1e-11 * ref
+
+
+# -- cavity: Newton trajectory max|du|, max|dv| of the reference CSV --------
+@pytest.mark.parametrize('re_,epochs', [(100, 6), (400, 4)])
+def test_cavity_newton_trajectory(re_, epochs):
+    g = _load('cavity_newton.json')['Re{}'.format(re_)]
+    dom = ops.cavity_domain(64)
+    op = ops.make_cavity_operator(1 / re_)
+    prev = [None]
+    got = []
+
+    def cb(epoch, packed, const):
+        if prev[0] is not None:
+            d = tape.unpack_fields(dom, packed - prev[0])
+            got.append((epoch, abs(d['u']).max(), abs(d['v']).max()))
+        prev[0] = packed.copy()
+
+    zero = {k: np.zeros((64, 64)) for k in 'uvp'}
+    newton.optimize_newton(op, dom, zero, epochs, callback=cb)
+    for epoch, du, dv in got:
+        assert abs(du - g['du'][epoch]) <= 1e-9 * g['du'][epoch]
+        assert abs(dv - g['dv'][epoch]) <= 1e-9 * g['dv'][epoch]
+
+
+def test_cavity_fixture_is_converged():
+    d = np.load(os.path.join(GOLD, 'cavity_N128_Re100.npz'))
+    ff = tape.eval_operator(ops.make_cavity_operator(0.01),
+                            ops.cavity_domain(128), {k: d[k] for k in 'uvp'})
+    for f in ff:
+        assert np.sqrt(np.mean(f**2)) < 1e-5
+
+
+# -- tape gradients against central finite differences ---------------------
+def _fd_grad(loss_fn, x, eps):
+    g = np.zeros_like(x)
+    it = np.nditer(x, flags=['multi_index'])
+    for _ in it:
+        i = it.multi_index
+        xp = x.copy()
+        xm = x.copy()
+        xp[i] += eps
+        xm[i] -= eps
+        g[i] = (loss_fn(xp) - loss_fn(xm)) / (2 * eps)
+    return g
+
+
+def test_wave_loss_gradient_fd():
+    dom, op, _ = ops.wave_gaussian_setup(8, 9, seed=3)
+    rng = np.random.default_rng(1)
+    u = rng.normal(size=(8, 9))
+    _, grads, _ = tape.eval_loss_grad(op, dom, {'u': u})
+    fd = _fd_grad(lambda x: tape.eval_loss_grad(op, dom, {'u': x})[0], u,
+                  1e-6)
+    np.testing.assert_allclose(grads['u'], fd, rtol=1e-6,
+                               atol=1e-8 * np.abs(fd).max())
+
+
+def test_cavity_loss_gradient_frozen_fd():
+    """Frozen fields are held at the base state while live fields move:
+    the tape gradient equals FD of that partially frozen loss."""
+    n = 6
+    rng = np.random.default_rng(2)
+    base = {k: rng.uniform(-1, 1, (n, n)) for k in 'uvp'}
+    dom = ops.cavity_domain(n)
+    op = ops.make_cavity_operator(0.05)
+    _, grads, _ = tape.eval_loss_grad(op, dom, base)
+
+    import torch
+
+    class Split(tape.Context):
+        # live values from `live`, frozen values from the base state
+        def __init__(self, live, *a, **k):
+            super().__init__(*a, **k)
+            self.live = live
+
+        def field(self, name, *shift, freeze=False):
+            if freeze or name in ('zeros', 'ones'):
+                return super().field(name, *shift, freeze=freeze)
+            d = self.domain
+            shift = tuple(shift) + (0, ) * (d.ndim - len(shift))
+            return torch.roll(self.live[name], tuple(-s for s in shift),
+                              tuple(range(d.ndim)))
+
+    def loss(name, x):
+        live = {k: torch.tensor(base[k]) for k in 'uvp'}
+        live[name] = torch.tensor(x)
+        ctx = Split(live, dom, {k: torch.tensor(base[k]) for k in 'uvp'},
+                    split_shifts=False)
+        ff = op(tape.TorchMod, ctx)
+        return float(sum(torch.mean(f**2) for f in ff))
+
+    for name in 'uvp':
+        fd = _fd_grad(lambda x: loss(name, x), base[name], 1e-6)
+        np.testing.assert_allclose(grads[name], fd, rtol=1e-5,
+                                   atol=1e-7 * np.abs(fd).max())
+
+
+def test_advection_gradient_fd():
+    dom, op, _ = ops.advection_setup(4, 5, 6, seed=4)
+    rng = np.random.default_rng(3)
+    p = rng.normal(size=dom.shape)
+    _, grads, _ = tape.eval_loss_grad(op, dom, {'p': p})
+    fd = _fd_grad(lambda x: tape.eval_loss_grad(op, dom, {'p': x})[0], p,
+                  1e-6)
+    np.testing.assert_allclose(grads['p'], fd, rtol=1e-6,
+                               atol=1e-8 * np.abs(fd).max())
+
+
+# -- multigrid: transfer operators, Galerkin, V-cycle ------------------------
+def test_interpolation_exact_for_linear_fields():
+    nw = (17, 9)
+    tt = omg.get_TT(nw)
+    xc, yc = np.meshgrid(np.arange(9) * 2., np.arange(5) * 2., indexing='ij')
+    xf, yf = np.meshgrid(np.arange(17.), np.arange(9.), indexing='ij')
+    uc = 3 * xc - 2 * yc + 1
+    uf = tt.mul_transpose_field(uc.ravel()).reshape(nw)
+    np.testing.assert_allclose(uf, 3 * xf - 2 * yf + 1, rtol=0, atol=1e-12)
+
+
+def test_restriction_weights():
+    R = omg.get_R((9, 9)).tocsr().toarray()
+    sums = R.sum(axis=1).reshape(5, 5)
+    np.testing.assert_allclose(sums, np.ones((5, 5)))
+    # boundary coarse nodes inject
+    row = np.ravel_multi_index((0, 2), (5, 5))
+    assert R[row, np.ravel_multi_index((0, 4), (9, 9))] == 1.
+    assert np.count_nonzero(R[row]) == 1
+
+
+def test_galerkin_modes_agree_for_symmetric_stencil():
+    nw = (11, 11)
+    pad = lambda: np.pad(np.ones((9, 9)), 1)
+    A = omg.StencilOp({(0, 0): 4 * np.ones(nw), (1, 0): -pad(),
+                       (-1, 0): -pad(), (0, 1): -pad(), (0, -1): -pad()}, nw)
+    R, TT = omg.get_R(nw), omg.get_TT(nw)
+    a = omg.galerkin(A, R, TT, 'exact').tocsr()
+    b = omg.galerkin(A, R, TT, 'reference').tocsr()
+    assert abs(a - b).max() < 1e-14
+    c = R.tocsr() @ A.tocsr() @ TT.tocsr().T
+    assert abs(a - c).max() < 1e-14
+
+
+def test_vcycle_matches_reference_golden():
+    g = np.load(os.path.join(GOLD, 'mg_vcycle.npz'))
+    n = 17
+    stf = {k: g['poisson17_c_{}_{}'.format(*k)]
+           for k in [(0, 0), (1, 0), (-1, 0), (0, 1), (0, -1)]}
+    M = omg.MultigridOp((n, n), nvar=1)
+    M.update_A([[omg.StencilOp(stf, (n, n))]])
+    u = [np.zeros(n * n)]
+    rhs = g['poisson17_rhs']
+    for it in range(3):
+        u = M.step(u, [rhs.copy()], 0.7)
+        ref = g['poisson17_u{}'.format(it + 1)]
+        assert np.abs(u[0] - ref).max() <= 1e-13 * np.abs(ref).max()
+
+    st = {k: g['cavity17_state_' + k] for k in 'uvp'}
+    const, grads, desc = tape.eval_grad(ops.make_cavity_operator(0.01),
+                                        ops.cavity_domain(n), st)
+    stf = omg.coeffs_from_tape(grads, desc, 3, 3)
+    mm, matr = omg.normal_operator(stf, (n, n), 3)
+    rhs = [-sum(mm[i][j].mul_transpose_field(const[i].ravel())
+                for i in range(3)) for j in range(3)]
+    np.testing.assert_allclose(np.array(rhs), g['cavity17_rhs'], rtol=0,
+                               atol=1e-10 * np.abs(g['cavity17_rhs']).max())
+    M = omg.MultigridOp((n, n), nvar=3, nlevels=4)
+    M.update_A(matr)
+    sol = [np.zeros(n * n) for _ in range(3)]
+    for it in range(3):
+        sol = M.step(sol, [r.copy() for r in rhs], 0.6, ndirect=3)
+        ref = g['cavity17_u{}'.format(it + 1)]
+        for k in range(3):
+            d = sol[k] - ref[k]
+            if k == 2:  # pressure is defined up to a constant
+                d = d - d.mean()
+            assert np.abs(d).max() <= 1e-10 * np.abs(ref[k]).max()
+
+
+def test_poisson_multigrid_converges():
+    """Acceptance of the reference's test_multigrid_poisson
+    (test_multigrid.py:251-368): 10 V-cycles, |rhs - A u| <= 1e-6."""
+    n = 17
+    dx = 1 / (n - 1)
+    pad = lambda: np.pad(np.ones((n - 2, n - 2)), 1) / dx**2
+    A = omg.StencilOp({(0, 0): 4 * np.ones((n, n)) / dx**2, (1, 0): -pad(),
+                       (-1, 0): -pad(), (0, 1): -pad(), (0, -1): -pad()},
+                      (n, n))
+    M = omg.MultigridOp((n, n))
+    M.update_A([[A]])
+    rhs = np.full(n * n, 10.)
+    u = [np.zeros(n * n)]
+    for _ in range(10):
+        u = M.step(u, [rhs], 0.7)
+    assert np.linalg.norm(rhs - A.mul_field(u[0])) <= 1e-6
+
+
+def test_normal_operator_is_jtj():
+    n = 7
+    rng = np.random.default_rng(5)
+    st = {k: rng.uniform(-1, 1, (n, n)) for k in 'uvp'}
+    const, grads, desc = tape.eval_grad(ops.make_cavity_operator(0.02),
+                                        ops.cavity_domain(n), st)
+    stf = omg.coeffs_from_tape(grads, desc, 3, 3)
+    mm, matr = omg.normal_operator(stf, (n, n), 3)
+    J = scipy.sparse.bmat([[mm[i][j].tocsr() for j in range(3)]
+                           for i in range(3)])
+    N = scipy.sparse.bmat([[matr[i][j].tocsr() for j in range(3)]
+                           for i in range(3)])
+    assert abs(J.T @ J - N).max() < 1e-9
+    # and the CSR Jacobian of `linearize` equals the stencil form
+    _, lin = tape.linearize(ops.make_cavity_operator(0.02),
+                            ops.cavity_domain(n), st)
+    assert abs(lin - J).max() < 1e-12
