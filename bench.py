@@ -1,0 +1,248 @@
+#!/usr/bin/env python3
+"""Benchmark: Newton steps/s of the lid-driven cavity at 1025x1025 (the
+1024^2-class grid of the reference's node-based multigrid, 2^10 + 1 cells per
+axis), mixed precision vs FP64, plus residual / adjoint kernel bandwidth.
+
+One step = one Newton iteration: residual + stencil Jacobian (FP64), right
+hand side -J^T f, normal operator J^T J, Galerkin hierarchy (7 levels, down
+to 17x17), coarsest FP64 solve operator, `maxiter` V(2,2) cycles, state
+update. Inputs start resident in HBM (zero state, Re = 100, lid u = 1).
+
+Prints one JSON line (see the keys below). `--gpus N` with N > 1 is not
+supported: this build runs on one GPU.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument('--gpus', type=int, default=1)
+    p.add_argument('--steps', type=int, default=20)
+    p.add_argument('--warmup', type=int, default=3)
+    p.add_argument('--n', type=int, default=1025)
+    p.add_argument('--re', type=float, default=100.)
+    p.add_argument('--maxiter', type=int, default=20)
+    p.add_argument('--linsolver', default='vcycle', choices=['vcycle', 'pcg'])
+    p.add_argument('--omega', type=float, default=0.5)
+    p.add_argument('--nlvl', type=int, default=7)
+    p.add_argument('--tol', type=float, default=1e-8)
+    p.add_argument('--max_epochs', type=int, default=150)
+    p.add_argument('--kernel_reps', type=int, default=50)
+    p.add_argument('--no_tol_run', action='store_true')
+    p.add_argument('--no_cpu', action='store_true')
+    p.add_argument('--cpu_n', type=int, default=129)
+    return p.parse_args()
+
+
+def make_solver(args, precision):
+    import torch
+    from odil_amd.newton import NewtonMultigrid
+    from odil_amd.operators import Cavity
+    op = Cavity((args.n, args.n), 1.0 / args.re)
+    nm = NewtonMultigrid(op, nlvl=args.nlvl, maxiter=args.maxiter,
+                         omega=args.omega, precision=precision,
+                         linsolver=args.linsolver, galerkin_mode='exact',
+                         restriction='transpose')
+    fields = [torch.zeros(args.n, args.n, dtype=torch.float64,
+                          device='cuda') for _ in range(3)]
+    return op, nm, fields
+
+
+def steps_per_second(args, precision):
+    import torch
+    _, nm, fields = make_solver(args, precision)
+    for _ in range(args.warmup):
+        nm.step(fields)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nm.step(fields)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return args.steps / (t1 - t0), (t1 - t0) / args.steps
+
+
+def time_to_tol(args, precision):
+    import torch
+    _, nm, fields = make_solver(args, precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = nm.run(fields, args.max_epochs, tol=args.tol)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ep, rms, _ = hist[-1]
+    return {'seconds': t1 - t0, 'newton_steps': ep, 'final_rms': max(rms),
+            'reached': max(rms) <= args.tol}, fields
+
+
+def kernel_bandwidth(args):
+    """Average duration of the cavity residual and fused loss-adjoint kernels
+    at n x n. `kernel_reps` launches are captured in a HIP graph and timed
+    with events on the capture stream, so host launch overhead does not enter.
+    Algorithmic bytes: residual reads u, v, p and writes fu, fv, fp (6 N B);
+    the adjoint reads u, v, p and writes three gradients (6 N B)."""
+    import torch
+    from odil_amd import _lib
+    from odil_amd.operators import Cavity
+    n = args.n
+    N = n * n
+    op = Cavity((n, n), 1.0 / args.re)
+    out = {}
+    for dt, name in ((torch.float32, 'fp32'), (torch.float64, 'fp64')):
+        g = torch.Generator(device='cuda').manual_seed(0)
+        fields = [torch.rand(n, n, dtype=dt, device='cuda', generator=g)
+                  for _ in range(3)]
+        res = [torch.empty_like(f) for f in fields]
+        grads = [torch.empty_like(f) for f in fields]
+        dtag = _lib.dtype_tag(dt)
+        B = 4 if dt == torch.float32 else 8
+        for kind in ('residual', 'adjoint'):
+            k = 0 if kind == 'residual' else 1
+            partial = torch.empty((3, op.nblocks(k, dtag)),
+                                  dtype=torch.float64, device='cuda')
+
+            def call():
+                if kind == 'residual':
+                    op.launch_residual(fields, res, partial)
+                else:
+                    op.launch_loss_grad(fields, grads, partial)
+            for _ in range(3):
+                call()
+            torch.cuda.synchronize()
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                for _ in range(args.kernel_reps):
+                    call()
+            graph.replay()
+            torch.cuda.synchronize()
+            s = torch.cuda.current_stream()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            graph.replay()
+            e1.record(s)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / args.kernel_reps
+            nbytes = 6 * N * B
+            out['{}_{}'.format(kind, name)] = {
+                'ms': ms, 'GBps': nbytes / (ms * 1e-3) / 1e9,
+                'bytes': nbytes}
+    return out
+
+
+def cpu_baseline(args):
+    """One Newton step of the same algorithm in the CPU oracle (NumPy/SciPy,
+    float64) on a bounded sample: the cavity at cpu_n x cpu_n with the same
+    level count rule and V-cycle count, scaled to 1025^2 by cell count (the
+    multigrid step is linear in the number of cells)."""
+    import numpy as np
+    import torch
+    torch.set_num_threads(1)
+    from oracle import multigrid as omg
+    from oracle import operators as ops
+    from oracle import tape
+    n = args.cpu_n
+    lv = len(omg.MultigridOp((n, n)).nnw)
+    st = {k: np.zeros((n, n)) for k in 'uvp'}
+    t0 = time.perf_counter()
+    const, grads, desc = tape.eval_grad(ops.make_cavity_operator(1 / args.re),
+                                        ops.cavity_domain(n), st)
+    stf = omg.coeffs_from_tape(grads, desc, 3, 3)
+    mm, matr = omg.normal_operator(stf, (n, n), 3)
+    rhs = [-sum(mm[i][j].mul_transpose_field(const[i].ravel())
+                for i in range(3)) for j in range(3)]
+    M = omg.MultigridOp((n, n), nvar=3, nlevels=min(lv, args.nlvl),
+                        galerkin_mode='exact', coarse_reg=1e-14)
+    M.update_A(matr)
+    sol = [np.zeros(n * n) for _ in range(3)]
+    for _ in range(args.maxiter):
+        sol = M.step(sol, rhs, args.omega, ndirect=3)
+    t1 = time.perf_counter()
+    sec = (t1 - t0) * (args.n**2) / (n * n)
+    return {'value': 1.0 / sec, 'unit': 'Newton steps/s', 'cores': 1,
+            'kind': 'port',
+            'sample': ('one oracle Newton step (tape Jacobian, stencil J^T J, '
+                       'exact Galerkin, {} V(2,2) cycles) at {}x{}, {:.2f} s, '
+                       'scaled to {}x{} by cell count').format(
+                           args.maxiter, n, n, t1 - t0, args.n, args.n)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only (this build has no multi-GPU '
+              'path); got --gpus {}'.format(args.gpus), file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print('no GPU visible', file=sys.stderr)
+        sys.exit(1)
+    sps_mixed, ms_mixed = steps_per_second(args, 'mixed')
+    sps_64, ms_64 = steps_per_second(args, 'fp64')
+    tol = {}
+    if not args.no_tol_run:
+        tol['mixed'], f_mixed = time_to_tol(args, 'mixed')
+        tol['fp64'], f_64 = time_to_tol(args, 'fp64')
+        # Error of the mixed-precision solution against the FP64 one.
+        d = [torch.max(torch.abs(a - b)).item()
+             for a, b in zip(f_mixed[:2], f_64[:2])]
+        pm = f_mixed[2] - f_mixed[2].mean()
+        p6 = f_64[2] - f_64[2].mean()
+        d.append(torch.max(torch.abs(pm - p6)).item())
+        tol['mixed_vs_fp64_maxabs'] = {'u': d[0], 'v': d[1], 'p': d[2]}
+    kb = kernel_bandwidth(args)
+    dom = kb['adjoint_fp32']
+    line = {
+        'metric': 'Newton steps/s + residual/adjoint GB/s at 1024^2, '
+                  'mixed vs FP64, same tol',
+        'value': sps_mixed,
+        'unit': 'Newton steps/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms_mixed * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'fp32+fp64',
+        'data': 'synthetic',
+        'config': {
+            'workload': 'lid-driven cavity {0}x{0}, Re={1:g}, Newton with '
+                        'GMG normal-equation solve, {2} levels, {3} V(2,2) '
+                        'cycles/step, omega={4}'.format(
+                            args.n, args.re, args.nlvl, args.maxiter,
+                            args.omega),
+            'grid': [args.n, args.n],
+            'parallelism': 'single GPU'},
+        'fp64_steps_per_s': sps_64,
+        'fp64_ms_per_step': ms_64 * 1e3,
+        'mixed_over_fp64_step_speedup': sps_mixed / sps_64,
+        'time_to_tol': tol,
+        'kernels': kb,
+        'roofline': {
+            'bound': 'hbm', 'kernel': 'k_loss_grad<float, CavityOp> '
+                                      '(fused residual + adjoint, fp32)',
+            'achieved': dom['GBps'], 'peak': HBM_PEAK_GBPS, 'unit': 'GB/s',
+            'frac': dom['GBps'] / HBM_PEAK_GBPS, 'traffic': None},
+    }
+    if 'mixed' in tol and 'fp64' in tol:
+        line['time_to_tol_speedup'] = (tol['fp64']['seconds'] /
+                                       tol['mixed']['seconds'])
+    if not args.no_cpu:
+        line['cpu_baseline'] = cpu_baseline(args)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

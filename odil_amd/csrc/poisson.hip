@@ -1,0 +1,291 @@
+// Laplace-type operators on a 2D cell-centred grid: residual, stencil
+// Jacobian and fused loss adjoint.
+//
+//  Poisson : f = lap(u) - src, Dirichlet u = 0 on the walls with halos from
+//            quadratic extrapolation through the wall value (the no-slip
+//            halo rule of ns_steady.py:219-225 applied to a scalar).
+//  Laplace : (lap(u), lap(v), 0) with linear-extrapolation halos
+//            (operator_laplace, util_op.py:1038-1079).
+#include "stencil2d.h"
+
+namespace odil {
+
+struct PoissonParams {
+  double dx, dy;
+  const void* src;  // same dtype as the fields, [n0][n1]
+};
+
+template <class T>
+__device__ inline void lap5(T c, T rxm, T rxp, T rym, T ryp, int i, int j,
+                            int n0, int n1, int kind, T dx, T dy, T* out,
+                            T* coef) {
+  const T xm = (i == 0) ? ghost(kind, rxp, c, T(0)) : rxm;
+  const T xp = (i == n0 - 1) ? ghost(kind, rxm, c, T(0)) : rxp;
+  const T ym = (j == 0) ? ghost(kind, ryp, c, T(0)) : rym;
+  const T yp = (j == n1 - 1) ? ghost(kind, rym, c, T(0)) : ryp;
+  *out = (xp - T(2) * c + xm) / (dx * dx) + (yp - T(2) * c + ym) / (dy * dy);
+  if (!coef) return;
+  const T ix2 = T(1) / (dx * dx), iy2 = T(1) / (dy * dy);
+  T e[5] = {-T(2) * ix2 - T(2) * iy2, ix2, ix2, iy2, iy2};
+  coef[0] = e[0];
+  coef[1] = e[1];
+  coef[2] = e[2];
+  coef[3] = e[3];
+  coef[4] = e[4];
+  if (i == 0) {
+    coef[2] += e[1] * ghost_dfar<T>(kind);
+    coef[0] += e[1] * ghost_dnear<T>(kind);
+    coef[1] = 0;
+  }
+  if (i == n0 - 1) {
+    coef[1] += e[2] * ghost_dfar<T>(kind);
+    coef[0] += e[2] * ghost_dnear<T>(kind);
+    coef[2] = 0;
+  }
+  if (j == 0) {
+    coef[4] += e[3] * ghost_dfar<T>(kind);
+    coef[0] += e[3] * ghost_dnear<T>(kind);
+    coef[3] = 0;
+  }
+  if (j == n1 - 1) {
+    coef[3] += e[4] * ghost_dfar<T>(kind);
+    coef[0] += e[4] * ghost_dnear<T>(kind);
+    coef[4] = 0;
+  }
+}
+
+template <class T>
+struct PoissonOp {
+  using Params = PoissonParams;
+  static constexpr int NF = 1, NE = 1, NS = 5, R = 1, RJ = 1;
+  static constexpr int SX[NS] = {0, -1, 1, 0, 0};
+  static constexpr int SY[NS] = {0, 0, 0, -1, 1};
+  static constexpr int NJ = 5;
+  static constexpr int JE[NJ] = {0, 0, 0, 0, 0};
+  static constexpr int JV[NJ] = {0, 0, 0, 0, 0};
+  static constexpr int JS[NJ] = {0, 1, 2, 3, 4};
+
+  template <class Acc>
+  __device__ static inline void go(const Acc& q, int i, int j, int n0,
+                                   int n1, const Params& P, T* f,
+                                   T (*J)[NF][NS]) {
+    T l;
+    lap5<T>(q(0, 0, 0), q(0, -1, 0), q(0, 1, 0), q(0, 0, -1), q(0, 0, 1), i,
+            j, n0, n1, QUADH, T(P.dx), T(P.dy), &l, J ? J[0][0] : nullptr);
+    f[0] = l - static_cast<const T*>(P.src)[(size_t)i * n1 + j];
+  }
+  template <class Acc>
+  __device__ static inline void residual(const Acc& q, int i, int j, int n0,
+                                         int n1, const Params& P, T* f) {
+    go(q, i, j, n0, n1, P, f, nullptr);
+  }
+  template <class Acc>
+  __device__ static inline void eval(const Acc& q, int i, int j, int n0,
+                                     int n1, const Params& P, T* f,
+                                     T (*J)[NF][NS]) {
+    go(q, i, j, n0, n1, P, f, J);
+  }
+};
+
+struct LaplaceParams {
+  double dx, dy;
+};
+
+template <class T>
+struct LaplaceOp {
+  using Params = LaplaceParams;
+  static constexpr int NF = 2, NE = 3, NS = 5, R = 1, RJ = 1;
+  static constexpr int SX[NS] = {0, -1, 1, 0, 0};
+  static constexpr int SY[NS] = {0, 0, 0, -1, 1};
+  static constexpr int NJ = 10;
+  static constexpr int JE[NJ] = {0, 0, 0, 0, 0, 1, 1, 1, 1, 1};
+  static constexpr int JV[NJ] = {0, 0, 0, 0, 0, 1, 1, 1, 1, 1};
+  static constexpr int JS[NJ] = {0, 1, 2, 3, 4, 0, 1, 2, 3, 4};
+
+  template <class Acc>
+  __device__ static inline void go(const Acc& q, int i, int j, int n0,
+                                   int n1, const Params& P, T* f,
+                                   T (*J)[NF][NS]) {
+    for (int k = 0; k < 2; ++k) {
+      T c[5];
+      lap5<T>(q(k, 0, 0), q(k, -1, 0), q(k, 1, 0), q(k, 0, -1), q(k, 0, 1),
+              i, j, n0, n1, LINEAR, T(P.dx), T(P.dy), &f[k], J ? c : nullptr);
+      if (J)
+        for (int s = 0; s < 5; ++s) {
+          J[k][k][s] = c[s];
+          J[k][1 - k][s] = 0;
+          J[2][k][s] = 0;
+        }
+    }
+    f[2] = 0;
+  }
+  template <class Acc>
+  __device__ static inline void residual(const Acc& q, int i, int j, int n0,
+                                         int n1, const Params& P, T* f) {
+    go(q, i, j, n0, n1, P, f, nullptr);
+  }
+  template <class Acc>
+  __device__ static inline void eval(const Acc& q, int i, int j, int n0,
+                                     int n1, const Params& P, T* f,
+                                     T (*J)[NF][NS]) {
+    go(q, i, j, n0, n1, P, f, J);
+  }
+};
+
+template <class Op>
+int entries_table(int* ent, int* shifts) {
+  for (int q = 0; q < Op::NJ; ++q) {
+    ent[3 * q] = Op::JE[q];
+    ent[3 * q + 1] = Op::JV[q];
+    ent[3 * q + 2] = Op::JS[q];
+  }
+  for (int s = 0; s < Op::NS; ++s) {
+    shifts[2 * s] = Op::SX[s];
+    shifts[2 * s + 1] = Op::SY[s];
+  }
+  return Op::NJ;
+}
+
+}  // namespace odil
+
+using namespace odil;
+
+extern "C" {
+
+// Poisson: `src` has the dtype of the fields.
+int odil_poisson_residual(int dtype, int n0, int n1, double dx, double dy,
+                          const void* src, const void* const* fields,
+                          void* const* res, double* partial, void* stream) {
+  if (n0 < 2 || n1 < 2 || !src || !fields) {
+    set_error("poisson_residual: need n >= 2, src and fields");
+    return kErrArg;
+  }
+  PoissonParams P{dx, dy, src};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    return launch_residual<double, PoissonOp<double>>(fields, res, partial, n0,
+                                                      n1, P, s, nullptr);
+  if (dtype == kF32)
+    return launch_residual<float, PoissonOp<float>>(fields, res, partial, n0,
+                                                    n1, P, s, nullptr);
+  set_error("poisson_residual: bad dtype %d", dtype);
+  return kErrArg;
+}
+
+int odil_poisson_jac(int dtype, int n0, int n1, double dx, double dy,
+                     const void* src, const void* const* fields, void* coef,
+                     void* const* res, void* stream) {
+  if (n0 < 2 || n1 < 2 || !src || !fields || !coef) {
+    set_error("poisson_jac: bad arguments");
+    return kErrArg;
+  }
+  PoissonParams P{dx, dy, src};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    return launch_jac<double, PoissonOp<double>>(fields, coef, res, n0, n1, P,
+                                                 s);
+  if (dtype == kF32)
+    return launch_jac<float, PoissonOp<float>>(fields, coef, res, n0, n1, P,
+                                               s);
+  set_error("poisson_jac: bad dtype %d", dtype);
+  return kErrArg;
+}
+
+int odil_poisson_jac_entries(int* ent, int* shifts) {
+  return entries_table<PoissonOp<double>>(ent, shifts);
+}
+
+int odil_poisson_loss_grad(int dtype, int n0, int n1, double dx, double dy,
+                           const void* src, const void* const* fields,
+                           void* const* grad, double* partial, int* nblocks,
+                           void* stream) {
+  if (n0 < 2 || n1 < 2 || !src || !fields || !grad) {
+    set_error("poisson_loss_grad: bad arguments");
+    return kErrArg;
+  }
+  PoissonParams P{dx, dy, src};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const double scale = 2.0 / ((double)n0 * n1);
+  if (dtype == kF64)
+    return launch_loss_grad_t<double, PoissonOp<double>, 8, 64>(
+        fields, grad, partial, n0, n1, scale, P, s, nblocks);
+  if (dtype == kF32)
+    return launch_loss_grad_t<float, PoissonOp<float>, 16, 64>(
+        fields, grad, partial, n0, n1, (float)scale, P, s, nblocks);
+  set_error("poisson_loss_grad: bad dtype %d", dtype);
+  return kErrArg;
+}
+
+int odil_poisson_nblocks(int kind, int dtype, int n0, int n1) {
+  if (kind == 0) return ceil_div(n1, 64) * ceil_div(n0, 4);
+  return ceil_div(n1, 64) * ceil_div(n0, dtype == kF32 ? 16 : 8);
+}
+
+int odil_laplace_residual(int dtype, int n0, int n1, double dx, double dy,
+                          const void* const* fields, void* const* res,
+                          double* partial, void* stream) {
+  if (n0 < 2 || n1 < 2 || !fields) {
+    set_error("laplace_residual: bad arguments");
+    return kErrArg;
+  }
+  LaplaceParams P{dx, dy};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    return launch_residual<double, LaplaceOp<double>>(fields, res, partial, n0,
+                                                      n1, P, s, nullptr);
+  if (dtype == kF32)
+    return launch_residual<float, LaplaceOp<float>>(fields, res, partial, n0,
+                                                    n1, P, s, nullptr);
+  set_error("laplace_residual: bad dtype %d", dtype);
+  return kErrArg;
+}
+
+int odil_laplace_jac(int dtype, int n0, int n1, double dx, double dy,
+                     const void* const* fields, void* coef, void* const* res,
+                     void* stream) {
+  if (n0 < 2 || n1 < 2 || !fields || !coef) {
+    set_error("laplace_jac: bad arguments");
+    return kErrArg;
+  }
+  LaplaceParams P{dx, dy};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    return launch_jac<double, LaplaceOp<double>>(fields, coef, res, n0, n1, P,
+                                                 s);
+  if (dtype == kF32)
+    return launch_jac<float, LaplaceOp<float>>(fields, coef, res, n0, n1, P,
+                                               s);
+  set_error("laplace_jac: bad dtype %d", dtype);
+  return kErrArg;
+}
+
+int odil_laplace_jac_entries(int* ent, int* shifts) {
+  return entries_table<LaplaceOp<double>>(ent, shifts);
+}
+
+int odil_laplace_loss_grad(int dtype, int n0, int n1, double dx, double dy,
+                           const void* const* fields, void* const* grad,
+                           double* partial, int* nblocks, void* stream) {
+  if (n0 < 2 || n1 < 2 || !fields || !grad) {
+    set_error("laplace_loss_grad: bad arguments");
+    return kErrArg;
+  }
+  LaplaceParams P{dx, dy};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const double scale = 2.0 / ((double)n0 * n1);
+  if (dtype == kF64)
+    return launch_loss_grad_t<double, LaplaceOp<double>, 8, 64>(
+        fields, grad, partial, n0, n1, scale, P, s, nblocks);
+  if (dtype == kF32)
+    return launch_loss_grad_t<float, LaplaceOp<float>, 16, 64>(
+        fields, grad, partial, n0, n1, (float)scale, P, s, nblocks);
+  set_error("laplace_loss_grad: bad dtype %d", dtype);
+  return kErrArg;
+}
+
+int odil_laplace_nblocks(int kind, int dtype, int n0, int n1) {
+  if (kind == 0) return ceil_div(n1, 64) * ceil_div(n0, 4);
+  return ceil_div(n1, 64) * ceil_div(n0, dtype == kF32 ? 16 : 8);
+}
+
+}  // extern "C"

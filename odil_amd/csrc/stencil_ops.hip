@@ -45,37 +45,53 @@ __device__ inline bool inside(int a, int b, int c, const Grid3& g) {
 // mode 0: y = A x; mode 1: y = f - A x;
 // mode 2: weighted Jacobi y = x + omega (f - A x) / diag(A)
 //         (smoother_jacobi, multigrid.py:543-552; diag = entry dent[o]).
-template <class T>
+// A group of LANES consecutive threads owns one (output field o, cell c)
+// pair, cells fastest so coefficient reads coalesce; the lanes split the
+// entries of o and combine by shuffles. Small grids use wide groups so the
+// serial chain of dependent loads per thread stays short. Loads use clamped
+// in-grid indices and a 0/1 weight instead of branches so they can all be
+// issued back to back.
+template <class T, int LANES>
 __global__ void __launch_bounds__(256)
     k_apply(const int* __restrict__ ent, const int* __restrict__ ostart,
             const int* __restrict__ dent, int nout, const T* __restrict__ coef,
             CVecPtrs<T> x, VecPtrs<T> y, CVecPtrs<T> f, Grid3 nw, Grid3 nwh,
             int stride, int mode, T omega) {
-  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t gid = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
+  const int lane = threadIdx.x % LANES;
   const size_t Nh = nwh.size();
-  if (c >= Nh) return;
+  if (gid >= Nh * (size_t)nout) return;  // whole groups exit together
+  const int o = (int)(gid / Nh);
+  const size_t c = gid - (size_t)o * Nh;
   int i0, i1, i2;
   unflat(c, nwh, i0, i1, i2);
   const int b0 = stride * i0, b1 = stride * i1, b2 = stride * i2;
-  for (int o = 0; o < nout; ++o) {
-    T acc = 0;
-    const int e1 = ostart[o + 1];
-    for (int e = ostart[o]; e < e1; ++e) {
-      const int k = ent[5 * e + 1];
-      const int p0 = b0 + ent[5 * e + 2], p1 = b1 + ent[5 * e + 3],
-                p2 = b2 + ent[5 * e + 4];
-      if (inside(p0, p1, p2, nw))
-        acc += coef[(size_t)e * Nh + c] *
-               x.p[k][((size_t)p0 * nw.n1 + p1) * nw.n2 + p2];
-    }
-    if (mode == 0) {
-      y.p[o][c] = acc;
-    } else if (mode == 1) {
-      y.p[o][c] = f.p[o][c] - acc;
-    } else {
-      const T d = coef[(size_t)dent[o] * Nh + c];
-      y.p[o][c] = x.p[o][c] + omega * (f.p[o][c] - acc) / d;
-    }
+  T acc = 0;
+  const int e1 = ostart[o + 1];
+#pragma unroll 4
+  for (int e = ostart[o] + lane; e < e1; e += LANES) {
+    const int k = ent[5 * e + 1];
+    const int p0 = b0 + ent[5 * e + 2], p1 = b1 + ent[5 * e + 3],
+              p2 = b2 + ent[5 * e + 4];
+    const bool ok = inside(p0, p1, p2, nw);
+    const int q0 = min(max(p0, 0), nw.n0 - 1);
+    const int q1 = min(max(p1, 0), nw.n1 - 1);
+    const int q2 = min(max(p2, 0), nw.n2 - 1);
+    const T cv = coef[(size_t)e * Nh + c];
+    const T xv = x.p[k][((size_t)q0 * nw.n1 + q1) * nw.n2 + q2];
+    acc += (ok ? cv : T(0)) * xv;
+  }
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1)
+    acc += __shfl_xor(acc, off, LANES);
+  if (lane != 0) return;
+  if (mode == 0) {
+    y.p[o][c] = acc;
+  } else if (mode == 1) {
+    y.p[o][c] = f.p[o][c] - acc;
+  } else {
+    const T d = coef[(size_t)dent[o] * Nh + c];
+    y.p[o][c] = x.p[o][c] + omega * (f.p[o][c] - acc) / d;
   }
 }
 
@@ -127,23 +143,24 @@ __global__ void __launch_bounds__(256)
     k_normal(const int* __restrict__ terms, const int* __restrict__ tstart,
              int nout, const Tin* __restrict__ jc, Tout* __restrict__ out,
              Grid3 g) {
-  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // One thread per (output entry o, cell m), cells fastest.
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t N = g.size();
-  if (m >= N) return;
+  if (t >= N * (size_t)nout) return;
+  const int o = (int)(t / N);
+  const size_t m = t - (size_t)o * N;
   int m0, m1, m2;
   unflat(m, g, m0, m1, m2);
-  for (int o = 0; o < nout; ++o) {
-    double acc = 0;
-    for (int t = tstart[o]; t < tstart[o + 1]; ++t) {
-      const int q0 = m0 - terms[5 * t + 2], q1 = m1 - terms[5 * t + 3],
-                q2 = m2 - terms[5 * t + 4];
-      if (!inside(q0, q1, q2, g)) continue;
-      const size_t n = ((size_t)q0 * g.n1 + q1) * g.n2 + q2;
-      acc += (double)jc[(size_t)terms[5 * t] * N + n] *
-             (double)jc[(size_t)terms[5 * t + 1] * N + n];
-    }
-    out[(size_t)o * N + m] = (Tout)acc;
+  double acc = 0;
+  for (int q = tstart[o]; q < tstart[o + 1]; ++q) {
+    const int q0 = m0 - terms[5 * q + 2], q1 = m1 - terms[5 * q + 3],
+              q2 = m2 - terms[5 * q + 4];
+    if (!inside(q0, q1, q2, g)) continue;
+    const size_t n = ((size_t)q0 * g.n1 + q1) * g.n2 + q2;
+    acc += (double)jc[(size_t)terms[5 * q] * N + n] *
+           (double)jc[(size_t)terms[5 * q + 1] * N + n];
   }
+  out[(size_t)o * N + m] = (Tout)acc;
 }
 
 // Galerkin coarse operator on node-based grids nw = 2 nwh - 1.
@@ -153,25 +170,31 @@ __global__ void __launch_bounds__(256)
 // Output entry o = (i, j, dc); terms int32 [nterm][5] = (e, r0, r1, r2, wt_id)
 // with fine entry e of block (i, j), restriction offset r, interpolation
 // weight wtab[wt_id] = 0.5^|t| for t = r + sigma (s_e - 2 dc).
-// Restriction weights: inner coarse nodes use rw[(r0+1)*9+(r1+1)*3+(r2+1)],
-// coarse boundary nodes inject (r = 0, weight 1) (multigrid.py:525-532).
+// Restriction weights rw[(r0+1)*9+(r1+1)*3+(r2+1)] at inner coarse nodes;
+// at coarse boundary nodes rmode 0 injects (r = 0, weight 1, the reference
+// get_R, multigrid.py:525-532) and rmode 1 keeps the weights (fine points
+// outside the grid dropped), i.e. R = T^T / 2^dim.
 template <class T>
 __global__ void __launch_bounds__(256)
     k_galerkin(const int* __restrict__ fent, const int* __restrict__ oent,
                const int* __restrict__ terms, const int* __restrict__ tstart,
                int nout, const double* __restrict__ rw,
                const double* __restrict__ wtab, const T* __restrict__ fc,
-               T* __restrict__ cc, Grid3 nw, Grid3 nwh, int dim, int sigma) {
-  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+               T* __restrict__ cc, Grid3 nw, Grid3 nwh, int dim, int sigma,
+               int rmode) {
+  // One thread per (output entry o, coarse node I), nodes fastest.
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t Nc = nwh.size();
-  if (I >= Nc) return;
+  if (tid >= Nc * (size_t)nout) return;
+  const int o = (int)(tid / Nc);
+  const size_t I = tid - (size_t)o * Nc;
   int I0, I1, I2;
   unflat(I, nwh, I0, I1, I2);
   const bool inner = (I0 > 0 && I0 < nwh.n0 - 1) &&
                      (I1 > 0 && I1 < nwh.n1 - 1) &&
                      (dim < 3 || (I2 > 0 && I2 < nwh.n2 - 1));
   const size_t Nf = nw.size();
-  for (int o = 0; o < nout; ++o) {
+  {
     const int J0 = I0 + sigma * oent[5 * o + 2],
               J1 = I1 + sigma * oent[5 * o + 3],
               J2 = I2 + sigma * oent[5 * o + 4];
@@ -182,7 +205,7 @@ __global__ void __launch_bounds__(256)
         const int r0 = terms[5 * t + 1], r1 = terms[5 * t + 2],
                   r2 = terms[5 * t + 3];
         double w;
-        if (inner)
+        if (inner || rmode == 1)
           w = rw[(r0 + 1) * 9 + (r1 + 1) * 3 + (r2 + 1)];
         else
           w = (r0 == 0 && r1 == 0 && r2 == 0) ? 1.0 : 0.0;
@@ -205,7 +228,8 @@ __global__ void __launch_bounds__(256)
 template <class T>
 __global__ void __launch_bounds__(256)
     k_restrict(CVecPtrs<T> fine, VecPtrs<T> coarse, int nvar,
-               const double* __restrict__ rw, Grid3 nw, Grid3 nwh, int dim) {
+               const double* __restrict__ rw, Grid3 nw, Grid3 nwh, int dim,
+               int rmode) {
   const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= nwh.size()) return;
   int I0, I1, I2;
@@ -217,7 +241,7 @@ __global__ void __launch_bounds__(256)
   for (int k = 0; k < nvar; ++k) {
     const T* f = fine.p[k];
     T acc;
-    if (!inner) {
+    if (!inner && rmode == 0) {
       acc = f[((size_t)c0 * nw.n1 + c1) * nw.n2 + c2];
     } else {
       acc = 0;
@@ -226,7 +250,7 @@ __global__ void __launch_bounds__(256)
         for (int r1 = -1; r1 <= 1; ++r1)
           for (int r2 = r2lo; r2 <= r2hi; ++r2) {
             const double w = rw[(r0 + 1) * 9 + (r1 + 1) * 3 + (r2 + 1)];
-            if (w != 0.0)
+            if (w != 0.0 && inside(c0 + r0, c1 + r1, c2 + r2, nw))
               acc += T(w) *
                      f[((size_t)(c0 + r0) * nw.n1 + (c1 + r1)) * nw.n2 +
                        (c2 + r2)];
@@ -390,22 +414,36 @@ int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
   Grid3 gh{(g.n0 - 1) / stride + 1, (g.n1 - 1) / stride + 1,
            ndim == 3 ? (g.n2 - 1) / stride + 1 : 1};
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (dtype == kF64)
-    hipLaunchKernelGGL(k_apply<double>, dim3(nblk(gh.size())), dim3(256), 0,
-                       s, ent, ostart, dent, nvar_out,
-                       static_cast<const double*>(coef),
-                       cvp<double>(x, nvar_in), vp<double>(y, nvar_out),
-                       cvp<double>(f, nvar_out), g, gh, stride, mode, omega);
-  else if (dtype == kF32)
-    hipLaunchKernelGGL(k_apply<float>, dim3(nblk(gh.size())), dim3(256), 0,
-                       s, ent, ostart, dent, nvar_out,
-                       static_cast<const float*>(coef), cvp<float>(x, nvar_in),
-                       vp<float>(y, nvar_out), cvp<float>(f, nvar_out), g, gh,
-                       stride, mode, (float)omega);
-  else {
+  // Lanes per (field, cell): enough groups to fill the chip (~1M threads),
+  // at most 16 lanes.
+  const size_t groups = gh.size() * (size_t)nvar_out;
+  int lanes = 1;
+  while (lanes < 16 && groups * lanes < (1u << 20)) lanes *= 2;
+  if (dtype != kF64 && dtype != kF32) {
     set_error("stencil_apply: bad dtype %d", dtype);
     return kErrArg;
   }
+  dim3 grid(nblk(groups * lanes));
+#define ODIL_APPLY(T, L)                                                    \
+  hipLaunchKernelGGL((k_apply<T, L>), grid, dim3(256), 0, s, ent, ostart,   \
+                     dent, nvar_out, static_cast<const T*>(coef),           \
+                     cvp<T>(x, nvar_in), vp<T>(y, nvar_out),                \
+                     cvp<T>(f, nvar_out), g, gh, stride, mode, (T)omega)
+#define ODIL_APPLY_L(T)              \
+  switch (lanes) {                   \
+    case 1: ODIL_APPLY(T, 1); break; \
+    case 2: ODIL_APPLY(T, 2); break; \
+    case 4: ODIL_APPLY(T, 4); break; \
+    case 8: ODIL_APPLY(T, 8); break; \
+    default: ODIL_APPLY(T, 16); break; \
+  }
+  if (dtype == kF64) {
+    ODIL_APPLY_L(double);
+  } else {
+    ODIL_APPLY_L(float);
+  }
+#undef ODIL_APPLY_L
+#undef ODIL_APPLY
   return check_launch("stencil_apply");
 }
 
@@ -455,7 +493,7 @@ int odil_stencil_normal(int dtype_in, int dtype_out, int ndim, const int* nw,
   if (bad_dim(ndim, "stencil_normal")) return kErrArg;
   Grid3 g = mk_grid(ndim, nw);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  dim3 grid(nblk(g.size()));
+  dim3 grid(nblk(g.size() * (size_t)nout));
   if (dtype_in == kF64 && dtype_out == kF64)
     hipLaunchKernelGGL((k_normal<double, double>), grid, dim3(256), 0, s,
                        terms, tstart, nout, static_cast<const double*>(jc),
@@ -483,7 +521,7 @@ int odil_stencil_galerkin(int dtype, int ndim, const int* nw, const int* fent,
                           const int* oent, const int* terms,
                           const int* tstart, int nout, const double* rw,
                           const double* wtab, const void* fc, void* cc,
-                          int sigma, void* stream) {
+                          int sigma, int rmode, void* stream) {
   if (sigma != 1 && sigma != -1) {
     set_error("stencil_galerkin: sigma must be +1 or -1");
     return kErrArg;
@@ -496,17 +534,17 @@ int odil_stencil_galerkin(int dtype, int ndim, const int* nw, const int* fent,
   }
   Grid3 gh{g.n0 / 2 + 1, g.n1 / 2 + 1, ndim == 3 ? g.n2 / 2 + 1 : 1};
   hipStream_t s = static_cast<hipStream_t>(stream);
-  dim3 grid(nblk(gh.size()));
+  dim3 grid(nblk(gh.size() * (size_t)nout));
   if (dtype == kF64)
     hipLaunchKernelGGL(k_galerkin<double>, grid, dim3(256), 0, s, fent, oent,
                        terms, tstart, nout, rw, wtab,
                        static_cast<const double*>(fc), static_cast<double*>(cc),
-                       g, gh, ndim, sigma);
+                       g, gh, ndim, sigma, rmode);
   else if (dtype == kF32)
     hipLaunchKernelGGL(k_galerkin<float>, grid, dim3(256), 0, s, fent, oent,
                        terms, tstart, nout, rw, wtab,
                        static_cast<const float*>(fc), static_cast<float*>(cc),
-                       g, gh, ndim, sigma);
+                       g, gh, ndim, sigma, rmode);
   else {
     set_error("stencil_galerkin: bad dtype %d", dtype);
     return kErrArg;
@@ -515,7 +553,7 @@ int odil_stencil_galerkin(int dtype, int ndim, const int* nw, const int* fent,
 }
 
 int odil_mg_restrict(int dtype, int ndim, const int* nw, int nvar,
-                     const double* rw, const void* const* fine,
+                     const double* rw, int rmode, const void* const* fine,
                      void* const* coarse, void* stream) {
   if (bad_dim(ndim, "mg_restrict") || bad_var(nvar, "mg_restrict"))
     return kErrArg;
@@ -525,11 +563,11 @@ int odil_mg_restrict(int dtype, int ndim, const int* nw, int nvar,
   if (dtype == kF64)
     hipLaunchKernelGGL(k_restrict<double>, dim3(nblk(gh.size())), dim3(256), 0,
                        s, cvp<double>(fine, nvar), vp<double>(coarse, nvar),
-                       nvar, rw, g, gh, ndim);
+                       nvar, rw, g, gh, ndim, rmode);
   else if (dtype == kF32)
     hipLaunchKernelGGL(k_restrict<float>, dim3(nblk(gh.size())), dim3(256), 0,
                        s, cvp<float>(fine, nvar), vp<float>(coarse, nvar),
-                       nvar, rw, g, gh, ndim);
+                       nvar, rw, g, gh, ndim, rmode);
   else {
     set_error("mg_restrict: bad dtype %d", dtype);
     return kErrArg;

@@ -221,50 +221,74 @@ class BlockStencil:
                             nvar_in=jac.nvar, nvar_out=jac.neq)
 
 
-def normal_operator(J, dtype=None):
+_PLANS = {}
+
+
+def _normal_plan(J):
+    key = ('normal', tuple(J.entries), str(J.device))
+    if key not in _PLANS:
+        out = {}
+        for a, (ea, ka, sa) in enumerate(J.entries):
+            for b, (eb, kb, sb) in enumerate(J.entries):
+                if ea != eb:
+                    continue
+                d = tuple(y - x for x, y in zip(sa, sb))
+                out.setdefault((ka, kb, d), []).append((a, b) + _shift3(sa))
+        keys = sorted(out)
+        terms = []
+        tstart = [0]
+        for k in keys:
+            terms.extend(out[k])
+            tstart.append(len(terms))
+        t_terms = torch.tensor(np.array(terms, dtype=np.int32).reshape(-1, 5),
+                               device=J.device)
+        t_start = torch.tensor(np.array(tstart, dtype=np.int32),
+                               device=J.device)
+        _PLANS[key] = (keys, t_terms, t_start)
+    return _PLANS[key]
+
+
+def normal_operator(J, dtype=None, out=None):
     """N = J^T J in stencil form (the sum over equations of
     `mul_transpose_op`, util.py:577-583 / multigrid.py:1035-1071):
     N_o[m] = sum_{(a,b,sa)} J_a[m - sa] J_b[m - sa], out var = field of a,
-    in var = field of b, shift = s_b - s_a. Accumulated in FP64."""
+    in var = field of b, shift = s_b - s_a. Accumulated in FP64.
+    With `out` (a BlockStencil of the same structure) the result is written
+    into its coefficient buffer."""
     if J.stride != 1:
         raise ValueError('normal_operator needs a stride-1 Jacobian')
-    dtype = dtype or J.dtype
-    out = {}
-    for a, (ea, ka, sa) in enumerate(J.entries):
-        for b, (eb, kb, sb) in enumerate(J.entries):
-            if ea != eb:
-                continue
-            d = tuple(y - x for x, y in zip(sa, sb))
-            out.setdefault((ka, kb, d), []).append((a, b) + _shift3(sa))
-    keys = sorted(out)
-    terms = []
-    tstart = [0]
-    for k in keys:
-        terms.extend(out[k])
-        tstart.append(len(terms))
-    dev = J.device
-    t_terms = torch.tensor(np.array(terms, dtype=np.int32).reshape(-1, 5),
-                           device=dev)
-    t_start = torch.tensor(np.array(tstart, dtype=np.int32), device=dev)
-    coef = torch.empty((len(keys), ) + J.nw, dtype=dtype, device=dev)
+    keys, t_terms, t_start = _normal_plan(J)
+    if out is not None:
+        if out.entries != keys or out.nw != J.nw:
+            raise ValueError('out has a different structure')
+        coef = out.coef
+        dtype = coef.dtype
+    else:
+        dtype = dtype or J.dtype
+        coef = torch.empty((len(keys), ) + J.nw, dtype=dtype,
+                           device=J.device)
     st = _lib.lib().odil_stencil_normal(
         c_int(_lib.dtype_tag(J.dtype)), c_int(_lib.dtype_tag(dtype)),
         c_int(J.ndim), _grid_arr(J.nw), ptr(t_terms), ptr(t_start),
         c_int(len(keys)), ptr(J.coef), ptr(coef), stream())
     check(st, 'stencil_normal')
+    if out is not None:
+        return out
     return BlockStencil(coef, keys, J.nw, nvar_in=J.nvar_in,
                         nvar_out=J.nvar_in)
 
 
 def restriction_weights(restriction, dim):
     """Inner-node restriction weights indexed by offset r in {-1,0,1}^3
-    (get_R, multigrid.py:496-534)."""
+    (get_R, multigrid.py:496-534). 'transpose' has the weights of 'full' on
+    every coarse node (R = T^T / 2^dim) instead of injecting at coarse
+    boundary nodes."""
     rw = np.zeros(27)
     for r in itertools.product([-1, 0, 1], repeat=3):
         if dim == 2 and r[2] != 0:
             continue
         a = sum(abs(x) for x in r)
-        if restriction == 'full':
+        if restriction in ('full', 'transpose'):
             w = 0.5**(a + dim)
         elif restriction == 'half':
             w = 0.5 if a == 0 else (0.25 / dim if a == 1 else 0.)
@@ -276,7 +300,45 @@ def restriction_weights(restriction, dim):
     return rw
 
 
-def galerkin(A, restriction='full', rw_dev=None, mode='reference'):
+def _galerkin_plan(A, sigma):
+    key = ('galerkin', tuple(A.entries), A.nw, sigma, str(A.device))
+    if key not in _PLANS:
+        dim = A.ndim
+        ofs = list(itertools.product([-1, 0, 1], repeat=dim))
+        dcs = list(itertools.product([-2, -1, 0, 1, 2], repeat=dim))
+        blocks = {}
+        for q, (o, i, s) in enumerate(A.entries):
+            blocks.setdefault((o, i), []).append((q, s))
+        keys, terms, tstart = [], [], [0]
+        for (o, i) in sorted(blocks):
+            for dc in dcs:
+                tl = []
+                for q, s in blocks[(o, i)]:
+                    for r in ofs:
+                        t = [r[d] + sigma * (s[d] - 2 * dc[d])
+                             for d in range(dim)]
+                        if all(abs(x) <= 1 for x in t):
+                            tl.append((q, ) + _shift3(r) +
+                                      (sum(abs(x) for x in t), ))
+                if tl:
+                    keys.append((o, i, dc))
+                    terms.extend(tl)
+                    tstart.append(len(terms))
+        dev = A.device
+        t = lambda a: torch.tensor(np.array(a, dtype=np.int32), device=dev)
+        oent = t([[o, i] + list(_shift3(dc)) for o, i, dc in keys])
+        wtab = torch.tensor([1., 0.5, 0.25, 0.125], dtype=torch.float64,
+                            device=dev)
+        _PLANS[key] = (keys, oent, t(terms).reshape(-1, 5), t(tstart), wtab)
+    return _PLANS[key]
+
+
+def _rmode(restriction):
+    return 1 if restriction == 'transpose' else 0
+
+
+def galerkin(A, restriction='full', rw_dev=None, mode='reference',
+             out=None):
     """Coarse operator on the node grid nw//2+1.
 
     mode 'exact': R A T (the CSR `Multigrid.update_A`, multigrid.py:131).
@@ -284,54 +346,38 @@ def galerkin(A, restriction='full', rw_dev=None, mode='reference'):
     `coarsen_sparse_operator` (multigrid.py:390-427), which pairs the fine
     coefficient c_s(a) with node a - s; it equals reflect(R reflect(A) T)
     with reflect: c_s(x) -> c_{-s}(x), and R A T when the stencil is
-    reflection-symmetric."""
+    reflection-symmetric.
+    With `out` the result is written into its coefficient buffer."""
     if mode not in ('reference', 'exact'):
         raise ValueError('Unknown galerkin mode ' + str(mode))
-    sigma = 1 if mode == 'exact' else -1
     if A.stride != 1 or A.nvar_in != A.nvar_out:
         raise ValueError('galerkin needs a square stride-1 operator')
-    dim = A.ndim
-    ofs = list(itertools.product([-1, 0, 1], repeat=dim))
-    dcs = list(itertools.product([-2, -1, 0, 1, 2], repeat=dim))
-    blocks = {}
-    for q, (o, i, s) in enumerate(A.entries):
-        blocks.setdefault((o, i), []).append((q, s))
-    keys, terms, tstart = [], [], [0]
-    for (o, i) in sorted(blocks):
-        for dc in dcs:
-            tl = []
-            for q, s in blocks[(o, i)]:
-                for r in ofs:
-                    t = [r[d] + sigma * (s[d] - 2 * dc[d])
-                         for d in range(dim)]
-                    if all(abs(x) <= 1 for x in t):
-                        tl.append((q, ) + _shift3(r) +
-                                  (sum(abs(x) for x in t), ))
-            if tl:
-                keys.append((o, i, dc))
-                terms.extend(tl)
-                tstart.append(len(terms))
-    dev = A.device
+    sigma = 1 if mode == 'exact' else -1
+    keys, oent, terms, tstart, wtab = _galerkin_plan(A, sigma)
     nwh = tuple(n // 2 + 1 for n in A.nw)
-    t = lambda a: torch.tensor(np.array(a, dtype=np.int32), device=dev)
-    oent = t([[o, i] + list(_shift3(dc)) for o, i, dc in keys])
-    wtab = torch.tensor([1., 0.5, 0.25, 0.125], dtype=torch.float64,
-                        device=dev)
     if rw_dev is None:
-        rw_dev = torch.tensor(restriction_weights(restriction, dim),
-                              device=dev)
-    coef = torch.empty((len(keys), ) + nwh, dtype=A.dtype, device=dev)
+        rw_dev = torch.tensor(restriction_weights(restriction, A.ndim),
+                              device=A.device)
+    if out is not None:
+        if out.entries != keys or out.nw != nwh:
+            raise ValueError('out has a different structure')
+        coef = out.coef
+    else:
+        coef = torch.empty((len(keys), ) + nwh, dtype=A.dtype,
+                           device=A.device)
     st = _lib.lib().odil_stencil_galerkin(
-        c_int(_lib.dtype_tag(A.dtype)), c_int(dim), _grid_arr(A.nw),
-        ptr(A.t_ent), ptr(oent), ptr(t(terms).reshape(-1, 5)),
-        ptr(t(tstart)), c_int(len(keys)), ptr(rw_dev), ptr(wtab), ptr(A.coef),
-        ptr(coef), c_int(sigma), stream())
+        c_int(_lib.dtype_tag(A.dtype)), c_int(A.ndim), _grid_arr(A.nw),
+        ptr(A.t_ent), ptr(oent), ptr(terms), ptr(tstart), c_int(len(keys)),
+        ptr(rw_dev), ptr(wtab), ptr(A.coef), ptr(coef), c_int(sigma),
+        c_int(_rmode(restriction)), stream())
     check(st, 'stencil_galerkin')
+    if out is not None:
+        return out
     return BlockStencil(coef, keys, nwh, nvar_in=A.nvar_in,
                         nvar_out=A.nvar_out)
 
 
-def restrict(fine, nw, rw_dev, coarse=None):
+def restrict(fine, nw, rw_dev, coarse=None, restriction='full'):
     """Restriction R of a list of fields from node grid `nw` to nw//2+1."""
     nwh = tuple(n // 2 + 1 for n in nw)
     if coarse is None:
@@ -339,7 +385,8 @@ def restrict(fine, nw, rw_dev, coarse=None):
                   for f in fine]
     st = _lib.lib().odil_mg_restrict(
         c_int(_lib.dtype_tag(fine[0].dtype)), c_int(len(nw)), _grid_arr(nw),
-        c_int(len(fine)), ptr(rw_dev), ptrs(fine), ptrs(coarse), stream())
+        c_int(len(fine)), ptr(rw_dev), c_int(_rmode(restriction)),
+        ptrs(fine), ptrs(coarse), stream())
     check(st, 'mg_restrict')
     return coarse
 
@@ -373,9 +420,10 @@ class MultigridOp:
     direct solve on the coarsest level, here an FP64 dense inverse applied by
     a GEMV kernel.
 
-    `coarse_reg` adds coarse_reg * max|diag| to the coarsest diagonal before
-    inversion; use a tiny value for operators with a null space (the cavity's
-    constant pressure), 0 otherwise.
+    `coarse_reg` > 0 switches the coarsest solve from the inverse to
+    Tikhonov-regularised least squares (see `_invert`); use a tiny value
+    (e.g. 1e-14) for operators with a null space such as the cavity's
+    constant pressure, 0 otherwise.
     """
 
     def __init__(self, nw, nvar=1, nlevels=None, restriction='full',
@@ -401,25 +449,58 @@ class MultigridOp:
 
     # -- setup -----------------------------------------------------------
     def update_A(self, A0):
-        """A0: BlockStencil of the finest operator (cast to self.dtype)."""
+        """A0: BlockStencil of the finest operator (cast to self.dtype).
+
+        A second call with an operator of the same structure refills the
+        level operators and coarse inverses in place, so device addresses
+        stay fixed (a captured V-cycle graph stays valid)."""
         if A0.nw != self.nnw[0] or A0.nvar_in != self.nvar:
             raise ValueError('operator grid/fields do not match the levels')
-        AA = [A0.astype(self.dtype)]
-        for level in range(1, self.nlevels):
-            AA.append(galerkin(AA[-1], self.restriction, self.rw,
-                               self.galerkin_mode))
-        self.AA = AA
-        self._coarse_inv = {}
+        same = (self.AA is not None and self.AA[0].entries == A0.entries)
+        if not same:
+            AA = [A0.astype(self.dtype)]
+            if AA[0] is A0:
+                AA[0] = BlockStencil(A0.coef.clone(), A0.entries, A0.nw,
+                                     nvar_in=A0.nvar_in,
+                                     nvar_out=A0.nvar_out)
+            for level in range(1, self.nlevels):
+                AA.append(galerkin(AA[-1], self.restriction, self.rw,
+                                   self.galerkin_mode))
+            self.AA = AA
+            self._coarse_inv = {}
+        else:
+            self.AA[0].coef.copy_(A0.coef)
+            for level in range(1, self.nlevels):
+                galerkin(self.AA[level - 1], self.restriction, self.rw,
+                         self.galerkin_mode, out=self.AA[level])
+            for level in list(self._coarse_inv):
+                self._coarse_inv[level].copy_(self._invert(level))
         return self
+
+    def _invert(self, level):
+        """Dense FP64 solution operator of the coarsest level.
+
+        coarse_reg == 0: the inverse. coarse_reg > 0: the Tikhonov
+        least-squares operator (M^T M + d I)^-1 M^T with
+        d = coarse_reg * max diag(M^T M). Singular coarse systems (the
+        cavity's constant pressure) are also inconsistent under the
+        reference restriction (injection rows at coarse boundary nodes), so
+        an inverse would amplify the incompatible part; least squares
+        discards it, like the minimum-norm solution."""
+        M = self.AA[level].to_dense()
+        if not self.coarse_reg:
+            # LAPACK-style results may come back column-major; the GEMV
+            # kernel reads row-major.
+            return torch.linalg.inv(M).contiguous()
+        G = M.T @ M
+        d = self.coarse_reg * G.diagonal().max()
+        G.diagonal().add_(d)
+        L = torch.linalg.cholesky(G)
+        return torch.cholesky_solve(M.T.contiguous(), L).contiguous()
 
     def _coarse(self, level):
         if level not in self._coarse_inv:
-            M = self.AA[level].to_dense()
-            if self.coarse_reg:
-                s = M.diagonal().abs().max()
-                M = M + torch.eye(M.shape[0], dtype=M.dtype,
-                                  device=M.device) * (self.coarse_reg * s)
-            self._coarse_inv[level] = torch.linalg.inv(M)
+            self._coarse_inv[level] = self._invert(level)
         return self._coarse_inv[level]
 
     def prepare(self, ndirect=5):
@@ -468,7 +549,8 @@ class MultigridOp:
             u = self.smoother_jacobi(level, u, f, omega)
         A = self.AA[level]
         r = A.residual(u, f)
-        rc = restrict(r, self.nnw[level], self.rw)
+        rc = restrict(r, self.nnw[level], self.rw,
+                      restriction=self.restriction)
         du = self.step([torch.zeros_like(x) for x in rc], rc, omega,
                        level=level + 1)
         u = [x.clone() for x in u]
@@ -520,7 +602,8 @@ class MultigridOp:
         r = buf[level]['r']
         A.residual(cur, f, r)
         nxt = buf[level + 1]
-        restrict(r, self.nnw[level], self.rw, nxt['f'])
+        restrict(r, self.nnw[level], self.rw, nxt['f'],
+                 restriction=self.restriction)
         for x in nxt['u']:
             x.zero_()
         res = self._vc(level + 1, nxt['u'], nxt['f'], nxt['v'], omega, 5, 2,

@@ -83,6 +83,28 @@ class Operator:
         return self._fn('nblocks')(c_int(kind), c_int(dtag),
                                    *self._shape_args())
 
+    # -- raw launches on preallocated buffers (no allocation, capturable) --
+    def launch_residual(self, fields, res, partial):
+        """Residual kernel only: `res` fields (or None) and per-block sums of
+        squares into `partial` ((neq, nblocks(0)) float64)."""
+        dtag = self._check_fields(fields)
+        st = self._fn('residual')(c_int(dtag), *self._shape_args(),
+                                  *self._args(), ptrs(fields),
+                                  ptrs(res) if res else None, ptr(partial),
+                                  _lib.stream())
+        check(st, self.prefix + '_residual')
+
+    def launch_loss_grad(self, fields, grads, partial):
+        """Fused loss-adjoint kernel only; `partial` is
+        (neq, nblocks(1)) float64."""
+        dtag = self._check_fields(fields)
+        nbo = c_int(0)
+        st = self._fn('loss_grad')(c_int(dtag), *self._shape_args(),
+                                   *self._args(), ptrs(fields), ptrs(grads),
+                                   ptr(partial), ctypes.byref(nbo),
+                                   _lib.stream())
+        check(st, self.prefix + '_loss_grad')
+
     # -- API -------------------------------------------------------------
     def residual(self, fields, want_fields=True):
         """Returns (list of residual fields or None, ssq) where `ssq` is a
@@ -193,3 +215,156 @@ class Cavity(Operator):
     def _args(self):
         return [c_double(self.dx), c_double(self.dy), c_double(self.mu),
                 c_int(self.bc), c_int(self.rhiechow)]
+
+
+def _dev64(a, device):
+    return torch.as_tensor(np.asarray(a, dtype=np.float64),
+                           device=device).contiguous()
+
+
+class Wave(Operator):
+    """1D wave equation u_tt = u_xx on a (nt, nx) space-time grid over
+    t in [lower[0], upper[0]], x in [lower[1], upper[1]] (wave.py:67-122).
+    `init_u`, `init_ut` are arrays over x at t = lower[0]; `left_u`,
+    `right_u` boundary values over t."""
+
+    fieldnames = ('u', )
+    neq = 1
+    prefix = 'wave'
+
+    def __init__(self, shape, init_u, init_ut, left_u, right_u,
+                 lower=(0., -1.), upper=(1., 1.), device=None):
+        super().__init__(shape)
+        nt, nx = self.shape
+        if nt < 3 or nx < 2:
+            raise ValueError('wave needs nt >= 3 and nx >= 2')
+        device = device or torch.device('cuda')
+        self.dt = (upper[0] - lower[0]) / nt
+        self.dx = (upper[1] - lower[1]) / nx
+        self.bnd = [_dev64(init_u, device), _dev64(init_ut, device),
+                    _dev64(left_u, device), _dev64(right_u, device)]
+        for a, n in zip(self.bnd, (nx, nx, nt, nt)):
+            if a.numel() != n:
+                raise ValueError('boundary array of size {} != {}'.format(
+                    a.numel(), n))
+
+    def _args(self):
+        return [c_double(self.dt), c_double(self.dx)] + [
+            ptr(a) for a in self.bnd]
+
+
+class Poisson(Operator):
+    """f = lap(u) - src on an (n0, n1) grid over the unit square (or
+    [lower, upper]) with Dirichlet u = 0 walls."""
+
+    fieldnames = ('u', )
+    neq = 1
+    prefix = 'poisson'
+
+    def __init__(self, shape, src, lower=(0., 0.), upper=(1., 1.)):
+        super().__init__(shape)
+        self.src = src
+        _lib.require_device(src)
+        if tuple(src.shape) != self.shape:
+            raise ValueError('src shape {} != {}'.format(tuple(src.shape),
+                                                         self.shape))
+        self.dx = (upper[0] - lower[0]) / self.shape[0]
+        self.dy = (upper[1] - lower[1]) / self.shape[1]
+
+    def _args(self):
+        return [c_double(self.dx), c_double(self.dy), ptr(self.src)]
+
+    def _check_fields(self, fields):
+        d = super()._check_fields(fields)
+        if fields[0].dtype != self.src.dtype:
+            raise ValueError('src dtype must match the fields')
+        return d
+
+
+class Laplace(Operator):
+    """(lap u, lap v, 0) with linear-extrapolation halos
+    (operator_laplace, util_op.py:1038-1079)."""
+
+    fieldnames = ('u', 'v')
+    neq = 3
+    prefix = 'laplace'
+
+    def __init__(self, shape, lower=(0., 0.), upper=(1., 1.)):
+        super().__init__(shape)
+        self.dx = (upper[0] - lower[0]) / self.shape[0]
+        self.dy = (upper[1] - lower[1]) / self.shape[1]
+
+    def _args(self):
+        return [c_double(self.dx), c_double(self.dy)]
+
+
+class Advection:
+    """p_t + a . grad p = 0 on a (nt, nx, ny) space-time grid, t in
+    [0, tmax], x, y in [0, 1], periodic in x and y, constant velocity
+    (ax, ay), initial field `p_init` (advection_2d.py:103-223)."""
+
+    fieldnames = ('p', )
+    neq = 1
+
+    def __init__(self, shape, ax, ay, p_init, tmax=1.):
+        self.shape = tuple(int(s) for s in shape)
+        self.ncell = int(np.prod(self.shape))
+        nt, nx, ny = self.shape
+        self.ax, self.ay = float(ax), float(ay)
+        self.dt = tmax / nt
+        self.dx = 1. / nx
+        self.dy = 1. / ny
+        _lib.require_device(p_init)
+        if tuple(p_init.shape) != (nx, ny):
+            raise ValueError('p_init must have shape (nx, ny)')
+        self.p_init = p_init
+
+    def _args(self, dtag):
+        return [c_int(dtag)] + [c_int(s) for s in self.shape] + [
+            c_double(self.dt), c_double(self.dx), c_double(self.dy),
+            c_double(self.ax), c_double(self.ay), ptr(self.p_init)]
+
+    def _check(self, fields):
+        (p, ) = fields
+        _lib.require_device(p)
+        if tuple(p.shape) != self.shape or p.dtype != self.p_init.dtype:
+            raise ValueError('field must have shape {} and dtype {}'.format(
+                self.shape, self.p_init.dtype))
+        return _lib.dtype_tag(p.dtype)
+
+    def residual(self, fields, want_fields=True):
+        dtag = self._check(fields)
+        p = fields[0]
+        res = torch.empty_like(p) if want_fields else None
+        nb = _lib.lib().odil_advection_nblocks(*[c_int(s)
+                                                 for s in self.shape])
+        partial = torch.empty((1, nb), dtype=torch.float64, device=p.device)
+        check(_lib.lib().odil_advection_residual(*self._args(dtag), ptr(p),
+                                                 ptr(res), ptr(partial),
+                                                 _lib.stream()),
+              'advection_residual')
+        ssq = torch.empty(1, dtype=torch.float64, device=p.device)
+        check(_lib.lib().odil_reduce_partials(ptr(partial), c_int(1),
+                                              c_int(nb), ptr(ssq),
+                                              _lib.stream()),
+              'reduce_partials')
+        return ([res] if want_fields else None), ssq
+
+    def loss_grad(self, fields, grads=None):
+        dtag = self._check(fields)
+        p = fields[0]
+        g = grads[0] if grads is not None else torch.empty_like(p)
+        nb = _lib.lib().odil_advection_nblocks(*[c_int(s)
+                                                 for s in self.shape])
+        partial = torch.empty((1, nb), dtype=torch.float64, device=p.device)
+        check(_lib.lib().odil_advection_loss_grad(*self._args(dtag), ptr(p),
+                                                  ptr(g), ptr(partial),
+                                                  _lib.stream()),
+              'advection_loss_grad')
+        ssq = torch.empty(1, dtype=torch.float64, device=p.device)
+        check(_lib.lib().odil_reduce_partials(ptr(partial), c_int(1),
+                                              c_int(nb), ptr(ssq),
+                                              _lib.stream()),
+              'reduce_partials')
+        split = ssq / self.ncell
+        return split.sum(), [g], split

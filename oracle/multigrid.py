@@ -125,7 +125,7 @@ def get_R(nw, restriction='full'):
     nw = np.array(nw)
     dim = len(nw)
     nwh = nw // 2 + 1
-    if restriction == 'full':
+    if restriction in ('full', 'transpose'):
         ddw = list(itertools.product([-1, 0, 1], repeat=dim))
         coeff = lambda dw: 0.5**(sum(abs(x) for x in dw) + dim)
     elif restriction == 'half':
@@ -145,6 +145,9 @@ def get_R(nw, restriction='full'):
         c = np.full(nwh, 1. if all(x == 0 for x in dw) else 0.)
         inner = tuple(slice(1, -1) for _ in range(dim))
         c[inner] = coeff(dw)
+        if restriction == 'transpose':
+            # Same weights on every coarse node: R = T^T / 2^dim.
+            c[...] = coeff(dw)
         coeffs[dw] = c
     return StencilOp(coeffs, tuple(nw), stride=2)
 
@@ -189,7 +192,7 @@ class MultigridOp:
     (`multigrid.py:342-602`)."""
 
     def __init__(self, nw, nvar=1, nlevels=None, restriction='full',
-                 galerkin_mode='reference'):
+                 galerkin_mode='reference', coarse_reg=None):
         nw = np.array(nw)
         max_nlevels = 0
         while 2**(max_nlevels + 1) < min(nw):
@@ -203,8 +206,10 @@ class MultigridOp:
         self.nlevels = nlevels
         self.nvar = nvar
         self.RR = [get_R(n, restriction) for n in self.nnw]
+        self.restriction = restriction
         self.TT = [get_TT(n) for n in self.nnw]
         self.galerkin_mode = galerkin_mode
+        self.coarse_reg = coarse_reg
 
     def update_A(self, A0):
         AA = [A0]
@@ -244,7 +249,14 @@ class MultigridOp:
         m = scipy.sparse.bmat([[A[i][j].tocsr() for j in range(self.nvar)]
                                for i in range(self.nvar)])
         b = np.concatenate(f)
-        if m.shape[0] <= 4096:
+        if self.coarse_reg:
+            # Tikhonov least squares (M^T M + d I)^-1 M^T b with
+            # d = coarse_reg * max diag(M^T M).
+            md = m.toarray()
+            g = md.T @ md
+            g[np.diag_indices_from(g)] += self.coarse_reg * g.diagonal().max()
+            x = np.linalg.solve(g, md.T @ b)
+        elif m.shape[0] <= 4096:
             x = np.linalg.lstsq(m.toarray(), b, rcond=1e-13)[0]
         else:
             x = scipy.sparse.linalg.spsolve(m.tocsc(), b)

@@ -1,0 +1,275 @@
+"""Parity of the HIP stencil / multigrid kernels with the CPU oracle."""
+
+import itertools
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse
+import torch
+
+from oracle import multigrid as omg
+from oracle import operators as ops
+from oracle import tape
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), 'golden')
+
+
+def _rand_stencil(dev, dtype, nw, stride, nin, nout, seed, density=0.6):
+    from odil_amd.multigrid import BlockStencil
+    rng = np.random.default_rng(seed)
+    dim = len(nw)
+    nwh = tuple((n - 1) // stride + 1 for n in nw)
+    shifts = list(itertools.product([-1, 0, 1], repeat=dim))
+    ent = []
+    for o in range(nout):
+        for i in range(nin):
+            for s in shifts:
+                if s == (0, ) * dim and o == i or rng.random() < density:
+                    ent.append((o, i, s))
+    coef = rng.uniform(-1, 1, (len(ent), ) + nwh)
+    for q, (o, i, s) in enumerate(ent):
+        if o == i and s == (0, ) * dim:
+            coef[q] += 4.  # keep Jacobi diagonals away from zero
+    return BlockStencil(torch.tensor(coef, dtype=dtype, device=dev), ent, nw,
+                        stride, nin, nout)
+
+
+def _vec(dev, dtype, n, shape, seed):
+    rng = np.random.default_rng(seed)
+    return [torch.tensor(rng.uniform(-1, 1, shape), dtype=dtype, device=dev)
+            for _ in range(n)]
+
+
+def _flat(xs):
+    return np.concatenate([x.double().cpu().numpy().ravel() for x in xs])
+
+
+CFG = [((13, 10), 1, 2, 3), ((13, 9), 2, 3, 1), ((7, 6, 5), 1, 1, 2),
+       ((9, 7, 5), 2, 2, 2)]
+
+
+@pytest.mark.parametrize('nw,stride,nin,nout', CFG)
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_apply_and_transpose(dev, nw, stride, nin, nout, dtype):
+    A = _rand_stencil(dev, dtype, nw, stride, nin, nout, seed=len(nw) + nin)
+    M = A.tocsr()
+    x = _vec(dev, dtype, nin, nw, 1)
+    tol = 1e-13 if dtype == torch.float64 else 1e-5
+    y = A.apply(x)
+    ref = M @ _flat(x)
+    assert np.abs(_flat(y) - ref).max() <= tol * np.abs(ref).max() * 10
+    xt = _vec(dev, dtype, nout, A.nwh, 2)
+    yt = A.apply_t(xt)
+    ref = M.T @ _flat(xt)
+    assert np.abs(_flat(yt) - ref).max() <= tol * np.abs(ref).max() * 10
+    if stride == 1 and nin == nout:
+        f = _vec(dev, dtype, nout, nw, 3)
+        r = A.residual(x, f)
+        ref = _flat(f) - M @ _flat(x)
+        assert np.abs(_flat(r) - ref).max() <= tol * 10 * np.abs(ref).max()
+        y = [torch.empty_like(a) for a in x]
+        A.jacobi(x, f, y, 0.6)
+        d = M.diagonal()
+        ref = _flat(x) + 0.6 * (_flat(f) - M @ _flat(x)) / d
+        assert np.abs(_flat(y) - ref).max() <= tol * 10 * np.abs(ref).max()
+
+
+def _cavity_jac(dev, n0, n1, seed, dtype=torch.float64):
+    from odil_amd.multigrid import BlockStencil
+    from odil_amd.operators import Cavity
+    rng = np.random.default_rng(seed)
+    st = {k: rng.uniform(-0.5, 0.5, (n0, n1)) for k in 'uvp'}
+    op = Cavity((n0, n1), 0.01)
+    f = [torch.tensor(st[k], dtype=dtype, device=dev) for k in 'uvp']
+    jac, res = op.jacobian(f, want_residual=True)
+    return st, BlockStencil.from_jacobian(jac), res
+
+
+def test_normal_operator(dev):
+    from odil_amd.multigrid import normal_operator
+    _, J, _ = _cavity_jac(dev, 17, 19, 1)
+    N = normal_operator(J)
+    Jc = J.tocsr()
+    ref = (Jc.T @ Jc).tocsr()
+    got = N.tocsr()
+    assert abs(got - ref).max() <= 1e-12 * abs(ref).max()
+    N32 = normal_operator(J, torch.float32)
+    assert abs(N32.tocsr() - ref).max() <= 1e-6 * abs(ref).max()
+
+
+@pytest.mark.parametrize('mode,restriction',
+                         [('exact', 'full'), ('reference', 'full'),
+                          ('exact', 'transpose')])
+def test_galerkin_cavity(dev, mode, restriction):
+    from odil_amd.multigrid import galerkin, normal_operator
+    _, J, _ = _cavity_jac(dev, 17, 17, 2)
+    N = normal_operator(J)
+    Nc = galerkin(N, restriction=restriction, mode=mode)
+    # oracle: per-block stencil ops
+    stf = {}
+    for q, (o, i, s) in enumerate(N.entries):
+        stf.setdefault((o, i), {})[s] = N.coef[q].cpu().numpy()
+    R, TT = omg.get_R((17, 17), restriction), omg.get_TT((17, 17))
+    ref = scipy.sparse.bmat([[
+        omg.galerkin(omg.StencilOp(stf.get((o, i), {}), (17, 17)), R, TT,
+                     mode).tocsr() for i in range(3)] for o in range(3)])
+    got = Nc.tocsr()
+    assert abs(got - ref).max() <= 1e-12 * abs(ref).max()
+
+
+def test_galerkin_3d(dev):
+    from odil_amd.multigrid import galerkin
+    A = _rand_stencil(dev, torch.float64, (9, 9, 5), 1, 1, 1, seed=9)
+    R, TT = omg.get_R((9, 9, 5)), omg.get_TT((9, 9, 5))
+    stf = {s: A.coef[q].cpu().numpy() for q, (_, _, s) in enumerate(A.entries)}
+    for mode in ('exact', 'reference'):
+        ref = omg.galerkin(omg.StencilOp(stf, (9, 9, 5)), R, TT, mode).tocsr()
+        got = galerkin(A, mode=mode).tocsr()
+        assert abs(got - ref).max() <= 1e-12 * abs(ref).max()
+
+
+@pytest.mark.parametrize('nw', [(17, 33), (9, 9, 17)])
+@pytest.mark.parametrize('restriction', ['full', 'transpose'])
+def test_restrict_prolong(dev, nw, restriction):
+    from odil_amd.multigrid import prolong_add, restrict, restriction_weights
+    rw = torch.tensor(restriction_weights(restriction, len(nw)), device=dev)
+    x = _vec(dev, torch.float64, 2, nw, 4)
+    xc = restrict(x, nw, rw, restriction=restriction)
+    R = omg.get_R(nw, restriction).tocsr()
+    for k in range(2):
+        ref = R @ x[k].cpu().numpy().ravel()
+        assert np.abs(xc[k].cpu().numpy().ravel() - ref).max() < 1e-14
+    nwh = tuple(n // 2 + 1 for n in nw)
+    e = _vec(dev, torch.float64, 2, nwh, 5)
+    u = [a.clone() for a in x]
+    prolong_add(e, u, nw)
+    T = omg.get_TT(nw).tocsr().T
+    for k in range(2):
+        ref = x[k].cpu().numpy().ravel() + T @ e[k].cpu().numpy().ravel()
+        assert np.abs(u[k].cpu().numpy().ravel() - ref).max() < 1e-14
+
+
+def _poisson17(dev, dtype=torch.float64):
+    from odil_amd.multigrid import BlockStencil
+    g = np.load(os.path.join(GOLD, 'mg_vcycle.npz'))
+    keys = [(0, 0), (1, 0), (-1, 0), (0, 1), (0, -1)]
+    coef = np.stack([g['poisson17_c_{}_{}'.format(*k)] for k in keys])
+    A = BlockStencil(torch.tensor(coef, dtype=dtype, device=dev),
+                     [(0, 0, k) for k in keys], (17, 17))
+    return g, A
+
+
+def test_coarse_direct_solve(dev):
+    """The coarsest-level solve returns x with A x = f (FP64 inverse)."""
+    from odil_amd.multigrid import MultigridOp
+    A = _rand_stencil(dev, torch.float64, (5, 7), 1, 2, 2, seed=11)
+    mg = MultigridOp((5, 7), nvar=2, nlevels=1, device=dev).update_A(A)
+    f = _vec(dev, torch.float64, 2, (5, 7), 12)
+    x = mg.smoother_direct(0, None, f)
+    r = A.residual(x, f)
+    assert max(a.abs().max().item() for a in r) < 1e-12
+
+
+def test_vcycle_poisson_golden(dev):
+    from odil_amd.multigrid import MultigridOp
+    g, A = _poisson17(dev)
+    mg = MultigridOp((17, 17), nvar=1, device=dev).update_A(A)
+    rhs = [torch.tensor(g['poisson17_rhs'].reshape(17, 17), device=dev)]
+    u = [torch.zeros(17, 17, dtype=torch.float64, device=dev)]
+    u2 = [torch.zeros(17, 17, dtype=torch.float64, device=dev)]
+    for it in range(3):
+        u = mg.step(u, rhs, omega=0.7)
+        mg.vcycle(u2, rhs, omega=0.7)
+        ref = g['poisson17_u{}'.format(it + 1)]
+        for a in (u[0], u2[0]):
+            err = np.abs(a.cpu().numpy().ravel() - ref).max()
+            assert err <= 1e-12 * np.abs(ref).max()
+
+
+def test_vcycle_cavity_golden(dev):
+    """Reference V-cycle iterates on the cavity normal equations (17x17, 4
+    levels, reference Galerkin). u and v match; p up to its null constant."""
+    from odil_amd.multigrid import BlockStencil, MultigridOp, normal_operator
+    from odil_amd.operators import Cavity
+    g = np.load(os.path.join(GOLD, 'mg_vcycle.npz'))
+    st = [torch.tensor(g['cavity17_state_' + k], device=dev) for k in 'uvp']
+    jac, res = Cavity((17, 17), 0.01).jacobian(st, want_residual=True)
+    J = BlockStencil.from_jacobian(jac)
+    rhs = J.apply_t([-r for r in res])
+    np.testing.assert_allclose(
+        _flat(rhs), g['cavity17_rhs'].ravel(), rtol=0,
+        atol=1e-11 * np.abs(g['cavity17_rhs']).max())
+    mg = MultigridOp((17, 17), nvar=3, nlevels=4, device=dev,
+                     coarse_reg=1e-15).update_A(normal_operator(J))
+    sol = [torch.zeros(17, 17, dtype=torch.float64, device=dev)
+           for _ in range(3)]
+    sol2 = [x.clone() for x in sol]
+    for it in range(3):
+        sol = mg.step(sol, rhs, omega=0.6, ndirect=3)
+        mg.vcycle(sol2, rhs, omega=0.6, ndirect=3)
+        ref = g['cavity17_u{}'.format(it + 1)]
+        for got in (sol, sol2):
+            for k in range(3):
+                d = got[k].cpu().numpy().ravel() - ref[k]
+                if k == 2:
+                    d = d - d.mean()
+                assert np.abs(d).max() <= 1e-9 * np.abs(ref[k]).max()
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_poisson_convergence(dev, dtype):
+    """test_multigrid_poisson acceptance: 10 V-cycles, |rhs - A u| <= 1e-6
+    (FP32: relative 1e-5 of |rhs|)."""
+    from odil_amd.multigrid import MultigridOp
+    _, A = _poisson17(dev, dtype)
+    mg = MultigridOp((17, 17), nvar=1, device=dev, dtype=dtype).update_A(A)
+    rhs = [torch.full((17, 17), 10., dtype=dtype, device=dev)]
+    u = [torch.zeros(17, 17, dtype=dtype, device=dev)]
+    for _ in range(10):
+        mg.vcycle(u, rhs, omega=0.7)
+    r = A.residual(u, rhs)[0].double().norm().item()
+    if dtype == torch.float64:
+        assert r <= 1e-6
+    else:
+        assert r <= 1e-5 * rhs[0].double().norm().item()
+
+
+@pytest.mark.parametrize('mode', ['reference', 'exact'])
+def test_vcycle_cavity_65_matches_oracle(dev, mode):
+    """Two V-cycles on the zero-state cavity normal equations, 65x65, five
+    levels: GPU equals the oracle (u, v; p up to its constant). Both use the
+    same Tikhonov least-squares coarsest solve."""
+    from odil_amd.multigrid import BlockStencil, MultigridOp, normal_operator
+    from odil_amd.operators import Cavity
+    n = 65
+    st = {k: np.zeros((n, n)) for k in 'uvp'}
+    const, grads, desc = tape.eval_grad(ops.make_cavity_operator(0.01),
+                                        ops.cavity_domain(n), st)
+    stf = omg.coeffs_from_tape(grads, desc, 3, 3)
+    mm, matr = omg.normal_operator(stf, (n, n), 3)
+    rhs = [-sum(mm[i][j].mul_transpose_field(const[i].ravel())
+                for i in range(3)) for j in range(3)]
+    M = omg.MultigridOp((n, n), nvar=3, nlevels=5, galerkin_mode=mode,
+                        coarse_reg=1e-15)
+    M.update_A(matr)
+    fields = [torch.zeros(n, n, dtype=torch.float64, device=dev)
+              for _ in range(3)]
+    jac, res = Cavity((n, n), 0.01).jacobian(fields, want_residual=True)
+    J = BlockStencil.from_jacobian(jac)
+    b = J.apply_t([-r for r in res])
+    mg = MultigridOp((n, n), nvar=3, nlevels=5, device=dev,
+                     galerkin_mode=mode,
+                     coarse_reg=1e-15).update_A(normal_operator(J))
+    x = [torch.zeros(n, n, dtype=torch.float64, device=dev)
+         for _ in range(3)]
+    sol = [np.zeros(n * n) for _ in range(3)]
+    for it in range(2):
+        sol = M.step(sol, [r.copy() for r in rhs], 0.5, ndirect=3)
+        mg.vcycle(x, b, omega=0.5, ndirect=3)
+        for k in range(3):
+            d = x[k].cpu().numpy().ravel() - sol[k]
+            if k == 2:
+                d = d - d.mean()
+            assert np.abs(d).max() <= 1e-9 * np.abs(sol[k]).max(), (it, k)

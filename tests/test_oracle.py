@@ -244,3 +244,10 @@ def test_normal_operator_is_jtj():
     _, lin = tape.linearize(ops.make_cavity_operator(0.02),
                             ops.cavity_domain(n), st)
     assert abs(lin - J).max() < 1e-12
+
+
+def test_transpose_restriction_is_scaled_interpolation_transpose():
+    for nw in [(9, 17), (5, 9, 9)]:
+        R = omg.get_R(nw, 'transpose').tocsr()
+        T = omg.get_TT(nw).tocsr()
+        assert abs(R - T / 2**len(nw)).max() < 1e-15
