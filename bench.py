@@ -3,10 +3,12 @@
 1024^2-class grid of the reference's node-based multigrid, 2^10 + 1 cells per
 axis), mixed precision vs FP64, plus residual / adjoint kernel bandwidth.
 
-One step = one Newton iteration: residual + stencil Jacobian (FP64), right
-hand side -J^T f, normal operator J^T J, Galerkin hierarchy (7 levels, down
-to 17x17), coarsest FP64 solve operator, `maxiter` V(2,2) cycles, state
-update. Inputs start resident in HBM (zero state, Re = 100, lid u = 1).
+One step = one Newton iteration: residual and right-hand side -J^T f (FP64),
+stencil Jacobian and stored normal operator J^T J for the Galerkin hierarchy
+(7 levels, down to 17x17), coarsest FP64 least-squares operator, `maxiter`
+iterations of flexible CG on J^T J (FP64, matrix-free) preconditioned by one
+V(2,2) cycle, state update. Mixed precision runs the preconditioner in FP32.
+Inputs start resident in HBM (zero state, Re = 100, lid u = 1).
 
 Prints one JSON line (see the keys below). `--gpus N` with N > 1 is not
 supported: this build runs on one GPU.
@@ -32,8 +34,8 @@ def parse():
     p.add_argument('--warmup', type=int, default=3)
     p.add_argument('--n', type=int, default=1025)
     p.add_argument('--re', type=float, default=100.)
-    p.add_argument('--maxiter', type=int, default=20)
-    p.add_argument('--linsolver', default='vcycle', choices=['vcycle', 'pcg'])
+    p.add_argument('--maxiter', type=int, default=10)
+    p.add_argument('--linsolver', default='pcg', choices=['vcycle', 'pcg'])
     p.add_argument('--omega', type=float, default=0.5)
     p.add_argument('--nlvl', type=int, default=7)
     p.add_argument('--tol', type=float, default=1e-8)
@@ -219,10 +221,11 @@ def main():
         'data': 'synthetic',
         'config': {
             'workload': 'lid-driven cavity {0}x{0}, Re={1:g}, Newton with '
-                        'GMG normal-equation solve, {2} levels, {3} V(2,2) '
-                        'cycles/step, omega={4}'.format(
+                        'GMG normal-equation solve ({5}, {3} iterations/step, '
+                        '{2} levels, V(2,2), omega={4})'.format(
                             args.n, args.re, args.nlvl, args.maxiter,
-                            args.omega),
+                            args.omega, 'FCG+V-cycle' if args.linsolver ==
+                            'pcg' else 'V-cycles'),
             'grid': [args.n, args.n],
             'parallelism': 'single GPU'},
         'fp64_steps_per_s': sps_64,

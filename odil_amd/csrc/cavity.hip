@@ -422,4 +422,39 @@ int odil_cavity_nblocks(int kind, int dtype, int nx, int ny) {
   return ceil_div(ny, 64) * ceil_div(nx, tx);
 }
 
+// Matrix-free normal operator J^T J with J recomputed from `state`
+// (mode 0: y = N x, 1: y = b - N x, 2: Jacobi sweep with diagonal d,
+// 3: y = diag N). Fields are (u, v, p) triples of the given dtype.
+int odil_cavity_normal_apply(int dtype, int nx, int ny, double dx, double dy,
+                             double mu, int bc, int rhiechow,
+                             const void* const* state, const void* const* x,
+                             const void* const* b, const void* const* d,
+                             void* const* y, double omega, int mode,
+                             void* stream) {
+  if (nx < 3 || ny < 3) {
+    set_error("cavity_normal_apply: need nx, ny >= 3");
+    return kErrArg;
+  }
+  CavityParams P{dx, dy, mu, bc, rhiechow};
+  return dispatch_normal_apply<CavityOp<double>, CavityOp<float>>(
+      dtype, state, x, b, d, y, nx, ny, omega, mode, P,
+      static_cast<hipStream_t>(stream));
+}
+
+// out = scale * J^T f(state) plus per-block sums of f^2 (as loss_grad).
+int odil_cavity_jtf(int dtype, int nx, int ny, double dx, double dy,
+                    double mu, int bc, int rhiechow,
+                    const void* const* fields, void* const* out,
+                    double scale, double* partial, int* nblocks,
+                    void* stream) {
+  if (nx < 3 || ny < 3 || !fields || !out) {
+    set_error("cavity_jtf: bad arguments");
+    return kErrArg;
+  }
+  CavityParams P{dx, dy, mu, bc, rhiechow};
+  return dispatch_jtf<CavityOp<double>, CavityOp<float>>(
+      dtype, fields, out, partial, nx, ny, scale, P,
+      static_cast<hipStream_t>(stream), nblocks);
+}
+
 }  // extern "C"

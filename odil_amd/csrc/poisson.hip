@@ -288,4 +288,62 @@ int odil_laplace_nblocks(int kind, int dtype, int n0, int n1) {
   return ceil_div(n1, 64) * ceil_div(n0, dtype == kF32 ? 16 : 8);
 }
 
+int odil_poisson_normal_apply(int dtype, int n0, int n1, double dx, double dy,
+                              const void* src, const void* const* state,
+                              const void* const* x, const void* const* b,
+                              const void* const* d, void* const* y,
+                              double omega, int mode, void* stream) {
+  if (n0 < 2 || n1 < 2 || !src) {
+    set_error("poisson_normal_apply: bad arguments");
+    return kErrArg;
+  }
+  PoissonParams P{dx, dy, src};
+  return dispatch_normal_apply<PoissonOp<double>, PoissonOp<float>>(
+      dtype, state, x, b, d, y, n0, n1, omega, mode, P,
+      static_cast<hipStream_t>(stream));
+}
+
+int odil_poisson_jtf(int dtype, int n0, int n1, double dx, double dy,
+                     const void* src, const void* const* fields,
+                     void* const* out, double scale, double* partial,
+                     int* nblocks, void* stream) {
+  if (n0 < 2 || n1 < 2 || !src || !fields || !out) {
+    set_error("poisson_jtf: bad arguments");
+    return kErrArg;
+  }
+  PoissonParams P{dx, dy, src};
+  return dispatch_jtf<PoissonOp<double>, PoissonOp<float>>(
+      dtype, fields, out, partial, n0, n1, scale, P,
+      static_cast<hipStream_t>(stream), nblocks);
+}
+
+int odil_laplace_normal_apply(int dtype, int n0, int n1, double dx, double dy,
+                              const void* const* state, const void* const* x,
+                              const void* const* b, const void* const* d,
+                              void* const* y, double omega, int mode,
+                              void* stream) {
+  if (n0 < 2 || n1 < 2) {
+    set_error("laplace_normal_apply: bad arguments");
+    return kErrArg;
+  }
+  LaplaceParams P{dx, dy};
+  return dispatch_normal_apply<LaplaceOp<double>, LaplaceOp<float>>(
+      dtype, state, x, b, d, y, n0, n1, omega, mode, P,
+      static_cast<hipStream_t>(stream));
+}
+
+int odil_laplace_jtf(int dtype, int n0, int n1, double dx, double dy,
+                     const void* const* fields, void* const* out,
+                     double scale, double* partial, int* nblocks,
+                     void* stream) {
+  if (n0 < 2 || n1 < 2 || !fields || !out) {
+    set_error("laplace_jtf: bad arguments");
+    return kErrArg;
+  }
+  LaplaceParams P{dx, dy};
+  return dispatch_jtf<LaplaceOp<double>, LaplaceOp<float>>(
+      dtype, fields, out, partial, n0, n1, scale, P,
+      static_cast<hipStream_t>(stream), nblocks);
+}
+
 }  // extern "C"

@@ -68,14 +68,16 @@ __global__ void __launch_bounds__(256)
       y.p[k][i] += a * x.p[k][i];
 }
 
-// p_k = z_k + (num / den) * p_k  (den == 0 -> p = z).
+// p_k = z_k + ((num - sub) / den) * p_k  (sub optional; den == 0 -> p = z).
 template <class T>
 __global__ void __launch_bounds__(256)
     k_xpay_ratio(VP<T> p, CVP<T> z, int nvar, size_t N,
                  const double* __restrict__ num,
+                 const double* __restrict__ sub,
                  const double* __restrict__ den) {
   const double d = *den;
-  const T b = d != 0.0 ? T((*num) / d) : T(0);
+  const double n = *num - (sub ? *sub : 0.0);
+  const T b = d != 0.0 ? T(n / d) : T(0);
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (int k = 0; k < nvar; ++k)
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
@@ -161,9 +163,11 @@ int odil_axpy_ratio(int dtype, int nvar, long long N, void* const* y,
   return check_launch("axpy_ratio");
 }
 
+// p = z + ((num - sub) / den) p; `sub` may be NULL (conjugate gradients:
+// num = r.z; flexible CG: sub = r.z_old).
 int odil_xpay_ratio(int dtype, int nvar, long long N, void* const* p,
                     const void* const* z, const double* num,
-                    const double* den, void* stream) {
+                    const double* sub, const double* den, void* stream) {
   if (nvar < 1 || nvar > kMaxVec) {
     set_error("xpay_ratio: bad nvar");
     return kErrArg;
@@ -172,11 +176,11 @@ int odil_xpay_ratio(int dtype, int nvar, long long N, void* const* p,
   if (dtype == kF64)
     hipLaunchKernelGGL(k_xpay_ratio<double>, dim3(grid_for(N)), dim3(256), 0,
                        s, mkv<double>(p, nvar), mkc<double>(z, nvar), nvar,
-                       (size_t)N, num, den);
+                       (size_t)N, num, sub, den);
   else if (dtype == kF32)
     hipLaunchKernelGGL(k_xpay_ratio<float>, dim3(grid_for(N)), dim3(256), 0,
                        s, mkv<float>(p, nvar), mkc<float>(z, nvar), nvar,
-                       (size_t)N, num, den);
+                       (size_t)N, num, sub, den);
   else {
     set_error("xpay_ratio: bad dtype");
     return kErrArg;

@@ -190,4 +190,31 @@ int odil_wave_nblocks(int kind, int dtype, int nt, int nx) {
   return ceil_div(nx, 64) * ceil_div(nt, dtype == kF32 ? 16 : 8);
 }
 
+int odil_wave_normal_apply(int dtype, int nt, int nx, double dt, double dx,
+                           const double* init_u, const double* init_ut,
+                           const double* left_u, const double* right_u,
+                           const void* const* state, const void* const* x,
+                           const void* const* b, const void* const* d,
+                           void* const* y, double omega, int mode,
+                           void* stream) {
+  if (wave_bad(nt, nx, init_u, init_ut, left_u, right_u)) return kErrArg;
+  WaveParams P{dt, dx, init_u, init_ut, left_u, right_u};
+  return dispatch_normal_apply<WaveOp<double>, WaveOp<float>>(
+      dtype, state, x, b, d, y, nt, nx, omega, mode, P,
+      static_cast<hipStream_t>(stream));
+}
+
+int odil_wave_jtf(int dtype, int nt, int nx, double dt, double dx,
+                  const double* init_u, const double* init_ut,
+                  const double* left_u, const double* right_u,
+                  const void* const* fields, void* const* out, double scale,
+                  double* partial, int* nblocks, void* stream) {
+  if (wave_bad(nt, nx, init_u, init_ut, left_u, right_u) || !out)
+    return kErrArg;
+  WaveParams P{dt, dx, init_u, init_ut, left_u, right_u};
+  return dispatch_jtf<WaveOp<double>, WaveOp<float>>(
+      dtype, fields, out, partial, nt, nx, scale, P,
+      static_cast<hipStream_t>(stream), nblocks);
+}
+
 }  // extern "C"

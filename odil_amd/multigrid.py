@@ -446,6 +446,10 @@ class MultigridOp:
                                device=self.device)
         self.AA = None
         self._buf = None
+        # Optional finest-level operator used instead of AA[0] for
+        # smoothing and residuals (same apply/residual/jacobi interface),
+        # e.g. a matrix-free J^T J; AA[0] still seeds the Galerkin levels.
+        self.fine = None
 
     # -- setup -----------------------------------------------------------
     def update_A(self, A0):
@@ -516,14 +520,19 @@ class MultigridOp:
                 return level
         return self.nlevels - 1
 
+    def _op(self, level):
+        if level == 0 and self.fine is not None:
+            return self.fine
+        return self.AA[level]
+
     # -- reference-shaped operations -------------------------------------
     def residual(self, A, u, f):
-        return A.residual(u, f)
+        return A.residual(u, f, [torch.empty_like(x) for x in f])
 
     def smoother_jacobi(self, level, u, f, omega=1.):
         """In-place weighted Jacobi (u is modified and returned)."""
         tmp = [torch.empty_like(x) for x in u]
-        self.AA[level].jacobi(u, f, tmp, omega)
+        self._op(level).jacobi(u, f, tmp, omega)
         for a, b in zip(u, tmp):
             a.copy_(b)
         return u
@@ -547,8 +556,8 @@ class MultigridOp:
             return self.smoother_direct(level, u, f)
         for _ in range(pre):
             u = self.smoother_jacobi(level, u, f, omega)
-        A = self.AA[level]
-        r = A.residual(u, f)
+        A = self._op(level)
+        r = A.residual(u, f, [torch.empty_like(x) for x in f])
         rc = restrict(r, self.nnw[level], self.rw,
                       restriction=self.restriction)
         du = self.step([torch.zeros_like(x) for x in rc], rc, omega,
@@ -593,7 +602,7 @@ class MultigridOp:
             for a, b in zip(u, spare):
                 a.copy_(b)
             return u
-        A = self.AA[level]
+        A = self._op(level)
         buf = self._buffers()
         cur, oth = u, spare
         for _ in range(pre):

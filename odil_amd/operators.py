@@ -89,7 +89,7 @@ class Operator:
         squares into `partial` ((neq, nblocks(0)) float64)."""
         dtag = self._check_fields(fields)
         st = self._fn('residual')(c_int(dtag), *self._shape_args(),
-                                  *self._args(), ptrs(fields),
+                                  *self._args_for(dtag), ptrs(fields),
                                   ptrs(res) if res else None, ptr(partial),
                                   _lib.stream())
         check(st, self.prefix + '_residual')
@@ -100,10 +100,51 @@ class Operator:
         dtag = self._check_fields(fields)
         nbo = c_int(0)
         st = self._fn('loss_grad')(c_int(dtag), *self._shape_args(),
-                                   *self._args(), ptrs(fields), ptrs(grads),
-                                   ptr(partial), ctypes.byref(nbo),
-                                   _lib.stream())
+                                   *self._args_for(dtag), ptrs(fields),
+                                   ptrs(grads), ptr(partial),
+                                   ctypes.byref(nbo), _lib.stream())
         check(st, self.prefix + '_loss_grad')
+
+    def normal_apply(self, state, x, y, b=None, d=None, omega=0., mode=0):
+        """Matrix-free N = J^T J with J recomputed from `state` (same dtype
+        as x, y): mode 0 y = N x, 1 y = b - N x, 2 Jacobi sweep
+        y = x + omega (b - N x) / d, 3 y = diag(N)."""
+        dtag = self._check_fields(state)
+        st = self._fn('normal_apply')(
+            c_int(dtag), *self._shape_args(), *self._args_for(dtag),
+            ptrs(state), ptrs(x) if x is not None else None,
+            ptrs(b) if b is not None else None,
+            ptrs(d) if d is not None else None, ptrs(y), c_double(omega),
+            c_int(mode), _lib.stream())
+        check(st, self.prefix + '_normal_apply')
+        return y
+
+    def jtf(self, state, out, scale, partial=None):
+        """out = scale * J^T f(state); returns per-equation sums of f^2
+        (float64 device tensor)."""
+        dtag = self._check_fields(state)
+        nb = self.nblocks(1, dtag)
+        if partial is None:
+            partial = torch.empty((self.neq, nb), dtype=torch.float64,
+                                  device=state[0].device)
+        nbo = c_int(0)
+        st = self._fn('jtf')(c_int(dtag), *self._shape_args(),
+                             *self._args_for(dtag), ptrs(state), ptrs(out),
+                             c_double(scale), ptr(partial),
+                             ctypes.byref(nbo), _lib.stream())
+        check(st, self.prefix + '_jtf')
+        ssq = torch.empty(self.neq, dtype=torch.float64,
+                          device=state[0].device)
+        check(_lib.lib().odil_reduce_partials(ptr(partial), c_int(self.neq),
+                                              c_int(nb), ptr(ssq),
+                                              _lib.stream()),
+              'reduce_partials')
+        return ssq
+
+    def _args_for(self, dtag):
+        """Parameters for a launch in dtype `dtag` (operators whose
+        parameters hold field-typed arrays override this)."""
+        return self._args()
 
     # -- API -------------------------------------------------------------
     def residual(self, fields, want_fields=True):
@@ -117,7 +158,7 @@ class Operator:
         partial = torch.empty((self.neq, nb), dtype=torch.float64,
                               device=dev)
         st = self._fn('residual')(c_int(dtag), *self._shape_args(),
-                                  *self._args(), ptrs(fields),
+                                  *self._args_for(dtag), ptrs(fields),
                                   ptrs(res) if res else None, ptr(partial),
                                   _lib.stream())
         check(st, self.prefix + '_residual')
@@ -154,7 +195,8 @@ class Operator:
                            device=dev)
         res = [torch.empty(self.shape, dtype=fields[0].dtype, device=dev)
                for _ in range(self.neq)] if want_residual else None
-        st = self._fn('jac')(c_int(dtag), *self._shape_args(), *self._args(),
+        st = self._fn('jac')(c_int(dtag), *self._shape_args(),
+                             *self._args_for(dtag),
                              ptrs(fields), ptr(coef),
                              ptrs(res) if res else None, _lib.stream())
         check(st, self.prefix + '_jac')
@@ -174,9 +216,9 @@ class Operator:
                               device=dev)
         nbo = c_int(0)
         st = self._fn('loss_grad')(c_int(dtag), *self._shape_args(),
-                                   *self._args(), ptrs(fields), ptrs(grads),
-                                   ptr(partial), ctypes.byref(nbo),
-                                   _lib.stream())
+                                   *self._args_for(dtag), ptrs(fields),
+                                   ptrs(grads), ptr(partial),
+                                   ctypes.byref(nbo), _lib.stream())
         check(st, self.prefix + '_loss_grad')
         if nbo.value != nb:
             raise _lib.OdilError('block count mismatch {} != {}'.format(
@@ -274,11 +316,18 @@ class Poisson(Operator):
     def _args(self):
         return [c_double(self.dx), c_double(self.dy), ptr(self.src)]
 
+    def _args_for(self, dtag):
+        want = torch.float64 if dtag == _lib.F64 else torch.float32
+        if self.src.dtype != want:
+            if getattr(self, '_src_cast', None) is None or \
+                    self._src_cast.dtype != want:
+                self._src_cast = self.src.to(want)
+            return [c_double(self.dx), c_double(self.dy),
+                    ptr(self._src_cast)]
+        return self._args()
+
     def _check_fields(self, fields):
-        d = super()._check_fields(fields)
-        if fields[0].dtype != self.src.dtype:
-            raise ValueError('src dtype must match the fields')
-        return d
+        return super()._check_fields(fields)
 
 
 class Laplace(Operator):

@@ -136,3 +136,72 @@ def test_advection(dev, shape, seed, dtype):
     np.testing.assert_allclose(l.item(), loss,
                                rtol=1e-11 if dtype == torch.float64 else 1e-4)
     _close(g[0].cpu(), lg['p'], rtol, np.abs(lg['p']).max())
+
+
+def _ops_for_normal(dev, dtype):
+    """(name, operator, state) for every 2D operator, random states."""
+    from odil_amd.operators import Cavity, Laplace, Poisson, Wave
+    rng = np.random.default_rng(7)
+    out = []
+    n0, n1 = 37, 70
+    st = [torch.tensor(rng.uniform(-1, 1, (n0, n1)), dtype=dtype, device=dev)
+          for _ in range(3)]
+    out.append(('cavity', Cavity((n0, n1), 0.01), st))
+    out.append(('cavity_bc2', Cavity((n0, n1), 0.03, bc=2), st))
+    dom, op_ref, bnd, u = _wave_case(33, 45, 3)
+    out.append(('wave', Wave((33, 45), *bnd, device=dev),
+                [torch.tensor(u, dtype=dtype, device=dev)]))
+    src = torch.tensor(rng.normal(size=(n0, n1)), dtype=dtype, device=dev)
+    out.append(('poisson', Poisson((n0, n1), src),
+                [torch.tensor(rng.normal(size=(n0, n1)), dtype=dtype,
+                              device=dev)]))
+    out.append(('laplace', Laplace((n0, n1)), st[:2]))
+    return out
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_normal_apply_matches_stored_normal(dev, dtype):
+    from odil_amd.multigrid import BlockStencil, normal_operator
+    tol = 1e-12 if dtype == torch.float64 else 3e-5
+    rng = np.random.default_rng(3)
+    for name, op, st in _ops_for_normal(dev, dtype):
+        jac, res = op.jacobian(st, want_residual=True)
+        J = BlockStencil.from_jacobian(jac)
+        N = normal_operator(J)
+        nf = len(st)
+        mk = lambda: [torch.tensor(rng.uniform(-1, 1, op.shape), dtype=dtype,
+                                   device=dev) for _ in range(nf)]
+        x, b = mk(), mk()
+        y = [torch.empty_like(a) for a in x]
+        ref = N.apply(x)
+        scale = max(a.abs().max().item() for a in ref)
+        op.normal_apply(st, x, y, mode=0)
+        for a, r in zip(y, ref):
+            assert (a - r).abs().max().item() <= tol * scale, name
+        op.normal_apply(st, x, y, b=b, mode=1)
+        for a, r, bb in zip(y, ref, b):
+            assert (a - (bb - r)).abs().max().item() <= tol * scale, name
+        d = [torch.empty_like(a) for a in x]
+        op.normal_apply(st, None, d, mode=3)
+        zero = (0, ) * len(op.shape)
+        for k in range(nf):
+            q = N.entries.index((k, k, zero))
+            dr = N.coef[q]
+            assert (d[k] - dr).abs().max().item() <= \
+                tol * dr.abs().max().item(), name
+        op.normal_apply(st, x, y, b=b, d=d, omega=0.6, mode=2)
+        for k in range(nf):
+            want = x[k] + 0.6 * (b[k] - ref[k]) / d[k]
+            assert (y[k] - want).abs().max().item() <= \
+                tol * 10 * want.abs().max().item(), name
+        # rhs = -J^T f with the residual's sums of squares
+        out = [torch.empty_like(a) for a in st]
+        ssq = op.jtf(st, out, -1.0)
+        ref = J.apply_t([-r for r in res])
+        sc = max(a.abs().max().item() for a in ref)
+        for a, r in zip(out, ref):
+            assert (a - r).abs().max().item() <= tol * sc, name
+        for e in range(len(res)):
+            np.testing.assert_allclose(
+                ssq[e].item(), (res[e].double()**2).sum().item(),
+                rtol=1e-9 if dtype == torch.float64 else 1e-4)
