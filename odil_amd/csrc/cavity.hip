@@ -45,9 +45,14 @@ struct Nbr {
   T c, xm, xp, ym, yp, xmm, xpp, ymm, ypp;
 };
 
-template <class T, class Acc>
+template <bool INT, class T, class Acc>
 __device__ inline Nbr<T> neighbours(const Acc& q, int k, int i, int j, int nx,
                                     int ny, const FaceRule& r) {
+  if (INT) {  // no wall within two cells: plain neighbours
+    return Nbr<T>{q(k, 0, 0),  q(k, -1, 0), q(k, 1, 0),
+                  q(k, 0, -1), q(k, 0, 1),  q(k, -2, 0),
+                  q(k, 2, 0),  q(k, 0, -2), q(k, 0, 2)};
+  }
   Nbr<T> s;
   const T c = q(k, 0, 0);
   const T rxm = q(k, -1, 0), rxp = q(k, 1, 0);
@@ -76,7 +81,7 @@ __device__ inline Nbr<T> neighbours(const Acc& q, int k, int i, int j, int nx,
 // Effective-neighbour coefficients folded onto raw values: where a
 // neighbour is a wall halo, its coefficient moves to the far and near inner
 // values of the extrapolation.
-template <class T>
+template <bool INT, class T>
 __device__ inline void fold(T* raw, const T* eff, int i, int j, int nx, int ny,
                             int kx, int ky) {
   // raw/eff order: c, xm, xp, ym, yp
@@ -85,6 +90,7 @@ __device__ inline void fold(T* raw, const T* eff, int i, int j, int nx, int ny,
   raw[2] = eff[2];
   raw[3] = eff[3];
   raw[4] = eff[4];
+  if (INT) return;
   if (i == 0) {
     raw[2] += eff[1] * ghost_dfar<T>(kx);
     raw[0] += eff[1] * ghost_dnear<T>(kx);
@@ -126,53 +132,57 @@ struct CavityOp {
                                  0, 1, 2, 3, 4, 0, 3, 4,        //
                                  0, 1, 2, 0, 3, 4, 0, 1, 2, 3, 4};
 
-  template <class Acc>
+  // 1/g with a hardware reciprocal in FP32 (1 ulp), exact in FP64.
+  __device__ static inline T recip(T g) {
+    if constexpr (sizeof(T) == 4)
+      return __builtin_amdgcn_rcpf(g);
+    else
+      return T(1) / g;
+  }
+
+  // INT: the cell and its distance-2 neighbours are inside the domain and
+  // away from the walls, so every halo rule drops out at compile time.
+  template <bool INT, class Acc>
   __device__ static inline void compute(const Acc& q, int i, int j, int nx,
                                         int ny, const Params& P, T* f,
                                         T (*J)[NF][NS], bool want_jac) {
     const T dx = T(P.dx), dy = T(P.dy), mu = T(P.mu);
     const FaceRule ru = cavity_rule(P.bc, 0), rv = cavity_rule(P.bc, 1);
     const FaceRule rp{LINEAR, LINEAR, 0, 0, 0, 0};
-    const Nbr<T> U = neighbours<T>(q, 0, i, j, nx, ny, ru);
-    const Nbr<T> V = neighbours<T>(q, 1, i, j, nx, ny, rv);
-    const Nbr<T> Pn = neighbours<T>(q, 2, i, j, nx, ny, rp);
+    const Nbr<T> U = neighbours<INT, T>(q, 0, i, j, nx, ny, ru);
+    const Nbr<T> V = neighbours<INT, T>(q, 1, i, j, nx, ny, rv);
+    const Nbr<T> Pn = neighbours<INT, T>(q, 2, i, j, nx, ny, rp);
+    const bool iw0 = !INT && i == 0, iw1 = !INT && i == nx - 1;
+    const bool jw0 = !INT && j == 0, jw1 = !INT && j == ny - 1;
     const T a = U.c, b = V.c;  // frozen velocities at the cell
     const T idx = T(1) / dx, idy = T(1) / dy;
     const T h2x = T(1) / (T(2) * dx), h2y = T(1) / (T(2) * dy);
 
     // Second-order upwind of the frozen field, first order at walls.
     auto hi2x = [&](const Nbr<T>& s) {
-      const T m = (i == 0) ? (s.c - s.xm) * idx
+      const T m = iw0 ? (s.c - s.xm) * idx
                            : (s.xmm - T(4) * s.xm + T(3) * s.c) * h2x;
-      const T p = (i == nx - 1) ? (s.xp - s.c) * idx
+      const T p = iw1 ? (s.xp - s.c) * idx
                                 : (-T(3) * s.c + T(4) * s.xp - s.xpp) * h2x;
       return a > T(0) ? m : p;
     };
     auto hi2y = [&](const Nbr<T>& s) {
-      const T m = (j == 0) ? (s.c - s.ym) * idy
+      const T m = jw0 ? (s.c - s.ym) * idy
                            : (s.ymm - T(4) * s.ym + T(3) * s.c) * h2y;
-      const T p = (j == ny - 1) ? (s.yp - s.c) * idy
+      const T p = jw1 ? (s.yp - s.c) * idy
                                 : (-T(3) * s.c + T(4) * s.yp - s.ypp) * h2y;
       return b > T(0) ? m : p;
     };
-    auto lo1x = [&](const Nbr<T>& s) {
-      return a > T(0) ? (s.c - s.xm) * idx : (s.xp - s.c) * idx;
-    };
-    auto lo1y = [&](const Nbr<T>& s) {
-      return b > T(0) ? (s.c - s.ym) * idy : (s.yp - s.c) * idy;
-    };
+    const T idx2 = idx * idx, idy2 = idy * idy;
     auto lap = [&](const Nbr<T>& s) {
-      return (s.xp - T(2) * s.c + s.xm) / (dx * dx) +
-             (s.yp - T(2) * s.c + s.ym) / (dy * dy);
+      return (s.xp - T(2) * s.c + s.xm) * idx2 +
+             (s.yp - T(2) * s.c + s.ym) * idy2;
     };
     {
-      T lx = lo1x(U), ly = lo1y(U);
-      T ux = (hi2x(U) + lx) - lx, uy = (hi2y(U) + ly) - ly;
-      f[0] = a * ux + b * uy - mu * lap(U) + (Pn.xp - Pn.xm) * h2x;
-      lx = lo1x(V);
-      ly = lo1y(V);
-      T vx = (hi2x(V) + lx) - lx, vy = (hi2y(V) + ly) - ly;
-      f[1] = a * vx + b * vy - mu * lap(V) + (Pn.yp - Pn.ym) * h2y;
+      // u_x = hi2(uf) + lo1(u) - lo1(uf); live and frozen values coincide,
+      // so the low-order terms cancel.
+      f[0] = a * hi2x(U) + b * hi2y(U) - mu * lap(U) + (Pn.xp - Pn.xm) * h2x;
+      f[1] = a * hi2x(V) + b * hi2y(V) - mu * lap(V) + (Pn.yp - Pn.ym) * h2y;
     }
     const T gdiag = mu * (T(2) / (dx * dx) + T(2) / (dy * dy));
     // Rhie-Chow coefficient g from frozen velocities (roll: periodic index,
@@ -184,36 +194,36 @@ struct CavityOp {
     const bool wall_keep = (P.bc == 2);
     T kxm = 0, kxp = 0, kym = 0, kyp = 0;
     if (P.rhiechow) {
-      const T ig = T(1) / g;
+      const T ig = recip(g);
       T qxm = 0, qxp = 0, qym = 0, qyp = 0;
       const T cx = ig * (Pn.xp - Pn.xm) * h2x * T(0.5);
       const T cy = ig * (Pn.yp - Pn.ym) * h2y * T(0.5);
-      if (i != 0) {
-        const T igm = T(1) / gat(-1, 0);
+      if (!iw0) {
+        const T igm = recip(gat(-1, 0));
         kxm = (igm + ig) * T(0.5) * idx;
         qxm = (U.c + U.xm) * T(0.5) + (igm * (Pn.c - Pn.xmm) * h2x) * T(0.5) +
               cx - (igm + ig) * T(0.5) * (Pn.c - Pn.xm) * idx;
       } else if (wall_keep) {
         qxm = (U.c + U.xm) * T(0.5);
       }
-      if (i != nx - 1) {
-        const T igp = T(1) / gat(1, 0);
+      if (!iw1) {
+        const T igp = recip(gat(1, 0));
         kxp = (igp + ig) * T(0.5) * idx;
         qxp = (U.c + U.xp) * T(0.5) + (igp * (Pn.xpp - Pn.c) * h2x) * T(0.5) +
               cx - (igp + ig) * T(0.5) * (Pn.xp - Pn.c) * idx;
       } else if (wall_keep) {
         qxp = (U.c + U.xp) * T(0.5);
       }
-      if (j != 0) {
-        const T igm = T(1) / gat(0, -1);
+      if (!jw0) {
+        const T igm = recip(gat(0, -1));
         kym = (igm + ig) * T(0.5) * idy;
         qym = (V.c + V.ym) * T(0.5) + (igm * (Pn.c - Pn.ymm) * h2y) * T(0.5) +
               cy - (igm + ig) * T(0.5) * (Pn.c - Pn.ym) * idy;
       } else if (wall_keep) {
         qym = (V.c + V.ym) * T(0.5);
       }
-      if (j != ny - 1) {
-        const T igp = T(1) / gat(0, 1);
+      if (!jw1) {
+        const T igp = recip(gat(0, 1));
         kyp = (igp + ig) * T(0.5) * idy;
         qyp = (V.c + V.yp) * T(0.5) + (igp * (Pn.ypp - Pn.c) * h2y) * T(0.5) +
               cy - (igp + ig) * T(0.5) * (Pn.yp - Pn.c) * idy;
@@ -251,14 +261,14 @@ struct CavityOp {
     }
     for (int k = 0; k < 3; ++k)
       for (int s = 0; s < 5; ++s) J[0][k][s] = J[1][k][s] = 0;
-    fold(J[0][0], e, i, j, nx, ny, ru.kx, ru.ky);
-    fold(J[1][1], e, i, j, nx, ny, rv.kx, rv.ky);
+    fold<INT>(J[0][0], e, i, j, nx, ny, ru.kx, ru.ky);
+    fold<INT>(J[1][1], e, i, j, nx, ny, rv.kx, rv.ky);
     // fu <- p: (p_xp - p_xm) / (2 dx); fv <- p along y.
     {
       T ep[5] = {0, -h2x, h2x, 0, 0};
-      fold(J[0][2], ep, i, j, nx, ny, LINEAR, LINEAR);
+      fold<INT>(J[0][2], ep, i, j, nx, ny, LINEAR, LINEAR);
       T eq[5] = {0, 0, 0, -h2y, h2y};
-      fold(J[1][2], eq, i, j, nx, ny, LINEAR, LINEAR);
+      fold<INT>(J[1][2], eq, i, j, nx, ny, LINEAR, LINEAR);
     }
     // fp <- u, v, p.
     {
@@ -266,19 +276,19 @@ struct CavityOp {
         ep[5] = {0, 0, 0, 0, 0};
       const T hx = T(0.5) * idx, hy = T(0.5) * idy;
       if (P.rhiechow) {
-        if (i != 0 || wall_keep) {
+        if (!iw0 || wall_keep) {
           eu[0] -= hx;
           eu[1] -= hx;
         }
-        if (i != nx - 1 || wall_keep) {
+        if (!iw1 || wall_keep) {
           eu[0] += hx;
           eu[2] += hx;
         }
-        if (j != 0 || wall_keep) {
+        if (!jw0 || wall_keep) {
           ev[0] -= hy;
           ev[3] -= hy;
         }
-        if (j != ny - 1 || wall_keep) {
+        if (!jw1 || wall_keep) {
           ev[0] += hy;
           ev[4] += hy;
         }
@@ -294,22 +304,22 @@ struct CavityOp {
         ev[3] = -h2y;
         ev[4] = h2y;
       }
-      fold(J[2][0], eu, i, j, nx, ny, ru.kx, ru.ky);
-      fold(J[2][1], ev, i, j, nx, ny, rv.kx, rv.ky);
-      fold(J[2][2], ep, i, j, nx, ny, LINEAR, LINEAR);
+      fold<INT>(J[2][0], eu, i, j, nx, ny, ru.kx, ru.ky);
+      fold<INT>(J[2][1], ev, i, j, nx, ny, rv.kx, rv.ky);
+      fold<INT>(J[2][2], ep, i, j, nx, ny, LINEAR, LINEAR);
     }
   }
 
-  template <class Acc>
+  template <bool INT = false, class Acc>
   __device__ static inline void residual(const Acc& q, int i, int j, int nx,
                                          int ny, const Params& P, T* f) {
-    compute(q, i, j, nx, ny, P, f, nullptr, false);
+    compute<INT>(q, i, j, nx, ny, P, f, nullptr, false);
   }
-  template <class Acc>
+  template <bool INT = false, class Acc>
   __device__ static inline void eval(const Acc& q, int i, int j, int nx,
                                      int ny, const Params& P, T* f,
                                      T (*J)[NF][NS]) {
-    compute(q, i, j, nx, ny, P, f, J, true);
+    compute<INT>(q, i, j, nx, ny, P, f, J, true);
   }
 };
 

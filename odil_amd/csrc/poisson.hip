@@ -74,12 +74,12 @@ struct PoissonOp {
             j, n0, n1, QUADH, T(P.dx), T(P.dy), &l, J ? J[0][0] : nullptr);
     f[0] = l - static_cast<const T*>(P.src)[(size_t)i * n1 + j];
   }
-  template <class Acc>
+  template <bool INT = false, class Acc>
   __device__ static inline void residual(const Acc& q, int i, int j, int n0,
                                          int n1, const Params& P, T* f) {
     go(q, i, j, n0, n1, P, f, nullptr);
   }
-  template <class Acc>
+  template <bool INT = false, class Acc>
   __device__ static inline void eval(const Acc& q, int i, int j, int n0,
                                      int n1, const Params& P, T* f,
                                      T (*J)[NF][NS]) {
@@ -119,12 +119,12 @@ struct LaplaceOp {
     }
     f[2] = 0;
   }
-  template <class Acc>
+  template <bool INT = false, class Acc>
   __device__ static inline void residual(const Acc& q, int i, int j, int n0,
                                          int n1, const Params& P, T* f) {
     go(q, i, j, n0, n1, P, f, nullptr);
   }
-  template <class Acc>
+  template <bool INT = false, class Acc>
   __device__ static inline void eval(const Acc& q, int i, int j, int n0,
                                      int n1, const Params& P, T* f,
                                      T (*J)[NF][NS]) {

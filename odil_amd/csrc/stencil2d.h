@@ -23,16 +23,27 @@
 
 namespace odil {
 
-template <class T, int NF>
+template <class T, int NF, bool CLAMP = true>
 struct GlobalAcc {
   const T* const* f;
   int n0, n1, i, j;
   __device__ inline T operator()(int k, int di, int dj) const {
-    int ii = min(max(i + di, 0), n0 - 1);
-    int jj = min(max(j + dj, 0), n1 - 1);
+    int ii = i + di, jj = j + dj;
+    if (CLAMP) {
+      ii = min(max(ii, 0), n0 - 1);
+      jj = min(max(jj, 0), n1 - 1);
+    }
     return f[k][(size_t)ii * n1 + jj];
   }
 };
+
+// True when every cell of rows [a0, a1) x cols [b0, b1) is at least R cells
+// from the domain edge, so operators may take their interior fast path
+// (no wall rules, no clamping).
+__device__ inline bool interior_box(int a0, int a1, int b0, int b1, int n0,
+                                    int n1, int R) {
+  return a0 >= R && a1 <= n0 - R && b0 >= R && b1 <= n1 - R;
+}
 
 template <class T>
 struct LdsAcc {
@@ -65,9 +76,15 @@ __global__ void __launch_bounds__(256)
   double ssq[Op::NE];
   for (int e = 0; e < Op::NE; ++e) ssq[e] = 0;
   if (i < n0 && j < n1) {
-    GlobalAcc<T, Op::NF> acc{in.f, n0, n1, i, j};
     T f[Op::NE];
-    Op::residual(acc, i, j, n0, n1, P, f);
+    if (interior_box(blockIdx.y * 4, blockIdx.y * 4 + 4, blockIdx.x * 64,
+                     blockIdx.x * 64 + 64, n0, n1, Op::R)) {
+      GlobalAcc<T, Op::NF, false> acc{in.f, n0, n1, i, j};
+      Op::template residual<true>(acc, i, j, n0, n1, P, f);
+    } else {
+      GlobalAcc<T, Op::NF> acc{in.f, n0, n1, i, j};
+      Op::template residual<false>(acc, i, j, n0, n1, P, f);
+    }
     const size_t c = (size_t)i * n1 + j;
     for (int e = 0; e < Op::NE; ++e) {
       if (out.f[e]) out.f[e][c] = f[e];
@@ -93,10 +110,16 @@ __global__ void __launch_bounds__(256)
   const int j = blockIdx.x * 64 + (threadIdx.x & 63);
   const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
   if (i >= n0 || j >= n1) return;
-  GlobalAcc<T, Op::NF> acc{in.f, n0, n1, i, j};
   T f[Op::NE];
   T J[Op::NE][Op::NF][Op::NS];
-  Op::eval(acc, i, j, n0, n1, P, f, J);
+  if (interior_box(blockIdx.y * 4, blockIdx.y * 4 + 4, blockIdx.x * 64,
+                   blockIdx.x * 64 + 64, n0, n1, Op::R)) {
+    GlobalAcc<T, Op::NF, false> acc{in.f, n0, n1, i, j};
+    Op::template eval<true>(acc, i, j, n0, n1, P, f, J);
+  } else {
+    GlobalAcc<T, Op::NF> acc{in.f, n0, n1, i, j};
+    Op::template eval<false>(acc, i, j, n0, n1, P, f, J);
+  }
   const size_t N = (size_t)n0 * n1;
   const size_t c = (size_t)i * n1 + j;
 #pragma unroll
@@ -106,122 +129,129 @@ __global__ void __launch_bounds__(256)
     if (res.f[e]) res.f[e][c] = f[e];
 }
 
+// Ring cell q of the tile grown by RJ (the cells around the TX x TY inner
+// tile): returns its E-local coordinates.
+template <int TX, int TY, int RJ>
+__device__ inline void ring_cell(int q, int& ea, int& eb) {
+  constexpr int E1 = TY + 2 * RJ;
+  if (q < RJ * E1) {  // top rows
+    ea = q / E1;
+    eb = q - ea * E1;
+  } else if (q < 2 * RJ * E1) {  // bottom rows
+    const int r = q - RJ * E1;
+    ea = TX + RJ + r / E1;
+    eb = r % E1;
+  } else {  // side columns
+    const int r = q - 2 * RJ * E1;
+    const int row = r / (2 * RJ);
+    const int col = r - row * (2 * RJ);
+    ea = RJ + row;
+    eb = col < RJ ? col : TY + col;
+  }
+}
+
+// Evaluate residual and Jacobian of cell (gi, gj) reading the state from
+// global memory (neighbour reuse through L1/L2); INT blocks skip clamping
+// and wall rules.
+template <class T, class Op>
+__device__ inline void eval_global(const FieldPtrs<T, Op::NF>& in, int gi,
+                                   int gj, int n0, int n1, bool inner,
+                                   const typename Op::Params& P, T* f,
+                                   T (*J)[Op::NF][Op::NS]) {
+  if (inner) {
+    GlobalAcc<T, Op::NF, false> acc{in.f, n0, n1, gi, gj};
+    Op::template eval<true>(acc, gi, gj, n0, n1, P, f, J);
+  } else {
+    GlobalAcc<T, Op::NF> acc{in.f, n0, n1, gi, gj};
+    Op::template eval<false>(acc, gi, gj, n0, n1, P, f, J);
+  }
+}
+
 // Fused loss + adjoint. One workgroup owns a TX x TY tile of output cells.
-//  1. load state for the tile grown by RJ + R into LDS;
-//  2. evaluate f and J on the tile grown by RJ (RJ = Jacobian reach); every
-//     off-centre product J[e][k][s] f_e goes to LDS, the centre product of
-//     inner cells stays in registers;
-//  3. gather grad_k(c) = sum_s out_{k,s}(c - shift_s).
+//  1. evaluate f and J on the tile grown by RJ (RJ = Jacobian reach),
+//     reading the state from global memory; every off-centre product
+//     J[e][k][s] f_e goes to LDS, the centre product of inner cells stays in
+//     registers;
+//  2. gather grad_k(c) = scale * sum_s out_{k,s}(c - shift_s).
 template <class T, class Op, int TX, int TY>
 __global__ void __launch_bounds__(256)
     k_loss_grad(FieldPtrs<T, Op::NF> in, OutPtrs<T, Op::NF> grad,
                 double* __restrict__ partial, int n0, int n1, T scale,
                 typename Op::Params P) {
-  constexpr int RJ = Op::RJ;
-  constexpr int H = RJ + Op::R;
-  constexpr int S0 = TX + 2 * H, S1 = TY + 2 * H;
+  constexpr int NF = Op::NF, NE = Op::NE, NS = Op::NS, RJ = Op::RJ;
   constexpr int E0 = TX + 2 * RJ, E1 = TY + 2 * RJ;
   constexpr int NT = 256;
   constexpr int NPT = (TX * TY) / NT;  // inner cells per thread
   static_assert((TX * TY) % NT == 0, "tile must be a multiple of 256");
   extern __shared__ unsigned char smem_raw[];
-  T* st = reinterpret_cast<T*>(smem_raw);           // [NF][S0][S1]
-  T* ot = st + Op::NF * S0 * S1;                    // [NF][NS-1][E0][E1]
+  T* ot = reinterpret_cast<T*>(smem_raw);  // [NF][NS-1][E0][E1]
   __shared__ double red[4];
 
   const int x0 = blockIdx.y * TX;
   const int y0 = blockIdx.x * TY;
   const int tid = threadIdx.x;
+  const bool inner = interior_box(x0 - RJ, x0 + TX + RJ, y0 - RJ,
+                                  y0 + TY + RJ, n0, n1, Op::R);
+  auto OT = [&](int k, int s, int ea, int eb) -> T& {
+    return ot[((k * (NS - 1) + s - 1) * E0 + ea) * E1 + eb];
+  };
 
-  // 1. state tile (values outside the domain are never selected; zero-fill).
-  for (int q = tid; q < Op::NF * S0 * S1; q += NT) {
-    const int k = q / (S0 * S1);
-    const int r = q - k * S0 * S1;
-    const int a = r / S1, b = r - a * S1;
-    const int gi = x0 - H + a, gj = y0 - H + b;
-    T v = 0;
-    if (gi >= 0 && gi < n0 && gj >= 0 && gj < n1)
-      v = in.f[k][(size_t)gi * n1 + gj];
-    st[q] = v;
-  }
-  __syncthreads();
-
-  // 2a. inner cells: keep centre products and squares in registers.
-  T cen[NPT][Op::NF];
-  double ssq[Op::NE];
-  for (int e = 0; e < Op::NE; ++e) ssq[e] = 0;
+  T cen[NPT][NF];
+  double ssq[NE];
+  for (int e = 0; e < NE; ++e) ssq[e] = 0;
   for (int p = 0; p < NPT; ++p) {
     const int q = tid + p * NT;
     const int a = q / TY, b = q - a * TY;
     const int gi = x0 + a, gj = y0 + b;
-    for (int k = 0; k < Op::NF; ++k) cen[p][k] = 0;
     const int ea = a + RJ, eb = b + RJ;
+    for (int k = 0; k < NF; ++k) cen[p][k] = 0;
     if (gi < n0 && gj < n1) {
-      LdsAcc<T> acc{st, S0, S1, a + H, b + H};
-      T f[Op::NE];
-      T J[Op::NE][Op::NF][Op::NS];
-      Op::eval(acc, gi, gj, n0, n1, P, f, J);
-      for (int e = 0; e < Op::NE; ++e) ssq[e] += (double)f[e] * (double)f[e];
+      T f[NE];
+      T J[NE][NF][NS];
+      eval_global<T, Op>(in, gi, gj, n0, n1, inner, P, f, J);
+      for (int e = 0; e < NE; ++e) ssq[e] += (double)f[e] * (double)f[e];
 #pragma unroll
-      for (int k = 0; k < Op::NF; ++k)
+      for (int k = 0; k < NF; ++k)
 #pragma unroll
-        for (int s = 0; s < Op::NS; ++s) {
+        for (int s = 0; s < NS; ++s) {
           T o = 0;
 #pragma unroll
-          for (int e = 0; e < Op::NE; ++e) o += J[e][k][s] * f[e];
+          for (int e = 0; e < NE; ++e) o += J[e][k][s] * f[e];
           if (s == 0)
             cen[p][k] = o;
           else
-            ot[((k * (Op::NS - 1) + s - 1) * E0 + ea) * E1 + eb] = o;
+            OT(k, s, ea, eb) = o;
         }
     } else {
-      for (int k = 0; k < Op::NF; ++k)
-        for (int s = 1; s < Op::NS; ++s)
-          ot[((k * (Op::NS - 1) + s - 1) * E0 + ea) * E1 + eb] = 0;
+      for (int k = 0; k < NF; ++k)
+        for (int s = 1; s < NS; ++s) OT(k, s, ea, eb) = 0;
     }
   }
-  // 2b. ring cells around the tile (only off-centre products are needed).
   constexpr int NRING = E0 * E1 - TX * TY;
   for (int q = tid; q < NRING; q += NT) {
     int ea, eb;
-    if (q < RJ * E1) {  // top rows
-      ea = q / E1;
-      eb = q - ea * E1;
-    } else if (q < 2 * RJ * E1) {  // bottom rows
-      const int r = q - RJ * E1;
-      ea = TX + RJ + r / E1;
-      eb = r % E1;
-    } else {  // side columns
-      const int r = q - 2 * RJ * E1;
-      const int row = r / (2 * RJ);
-      const int col = r - row * (2 * RJ);
-      ea = RJ + row;
-      eb = col < RJ ? col : TY + col;
-    }
+    ring_cell<TX, TY, RJ>(q, ea, eb);
     const int gi = x0 - RJ + ea, gj = y0 - RJ + eb;
     if (gi >= 0 && gi < n0 && gj >= 0 && gj < n1) {
-      LdsAcc<T> acc{st, S0, S1, ea + Op::R, eb + Op::R};
-      T f[Op::NE];
-      T J[Op::NE][Op::NF][Op::NS];
-      Op::eval(acc, gi, gj, n0, n1, P, f, J);
+      T f[NE];
+      T J[NE][NF][NS];
+      eval_global<T, Op>(in, gi, gj, n0, n1, inner, P, f, J);
 #pragma unroll
-      for (int k = 0; k < Op::NF; ++k)
+      for (int k = 0; k < NF; ++k)
 #pragma unroll
-        for (int s = 1; s < Op::NS; ++s) {
+        for (int s = 1; s < NS; ++s) {
           T o = 0;
 #pragma unroll
-          for (int e = 0; e < Op::NE; ++e) o += J[e][k][s] * f[e];
-          ot[((k * (Op::NS - 1) + s - 1) * E0 + ea) * E1 + eb] = o;
+          for (int e = 0; e < NE; ++e) o += J[e][k][s] * f[e];
+          OT(k, s, ea, eb) = o;
         }
     } else {
-      for (int k = 0; k < Op::NF; ++k)
-        for (int s = 1; s < Op::NS; ++s)
-          ot[((k * (Op::NS - 1) + s - 1) * E0 + ea) * E1 + eb] = 0;
+      for (int k = 0; k < NF; ++k)
+        for (int s = 1; s < NS; ++s) OT(k, s, ea, eb) = 0;
     }
   }
   __syncthreads();
 
-  // 3. gather.
   for (int p = 0; p < NPT; ++p) {
     const int q = tid + p * NT;
     const int a = q / TY, b = q - a * TY;
@@ -229,21 +259,18 @@ __global__ void __launch_bounds__(256)
     if (gi >= n0 || gj >= n1) continue;
     const size_t c = (size_t)gi * n1 + gj;
 #pragma unroll
-    for (int k = 0; k < Op::NF; ++k) {
+    for (int k = 0; k < NF; ++k) {
       T g = cen[p][k];
 #pragma unroll
-      for (int s = 1; s < Op::NS; ++s) {
-        // Cell n = c - shift_s contributed J[.][k][s] f(n) to c.
-        const int ea = a + RJ - Op::SX[s], eb = b + RJ - Op::SY[s];
-        g += ot[((k * (Op::NS - 1) + s - 1) * E0 + ea) * E1 + eb];
-      }
+      for (int s = 1; s < NS; ++s)
+        g += OT(k, s, a + RJ - Op::SX[s], b + RJ - Op::SY[s]);
       if (grad.f[k]) grad.f[k][c] = g * scale;
     }
   }
   if (partial) {
     const int bidx = blockIdx.y * gridDim.x + blockIdx.x;
     const int nb = gridDim.x * gridDim.y;
-    for (int e = 0; e < Op::NE; ++e) {
+    for (int e = 0; e < NE; ++e) {
       double s = block_sum(ssq[e], red);
       if (tid == 0) partial[(size_t)e * nb + bidx] = s;
     }
@@ -252,75 +279,54 @@ __global__ void __launch_bounds__(256)
 
 template <class T, class Op, int TX, int TY>
 constexpr size_t loss_grad_smem() {
-  constexpr int H = Op::RJ + Op::R;
-  return sizeof(T) * (size_t)(Op::NF * (TX + 2 * H) * (TY + 2 * H) +
-                              Op::NF * (Op::NS - 1) * (TX + 2 * Op::RJ) *
-                                  (TY + 2 * Op::RJ));
+  return sizeof(T) * (size_t)(Op::NF * (Op::NS - 1) * (TX + 2 * Op::RJ) *
+                              (TY + 2 * Op::RJ));
 }
 
 // Fused matrix-free normal operator N = J^T J with the Jacobian recomputed
-// from the state in LDS (no stored coefficients):
+// from the state (no stored coefficients):
 //   mode 0: y = N x          mode 1: y = b - N x
 //   mode 2: y = x + omega (b - N x) / d   (weighted Jacobi, d = diag N)
 //   mode 3: y = diag(N) = sum_e sum_s J[e][k][s](c - shift_s)^2
-// Tile layout as in k_loss_grad; x is staged with halo 2 RJ because
-// (J x)(n) reads x at n + shift for every n in the tile grown by RJ.
+// Same tile scheme as k_loss_grad with f replaced by v = J x; state and x
+// are read from global memory.
 template <class T, class Op, int TX, int TY>
 __global__ void __launch_bounds__(256)
     k_normal_apply(FieldPtrs<T, Op::NF> st_in, FieldPtrs<T, Op::NF> x_in,
                    FieldPtrs<T, Op::NF> b_in, FieldPtrs<T, Op::NF> d_in,
                    OutPtrs<T, Op::NF> y, int n0, int n1, T omega, int mode,
                    typename Op::Params P) {
-  constexpr int NF = Op::NF, NE = Op::NE, NS = Op::NS;
-  constexpr int RJ = Op::RJ;
-  constexpr int H = RJ + Op::R;
-  constexpr int HX = 2 * RJ;
-  constexpr int S0 = TX + 2 * H, S1 = TY + 2 * H;
-  constexpr int X0 = TX + 2 * HX, X1 = TY + 2 * HX;
+  constexpr int NF = Op::NF, NE = Op::NE, NS = Op::NS, RJ = Op::RJ;
   constexpr int E0 = TX + 2 * RJ, E1 = TY + 2 * RJ;
   constexpr int NT = 256;
   constexpr int NPT = (TX * TY) / NT;
   static_assert((TX * TY) % NT == 0, "tile must be a multiple of 256");
   extern __shared__ unsigned char smem_raw[];
-  T* st = reinterpret_cast<T*>(smem_raw);  // [NF][S0][S1]
-  T* xs = st + NF * S0 * S1;               // [NF][X0][X1]
-  T* ot = xs + NF * X0 * X1;               // [NF][NS-1][E0][E1]
+  T* ot = reinterpret_cast<T*>(smem_raw);  // [NF][NS-1][E0][E1]
 
   const int x0 = blockIdx.y * TX;
   const int y0 = blockIdx.x * TY;
   const int tid = threadIdx.x;
   const bool diag = (mode == 3);
-
-  for (int q = tid; q < NF * S0 * S1; q += NT) {
-    const int k = q / (S0 * S1);
-    const int r = q - k * S0 * S1;
-    const int a = r / S1, b = r - a * S1;
-    const int gi = x0 - H + a, gj = y0 - H + b;
-    T v = 0;
-    if (gi >= 0 && gi < n0 && gj >= 0 && gj < n1)
-      v = st_in.f[k][(size_t)gi * n1 + gj];
-    st[q] = v;
-  }
-  if (!diag)
-    for (int q = tid; q < NF * X0 * X1; q += NT) {
-      const int k = q / (X0 * X1);
-      const int r = q - k * X0 * X1;
-      const int a = r / X1, b = r - a * X1;
-      const int gi = x0 - HX + a, gj = y0 - HX + b;
-      T v = 0;
-      if (gi >= 0 && gi < n0 && gj >= 0 && gj < n1)
-        v = x_in.f[k][(size_t)gi * n1 + gj];
-      xs[q] = v;
+  const bool inner = interior_box(x0 - RJ, x0 + TX + RJ, y0 - RJ,
+                                  y0 + TY + RJ, n0, n1, Op::R);
+  auto OT = [&](int k, int s, int ea, int eb) -> T& {
+    return ot[((k * (NS - 1) + s - 1) * E0 + ea) * E1 + eb];
+  };
+  auto xat = [&](int k, int i, int j) -> T {
+    if (!inner) {  // coefficients reaching outside are zero; clamp reads
+      i = min(max(i, 0), n0 - 1);
+      j = min(max(j, 0), n1 - 1);
     }
-  __syncthreads();
+    return x_in.f[k][(size_t)i * n1 + j];
+  };
 
-  // Products of one E-region cell (E-local ea, eb): o[k][s] =
-  // sum_e J[e][k][s] v_e with v = J x (or J[e][k][s] itself for diag).
-  auto cell = [&](int ea, int eb, int gi, int gj, T (*o)[NS]) {
-    LdsAcc<T> acc{st, S0, S1, ea + Op::R, eb + Op::R};
+  // Products of one cell: o[k][s] = sum_e J[e][k][s] v_e with v = J x (or
+  // J[e][k][s]^2 summed over e for the diagonal).
+  auto cell = [&](int gi, int gj, T (*o)[NS]) {
     T f[NE];
     T J[NE][NF][NS];
-    Op::eval(acc, gi, gj, n0, n1, P, f, J);
+    eval_global<T, Op>(st_in, gi, gj, n0, n1, inner, P, f, J);
     if (diag) {
 #pragma unroll
       for (int k = 0; k < NF; ++k)
@@ -333,6 +339,12 @@ __global__ void __launch_bounds__(256)
         }
       return;
     }
+    T xv[NF][NS];
+#pragma unroll
+    for (int k = 0; k < NF; ++k)
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        xv[k][s] = xat(k, gi + Op::SX[s], gj + Op::SY[s]);
     T v[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
@@ -340,9 +352,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int k = 0; k < NF; ++k)
 #pragma unroll
-        for (int s = 0; s < NS; ++s)
-          a += J[e][k][s] *
-               xs[(k * X0 + ea + RJ + Op::SX[s]) * X1 + eb + RJ + Op::SY[s]];
+        for (int s = 0; s < NS; ++s) a += J[e][k][s] * xv[k][s];
       v[e] = a;
     }
 #pragma unroll
@@ -364,45 +374,30 @@ __global__ void __launch_bounds__(256)
     const int ea = a + RJ, eb = b + RJ;
     T o[NF][NS];
     if (gi < n0 && gj < n1) {
-      cell(ea, eb, gi, gj, o);
+      cell(gi, gj, o);
     } else {
       for (int k = 0; k < NF; ++k)
         for (int s = 0; s < NS; ++s) o[k][s] = 0;
     }
     for (int k = 0; k < NF; ++k) {
       cen[p][k] = o[k][0];
-      for (int s = 1; s < NS; ++s)
-        ot[((k * (NS - 1) + s - 1) * E0 + ea) * E1 + eb] = o[k][s];
+      for (int s = 1; s < NS; ++s) OT(k, s, ea, eb) = o[k][s];
     }
   }
   constexpr int NRING = E0 * E1 - TX * TY;
   for (int q = tid; q < NRING; q += NT) {
     int ea, eb;
-    if (q < RJ * E1) {
-      ea = q / E1;
-      eb = q - ea * E1;
-    } else if (q < 2 * RJ * E1) {
-      const int r = q - RJ * E1;
-      ea = TX + RJ + r / E1;
-      eb = r % E1;
-    } else {
-      const int r = q - 2 * RJ * E1;
-      const int row = r / (2 * RJ);
-      const int col = r - row * (2 * RJ);
-      ea = RJ + row;
-      eb = col < RJ ? col : TY + col;
-    }
+    ring_cell<TX, TY, RJ>(q, ea, eb);
     const int gi = x0 - RJ + ea, gj = y0 - RJ + eb;
     T o[NF][NS];
     if (gi >= 0 && gi < n0 && gj >= 0 && gj < n1) {
-      cell(ea, eb, gi, gj, o);
+      cell(gi, gj, o);
     } else {
       for (int k = 0; k < NF; ++k)
         for (int s = 0; s < NS; ++s) o[k][s] = 0;
     }
     for (int k = 0; k < NF; ++k)
-      for (int s = 1; s < NS; ++s)
-        ot[((k * (NS - 1) + s - 1) * E0 + ea) * E1 + eb] = o[k][s];
+      for (int s = 1; s < NS; ++s) OT(k, s, ea, eb) = o[k][s];
   }
   __syncthreads();
 
@@ -416,18 +411,15 @@ __global__ void __launch_bounds__(256)
     for (int k = 0; k < NF; ++k) {
       T g = cen[p][k];
 #pragma unroll
-      for (int s = 1; s < NS; ++s) {
-        const int ea = a + RJ - Op::SX[s], eb = b + RJ - Op::SY[s];
-        g += ot[((k * (NS - 1) + s - 1) * E0 + ea) * E1 + eb];
-      }
+      for (int s = 1; s < NS; ++s)
+        g += OT(k, s, a + RJ - Op::SX[s], b + RJ - Op::SY[s]);
       T out;
       if (mode == 0 || mode == 3)
         out = g;
       else if (mode == 1)
         out = b_in.f[k][c] - g;
       else
-        out = xs[(k * X0 + a + HX) * X1 + b + HX] +
-              omega * (b_in.f[k][c] - g) / d_in.f[k][c];
+        out = x_in.f[k][c] + omega * (b_in.f[k][c] - g) / d_in.f[k][c];
       y.f[k][c] = out;
     }
   }
@@ -435,12 +427,8 @@ __global__ void __launch_bounds__(256)
 
 template <class T, class Op, int TX, int TY>
 constexpr size_t normal_apply_smem() {
-  constexpr int H = Op::RJ + Op::R, HX = 2 * Op::RJ;
-  return sizeof(T) *
-         (size_t)(Op::NF * (TX + 2 * H) * (TY + 2 * H) +
-                  Op::NF * (TX + 2 * HX) * (TY + 2 * HX) +
-                  Op::NF * (Op::NS - 1) * (TX + 2 * Op::RJ) *
-                      (TY + 2 * Op::RJ));
+  return sizeof(T) * (size_t)(Op::NF * (Op::NS - 1) * (TX + 2 * Op::RJ) *
+                              (TY + 2 * Op::RJ));
 }
 
 // Deterministic final reduction of per-block partials: out[e] = sum_b

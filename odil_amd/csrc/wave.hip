@@ -85,12 +85,12 @@ struct WaveOp {
     J[0][0][4] = cxp;
   }
 
-  template <class Acc>
+  template <bool INT = false, class Acc>
   __device__ static inline void residual(const Acc& q, int i, int j, int n0,
                                          int n1, const Params& P, T* f) {
     compute(q, i, j, n0, n1, P, f, nullptr, false);
   }
-  template <class Acc>
+  template <bool INT = false, class Acc>
   __device__ static inline void eval(const Acc& q, int i, int j, int n0,
                                      int n1, const Params& P, T* f,
                                      T (*J)[NF][NS]) {
