@@ -44,6 +44,9 @@ def parse():
     p.add_argument('--no_tol_run', action='store_true')
     p.add_argument('--no_cpu', action='store_true')
     p.add_argument('--cpu_n', type=int, default=129)
+    p.add_argument('--no_graph', action='store_true',
+                   help='launch the linear solve eagerly instead of from a '
+                   'captured HIP graph (kernel tracers see every launch)')
     return p.parse_args()
 
 
@@ -55,7 +58,8 @@ def make_solver(args, precision):
     nm = NewtonMultigrid(op, nlvl=args.nlvl, maxiter=args.maxiter,
                          omega=args.omega, precision=precision,
                          linsolver=args.linsolver, galerkin_mode='exact',
-                         restriction='transpose')
+                         restriction='transpose',
+                         use_graph=not args.no_graph)
     fields = [torch.zeros(args.n, args.n, dtype=torch.float64,
                           device='cuda') for _ in range(3)]
     return op, nm, fields

@@ -45,40 +45,47 @@ __device__ inline bool inside(int a, int b, int c, const Grid3& g) {
 // mode 0: y = A x; mode 1: y = f - A x;
 // mode 2: weighted Jacobi y = x + omega (f - A x) / diag(A)
 //         (smoother_jacobi, multigrid.py:543-552; diag = entry dent[o]).
-// A group of LANES consecutive threads owns one (output field o, cell c)
-// pair, cells fastest so coefficient reads coalesce; the lanes split the
-// entries of o and combine by shuffles. Small grids use wide groups so the
-// serial chain of dependent loads per thread stays short. Loads use clamped
-// in-grid indices and a 0/1 weight instead of branches so they can all be
-// issued back to back.
+// Workgroup (bx, o) serves output field o and stages that field's entries
+// in LDS. A group of LANES consecutive threads owns one cell, cells fastest
+// so coefficient reads coalesce; the lanes split the entries and combine by
+// shuffles. Small grids use wide groups so the serial chain of dependent
+// loads per thread stays short. Loads use clamped in-grid indices and a 0/1
+// weight instead of branches so they can all be issued back to back.
+constexpr int kMaxEntries = 512;
+
 template <class T, int LANES>
 __global__ void __launch_bounds__(256)
     k_apply(const int* __restrict__ ent, const int* __restrict__ ostart,
-            const int* __restrict__ dent, int nout, const T* __restrict__ coef,
+            const int* __restrict__ dent, const T* __restrict__ coef,
             CVecPtrs<T> x, VecPtrs<T> y, CVecPtrs<T> f, Grid3 nw, Grid3 nwh,
             int stride, int mode, T omega) {
-  const size_t gid = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
-  const int lane = threadIdx.x % LANES;
+  __shared__ int4 tab[kMaxEntries];  // (in, s0, s1, s2)
+  const int o = blockIdx.y;
+  const int e0 = ostart[o];
+  const int ne = ostart[o + 1] - e0;
+  for (int t = threadIdx.x; t < ne; t += blockDim.x) {
+    const int* it = ent + 5 * (size_t)(e0 + t);
+    tab[t] = make_int4(it[1], it[2], it[3], it[4]);
+  }
+  __syncthreads();
   const size_t Nh = nwh.size();
-  if (gid >= Nh * (size_t)nout) return;  // whole groups exit together
-  const int o = (int)(gid / Nh);
-  const size_t c = gid - (size_t)o * Nh;
+  const size_t c = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
+  const int lane = threadIdx.x % LANES;
+  if (c >= Nh) return;  // whole groups exit together
   int i0, i1, i2;
   unflat(c, nwh, i0, i1, i2);
   const int b0 = stride * i0, b1 = stride * i1, b2 = stride * i2;
   T acc = 0;
-  const int e1 = ostart[o + 1];
 #pragma unroll 4
-  for (int e = ostart[o] + lane; e < e1; e += LANES) {
-    const int k = ent[5 * e + 1];
-    const int p0 = b0 + ent[5 * e + 2], p1 = b1 + ent[5 * e + 3],
-              p2 = b2 + ent[5 * e + 4];
+  for (int e = lane; e < ne; e += LANES) {
+    const int4 it = tab[e];
+    const int p0 = b0 + it.y, p1 = b1 + it.z, p2 = b2 + it.w;
     const bool ok = inside(p0, p1, p2, nw);
     const int q0 = min(max(p0, 0), nw.n0 - 1);
     const int q1 = min(max(p1, 0), nw.n1 - 1);
     const int q2 = min(max(p2, 0), nw.n2 - 1);
-    const T cv = coef[(size_t)e * Nh + c];
-    const T xv = x.p[k][((size_t)q0 * nw.n1 + q1) * nw.n2 + q2];
+    const T cv = coef[(size_t)(e0 + e) * Nh + c];
+    const T xv = x.p[it.x][((size_t)q0 * nw.n1 + q1) * nw.n2 + q2];
     acc += (ok ? cv : T(0)) * xv;
   }
 #pragma unroll
@@ -91,6 +98,102 @@ __global__ void __launch_bounds__(256)
     y.p[o][c] = f.p[o][c] - acc;
   } else {
     const T d = coef[(size_t)dent[o] * Nh + c];
+    y.p[o][c] = x.p[o][c] + omega * (f.p[o][c] - acc) / d;
+  }
+}
+
+// Symmetric square operator (stride 1) stored as its upper half: entries
+// (o, i, s) with o < i, or o == i and s >= 0 (lexicographic). The lower
+// half follows from A_(i,o,-s)(c) = A_(o,i,s)(c - s). Items list, per output
+// field o, every product that reaches it:
+//   direct:     coef_q[c]     * x_var[c + s]
+//   transposed: coef_q[c - s] * x_var[c - s]
+// as int32 [nitem][10] = (q, var, cs_lin, xs_lin, cs0, cs1, cs2, xs0, xs1,
+// xs2): coefficient point c + cs, x point c + xs (cs_lin, xs_lin the same
+// offsets flattened). `reach` bounds |cs|, |xs| per axis; cells at least
+// that far from every face use the flattened offsets without bounds checks.
+// Modes as k_apply.
+// Workgroup (bx, o) serves output field o: its item list is staged in LDS
+// once as flat offsets (q N + cs_lin, var N + xs_lin), so an inner cell
+// reads coef[c + a] and xb[c + b] with 32-bit index arithmetic. CONTIG: the
+// x fields are one allocation, x_k = x_0 + k N (otherwise the field pointer
+// is selected per item). A group of LANES threads owns one cell.
+constexpr int kMaxItems = 512;
+
+template <class T>
+__device__ inline const T* pick(const CVecPtrs<T>& x, int k) {
+  const T* p = x.p[0];
+  p = k == 1 ? x.p[1] : p;
+  p = k == 2 ? x.p[2] : p;
+  p = k == 3 ? x.p[3] : p;
+  return p;
+}
+
+template <class T, int LANES, bool CONTIG>
+__global__ void __launch_bounds__(256)
+    k_apply_sym(const int* __restrict__ items, const int* __restrict__ ostart,
+                const int* __restrict__ dent, const T* __restrict__ coef,
+                CVecPtrs<T> x, VecPtrs<T> y, CVecPtrs<T> f, Grid3 nw,
+                Grid3 reach, int mode, T omega) {
+  __shared__ int2 fast[kMaxItems];
+  __shared__ int full[kMaxItems * 8];  // (q, var, cs0..2, xs0..2)
+  const int o = blockIdx.y;
+  const int i0e = ostart[o];
+  const int ne = ostart[o + 1] - i0e;
+  const int N = (int)nw.size();
+  for (int t = threadIdx.x; t < ne; t += blockDim.x) {
+    const int* it = items + 10 * (i0e + t);
+    fast[t] = make_int2(it[0] * N + it[2], (CONTIG ? it[1] * N : 0) + it[3]);
+    full[8 * t] = it[0];
+    full[8 * t + 1] = it[1];
+    for (int k = 0; k < 6; ++k) full[8 * t + 2 + k] = it[4 + k];
+  }
+  __syncthreads();
+  const int c = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) / LANES);
+  const int lane = threadIdx.x % LANES;
+  if (c >= N) return;  // whole groups exit together
+  int i0, i1, i2;
+  unflat(c, nw, i0, i1, i2);
+  const bool inner = i0 >= reach.n0 && i0 < nw.n0 - reach.n0 &&
+                     i1 >= reach.n1 && i1 < nw.n1 - reach.n1 &&
+                     i2 >= reach.n2 && i2 < nw.n2 - reach.n2;
+  T acc = 0;
+  if (inner) {
+    if (CONTIG) {
+      const T* __restrict__ xb = x.p[0];
+#pragma unroll 8
+      for (int e = lane; e < ne; e += LANES) {
+        const int2 it = fast[e];
+        acc += coef[(unsigned)(c + it.x)] * xb[(unsigned)(c + it.y)];
+      }
+    } else {
+#pragma unroll 4
+      for (int e = lane; e < ne; e += LANES) {
+        const int2 it = fast[e];
+        acc += coef[(unsigned)(c + it.x)] *
+               pick(x, full[8 * e + 1])[(unsigned)(c + it.y)];
+      }
+    }
+  } else {
+    for (int e = lane; e < ne; e += LANES) {
+      const int* fl = full + 8 * e;
+      const int a0 = i0 + fl[2], a1 = i1 + fl[3], a2 = i2 + fl[4];
+      const int b0 = i0 + fl[5], b1 = i1 + fl[6], b2 = i2 + fl[7];
+      if (!inside(a0, a1, a2, nw) || !inside(b0, b1, b2, nw)) continue;
+      acc += coef[(size_t)fl[0] * N + ((size_t)a0 * nw.n1 + a1) * nw.n2 + a2] *
+             pick(x, fl[1])[((size_t)b0 * nw.n1 + b1) * nw.n2 + b2];
+    }
+  }
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1)
+    acc += __shfl_xor(acc, off, LANES);
+  if (lane != 0) return;
+  if (mode == 0) {
+    y.p[o][c] = acc;
+  } else if (mode == 1) {
+    y.p[o][c] = f.p[o][c] - acc;
+  } else {
+    const T d = coef[(size_t)dent[o] * N + c];
     y.p[o][c] = x.p[o][c] + omega * (f.p[o][c] - acc) / d;
   }
 }
@@ -143,12 +246,12 @@ __global__ void __launch_bounds__(256)
     k_normal(const int* __restrict__ terms, const int* __restrict__ tstart,
              int nout, const Tin* __restrict__ jc, Tout* __restrict__ out,
              Grid3 g) {
-  // One thread per (output entry o, cell m), cells fastest.
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // Workgroup (bx, o): output entry o (uniform, so the term table is read
+  // with scalar loads), cells m along x.
   const size_t N = g.size();
-  if (t >= N * (size_t)nout) return;
-  const int o = (int)(t / N);
-  const size_t m = t - (size_t)o * N;
+  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int o = blockIdx.y;
+  if (m >= N) return;
   int m0, m1, m2;
   unflat(m, g, m0, m1, m2);
   double acc = 0;
@@ -166,62 +269,61 @@ __global__ void __launch_bounds__(256)
 // Galerkin coarse operator on node-based grids nw = 2 nwh - 1.
 // sigma = +1: A_c = R A T. sigma = -1: the arithmetic of the reference's
 // coarsen_sparse_operator (multigrid.py:390-427), which couples fine node a
-// to a - s_e and coarse node I to I - dc, i.e. reflect(R reflect(A) T).
-// Output entry o = (i, j, dc); terms int32 [nterm][5] = (e, r0, r1, r2, wt_id)
-// with fine entry e of block (i, j), restriction offset r, interpolation
-// weight wtab[wt_id] = 0.5^|t| for t = r + sigma (s_e - 2 dc).
+// to a - s and coarse node I to I - dc, i.e. reflect(R reflect(A) T).
+// Output entry o = (i, j, dc); terms int32 [nterm][9] =
+// (q, r0, r1, r2, wt_id, s0, s1, s2, tr): fine coefficient array q, logical
+// fine entry of block (i, j) at shift s, restriction offset r,
+// interpolation weight wtab[wt_id] = 0.5^|t| for t = r + sigma (s - 2 dc).
+// tr = 1 reads the logical entry from the stored upper half of a symmetric
+// operator, A_(i,j,s)(a) = A_(j,i,-s)(a + s) (sigma = +1 only).
 // Restriction weights rw[(r0+1)*9+(r1+1)*3+(r2+1)] at inner coarse nodes;
 // at coarse boundary nodes rmode 0 injects (r = 0, weight 1, the reference
 // get_R, multigrid.py:525-532) and rmode 1 keeps the weights (fine points
 // outside the grid dropped), i.e. R = T^T / 2^dim.
 template <class T>
 __global__ void __launch_bounds__(256)
-    k_galerkin(const int* __restrict__ fent, const int* __restrict__ oent,
-               const int* __restrict__ terms, const int* __restrict__ tstart,
-               int nout, const double* __restrict__ rw,
-               const double* __restrict__ wtab, const T* __restrict__ fc,
-               T* __restrict__ cc, Grid3 nw, Grid3 nwh, int dim, int sigma,
-               int rmode) {
-  // One thread per (output entry o, coarse node I), nodes fastest.
-  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    k_galerkin(const int* __restrict__ oent, const int* __restrict__ terms,
+               const int* __restrict__ tstart, int nout,
+               const double* __restrict__ rw, const double* __restrict__ wtab,
+               const T* __restrict__ fc, T* __restrict__ cc, Grid3 nw,
+               Grid3 nwh, int dim, int sigma, int rmode) {
+  // Workgroup (bx, o): output entry o (uniform, so the term table is read
+  // with scalar loads), coarse nodes I along x.
   const size_t Nc = nwh.size();
-  if (tid >= Nc * (size_t)nout) return;
-  const int o = (int)(tid / Nc);
-  const size_t I = tid - (size_t)o * Nc;
+  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int o = blockIdx.y;
+  if (I >= Nc) return;
   int I0, I1, I2;
   unflat(I, nwh, I0, I1, I2);
   const bool inner = (I0 > 0 && I0 < nwh.n0 - 1) &&
                      (I1 > 0 && I1 < nwh.n1 - 1) &&
                      (dim < 3 || (I2 > 0 && I2 < nwh.n2 - 1));
   const size_t Nf = nw.size();
-  {
-    const int J0 = I0 + sigma * oent[5 * o + 2],
-              J1 = I1 + sigma * oent[5 * o + 3],
-              J2 = I2 + sigma * oent[5 * o + 4];
-    double acc = 0;
-    if (inside(J0, J1, J2, nwh)) {
-      for (int t = tstart[o]; t < tstart[o + 1]; ++t) {
-        const int e = terms[5 * t];
-        const int r0 = terms[5 * t + 1], r1 = terms[5 * t + 2],
-                  r2 = terms[5 * t + 3];
-        double w;
-        if (inner || rmode == 1)
-          w = rw[(r0 + 1) * 9 + (r1 + 1) * 3 + (r2 + 1)];
-        else
-          w = (r0 == 0 && r1 == 0 && r2 == 0) ? 1.0 : 0.0;
-        if (w == 0.0) continue;
-        const int a0 = 2 * I0 + r0, a1 = 2 * I1 + r1, a2 = 2 * I2 + r2;
-        if (!inside(a0, a1, a2, nw)) continue;
-        const int b0 = a0 + sigma * fent[5 * e + 2],
-                  b1 = a1 + sigma * fent[5 * e + 3],
-                  b2 = a2 + sigma * fent[5 * e + 4];
-        if (!inside(b0, b1, b2, nw)) continue;
-        const size_t a = ((size_t)a0 * nw.n1 + a1) * nw.n2 + a2;
-        acc += w * wtab[terms[5 * t + 4]] * (double)fc[(size_t)e * Nf + a];
-      }
+  const int J0 = I0 + sigma * oent[5 * o + 2],
+            J1 = I1 + sigma * oent[5 * o + 3],
+            J2 = I2 + sigma * oent[5 * o + 4];
+  double acc = 0;
+  if (inside(J0, J1, J2, nwh)) {
+    for (int t = tstart[o]; t < tstart[o + 1]; ++t) {
+      const int* tm = terms + 9 * t;
+      const int r0 = tm[1], r1 = tm[2], r2 = tm[3];
+      double w;
+      if (inner || rmode == 1)
+        w = rw[(r0 + 1) * 9 + (r1 + 1) * 3 + (r2 + 1)];
+      else
+        w = (r0 == 0 && r1 == 0 && r2 == 0) ? 1.0 : 0.0;
+      if (w == 0.0) continue;
+      const int a0 = 2 * I0 + r0, a1 = 2 * I1 + r1, a2 = 2 * I2 + r2;
+      if (!inside(a0, a1, a2, nw)) continue;
+      const int b0 = a0 + sigma * tm[5], b1 = a1 + sigma * tm[6],
+                b2 = a2 + sigma * tm[7];
+      if (!inside(b0, b1, b2, nw)) continue;
+      const size_t a = tm[8] ? ((size_t)b0 * nw.n1 + b1) * nw.n2 + b2
+                             : ((size_t)a0 * nw.n1 + a1) * nw.n2 + a2;
+      acc += w * wtab[tm[4]] * (double)fc[(size_t)tm[0] * Nf + a];
     }
-    cc[(size_t)o * Nc + I] = (T)acc;
   }
+  cc[(size_t)o * Nc + I] = (T)acc;
 }
 
 // Restriction of nvar fields (R.mul_field, multigrid.py:496-534).
@@ -286,11 +388,13 @@ __global__ void __launch_bounds__(256)
 }
 
 // Dense matrix of a square block stencil operator (stride 1), row-major,
-// n = nvar * N; used for the coarsest-level direct solve.
+// n = nvar * N; used for the coarsest-level direct solve. sym != 0: the
+// entries are the upper half of a symmetric operator.
 template <class T>
 __global__ void __launch_bounds__(256)
     k_to_dense(const int* __restrict__ ent, int nent, int nvar,
-               const T* __restrict__ coef, double* __restrict__ M, Grid3 g) {
+               const T* __restrict__ coef, double* __restrict__ M, Grid3 g,
+               int sym) {
   const size_t N = g.size();
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= N * nent) return;
@@ -305,8 +409,11 @@ __global__ void __launch_bounds__(256)
                      ((size_t)q0 * g.n1 + q1) * g.n2 + q2;
   const size_t row = (size_t)ent[5 * e] * N + c;
   // Entries of one operator have distinct (out, in, shift), so every matrix
-  // element is written at most once.
-  M[row * (nvar * N) + col] = (double)coef[(size_t)e * N + c];
+  // element is written at most once (an upper-half symmetric operator
+  // writes the mirrored element of each off-diagonal entry too).
+  const double v = (double)coef[(size_t)e * N + c];
+  M[row * (nvar * N) + col] = v;
+  if (sym && row != col) M[col * (nvar * N) + row] = v;
 }
 
 // y = M x for a dense row-major FP64 matrix, one wavefront per row; x and y
@@ -388,11 +495,12 @@ extern "C" {
 
 // y = A x (mode 0), y = f - A x (mode 1), or one weighted-Jacobi sweep
 // y = x + omega (f - A x) / diag (mode 2, stride 1, nvar_in == nvar_out).
-// `ent`, `ostart`, `dent` are device int32 arrays (see file comment).
+// `ent`, `ostart`, `dent` are device int32 arrays (see file comment);
+// `max_entries` is the largest entry count of one output field (<= 512).
 int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
                        int nvar_in, int nvar_out, const int* ent,
-                       const int* ostart, const int* dent, const void* coef,
-                       const void* const* x, void* const* y,
+                       const int* ostart, const int* dent, int max_entries,
+                       const void* coef, const void* const* x, void* const* y,
                        const void* const* f, int mode, double omega,
                        void* stream) {
   if (bad_dim(ndim, "stencil_apply") || bad_var(nvar_in, "stencil_apply") ||
@@ -401,6 +509,11 @@ int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
   if (stride != 1 && stride != 2) {
     set_error("stencil_apply: stride must be 1 or 2");
     return kErrArg;
+  }
+  if (max_entries < 0 || max_entries > kMaxEntries) {
+    set_error("stencil_apply: %d entries per field exceed %d", max_entries,
+              kMaxEntries);
+    return kErrCapacity;
   }
   if (mode == 2 && (stride != 1 || nvar_in != nvar_out || !dent)) {
     set_error("stencil_apply: Jacobi needs stride 1, square blocks, diag");
@@ -423,12 +536,12 @@ int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
     set_error("stencil_apply: bad dtype %d", dtype);
     return kErrArg;
   }
-  dim3 grid(nblk(groups * lanes));
+  dim3 grid(nblk(gh.size() * lanes), nvar_out);
 #define ODIL_APPLY(T, L)                                                    \
   hipLaunchKernelGGL((k_apply<T, L>), grid, dim3(256), 0, s, ent, ostart,   \
-                     dent, nvar_out, static_cast<const T*>(coef),           \
-                     cvp<T>(x, nvar_in), vp<T>(y, nvar_out),                \
-                     cvp<T>(f, nvar_out), g, gh, stride, mode, (T)omega)
+                     dent, static_cast<const T*>(coef), cvp<T>(x, nvar_in), \
+                     vp<T>(y, nvar_out), cvp<T>(f, nvar_out), g, gh,        \
+                     stride, mode, (T)omega)
 #define ODIL_APPLY_L(T)              \
   switch (lanes) {                   \
     case 1: ODIL_APPLY(T, 1); break; \
@@ -445,6 +558,90 @@ int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
 #undef ODIL_APPLY_L
 #undef ODIL_APPLY
   return check_launch("stencil_apply");
+}
+
+// Symmetric operator stored as its upper half (stride 1, square): same
+// modes as odil_stencil_apply. `items` int32 [nitem][10] and `ostart`
+// [nvar + 1] list the products reaching each output field (see
+// k_apply_sym); `reach` (host, 3 ints) bounds |cs|, |xs| per axis;
+// `max_items` is the longest per-field list (at most 512); `nent` the number
+// of coefficient arrays. x fields laid out as one allocation (x_k = x_0 +
+// k N) take the fast path.
+int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
+                           const int* items, const int* ostart,
+                           const int* dent, const int* reach, int max_items,
+                           int nent, const void* coef, const void* const* x,
+                           void* const* y, const void* const* f, int mode,
+                           double omega, void* stream) {
+  if (bad_dim(ndim, "stencil_apply_sym") ||
+      bad_var(nvar, "stencil_apply_sym"))
+    return kErrArg;
+  if (mode == 2 && !dent) {
+    set_error("stencil_apply_sym: Jacobi needs the diagonal entries");
+    return kErrArg;
+  }
+  if ((mode == 1 || mode == 2) && !f) {
+    set_error("stencil_apply_sym: mode %d needs f", mode);
+    return kErrArg;
+  }
+  if (max_items < 0 || max_items > kMaxItems) {
+    set_error("stencil_apply_sym: %d products per field exceed %d",
+              max_items, kMaxItems);
+    return kErrCapacity;
+  }
+  if (!reach || reach[0] < 0 || reach[1] < 0 || reach[2] < 0 ||
+      (ndim == 2 && reach[2] != 0)) {
+    set_error("stencil_apply_sym: bad reach");
+    return kErrArg;
+  }
+  Grid3 g = mk_grid(ndim, nw);
+  Grid3 rc{reach[0], reach[1], reach[2]};
+  // Flat 32-bit offsets: every coefficient and x element index < 2^31.
+  if ((double)g.size() * (double)(nent > nvar ? nent : nvar) > 2.0e9) {
+    set_error("stencil_apply_sym: grid too large for 32-bit offsets");
+    return kErrCapacity;
+  }
+  const size_t esz = dtype == kF64 ? 8 : 4;
+  bool contig = true;
+  for (int k = 1; k < nvar; ++k)
+    contig = contig && static_cast<const char*>(x[k]) ==
+                           static_cast<const char*>(x[0]) + k * g.size() * esz;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t groups = g.size() * (size_t)nvar;
+  int lanes = 1;
+  while (lanes < 16 && groups * lanes < (1u << 20)) lanes *= 2;
+  if (dtype != kF64 && dtype != kF32) {
+    set_error("stencil_apply_sym: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  dim3 grid(nblk(g.size() * lanes), nvar);
+#define ODIL_APPLY2(T, L, C)                                                 \
+  hipLaunchKernelGGL((k_apply_sym<T, L, C>), grid, dim3(256), 0, s, items,   \
+                     ostart, dent, static_cast<const T*>(coef),              \
+                     cvp<T>(x, nvar), vp<T>(y, nvar), cvp<T>(f, nvar), g,    \
+                     rc, mode, (T)omega)
+#define ODIL_APPLY(T, L)      \
+  if (contig)                 \
+    ODIL_APPLY2(T, L, true);  \
+  else                        \
+    ODIL_APPLY2(T, L, false)
+#define ODIL_APPLY_L(T)              \
+  switch (lanes) {                   \
+    case 1: ODIL_APPLY(T, 1); break; \
+    case 2: ODIL_APPLY(T, 2); break; \
+    case 4: ODIL_APPLY(T, 4); break; \
+    case 8: ODIL_APPLY(T, 8); break; \
+    default: ODIL_APPLY(T, 16); break; \
+  }
+  if (dtype == kF64) {
+    ODIL_APPLY_L(double);
+  } else {
+    ODIL_APPLY_L(float);
+  }
+#undef ODIL_APPLY_L
+#undef ODIL_APPLY
+#undef ODIL_APPLY2
+  return check_launch("stencil_apply_sym");
 }
 
 // y = A^T x (accumulate != 0: y += A^T x). `tent`/`istart` group entry
@@ -491,9 +688,13 @@ int odil_stencil_normal(int dtype_in, int dtype_out, int ndim, const int* nw,
                         const int* terms, const int* tstart, int nout,
                         const void* jc, void* out, void* stream) {
   if (bad_dim(ndim, "stencil_normal")) return kErrArg;
+  if (nout < 1 || nout > 65535) {
+    set_error("stencil_normal: bad number of output entries %d", nout);
+    return kErrArg;
+  }
   Grid3 g = mk_grid(ndim, nw);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  dim3 grid(nblk(g.size() * (size_t)nout));
+  dim3 grid(nblk(g.size()), nout);
   if (dtype_in == kF64 && dtype_out == kF64)
     hipLaunchKernelGGL((k_normal<double, double>), grid, dim3(256), 0, s,
                        terms, tstart, nout, static_cast<const double*>(jc),
@@ -515,13 +716,13 @@ int odil_stencil_normal(int dtype_in, int dtype_out, int ndim, const int* nw,
 }
 
 // Galerkin coarse operator; `rw` (27 doubles) inner restriction weights,
-// `wtab` interpolation weights indexed by the terms; sigma = +1 exact
-// R A T, -1 the reference's stencil coarsening.
-int odil_stencil_galerkin(int dtype, int ndim, const int* nw, const int* fent,
-                          const int* oent, const int* terms,
-                          const int* tstart, int nout, const double* rw,
-                          const double* wtab, const void* fc, void* cc,
-                          int sigma, int rmode, void* stream) {
+// `wtab` interpolation weights indexed by the terms (int32 [nterm][9], see
+// k_galerkin); sigma = +1 exact R A T, -1 the reference's stencil
+// coarsening.
+int odil_stencil_galerkin(int dtype, int ndim, const int* nw, const int* oent,
+                          const int* terms, const int* tstart, int nout,
+                          const double* rw, const double* wtab, const void* fc,
+                          void* cc, int sigma, int rmode, void* stream) {
   if (sigma != 1 && sigma != -1) {
     set_error("stencil_galerkin: sigma must be +1 or -1");
     return kErrArg;
@@ -532,19 +733,21 @@ int odil_stencil_galerkin(int dtype, int ndim, const int* nw, const int* fent,
     set_error("stencil_galerkin: node grids need odd sizes");
     return kErrArg;
   }
+  if (nout < 1 || nout > 65535) {
+    set_error("stencil_galerkin: bad number of output entries %d", nout);
+    return kErrArg;
+  }
   Grid3 gh{g.n0 / 2 + 1, g.n1 / 2 + 1, ndim == 3 ? g.n2 / 2 + 1 : 1};
   hipStream_t s = static_cast<hipStream_t>(stream);
-  dim3 grid(nblk(gh.size() * (size_t)nout));
+  dim3 grid(nblk(gh.size()), nout);
   if (dtype == kF64)
-    hipLaunchKernelGGL(k_galerkin<double>, grid, dim3(256), 0, s, fent, oent,
-                       terms, tstart, nout, rw, wtab,
-                       static_cast<const double*>(fc), static_cast<double*>(cc),
-                       g, gh, ndim, sigma, rmode);
+    hipLaunchKernelGGL(k_galerkin<double>, grid, dim3(256), 0, s, oent, terms,
+                       tstart, nout, rw, wtab, static_cast<const double*>(fc),
+                       static_cast<double*>(cc), g, gh, ndim, sigma, rmode);
   else if (dtype == kF32)
-    hipLaunchKernelGGL(k_galerkin<float>, grid, dim3(256), 0, s, fent, oent,
-                       terms, tstart, nout, rw, wtab,
-                       static_cast<const float*>(fc), static_cast<float*>(cc),
-                       g, gh, ndim, sigma, rmode);
+    hipLaunchKernelGGL(k_galerkin<float>, grid, dim3(256), 0, s, oent, terms,
+                       tstart, nout, rw, wtab, static_cast<const float*>(fc),
+                       static_cast<float*>(cc), g, gh, ndim, sigma, rmode);
   else {
     set_error("stencil_galerkin: bad dtype %d", dtype);
     return kErrArg;
@@ -603,9 +806,10 @@ int odil_mg_prolong_add(int dtype, int ndim, const int* nw, int nvar,
 }
 
 // Dense FP64 matrix (zero-initialised by the caller) of a square stride-1
-// block stencil operator.
+// block stencil operator; sym != 0 when `ent` is the upper half of a
+// symmetric operator.
 int odil_stencil_to_dense(int dtype, int ndim, const int* nw, int nvar,
-                          const int* ent, int nent, const void* coef,
+                          const int* ent, int nent, const void* coef, int sym,
                           double* M, void* stream) {
   if (bad_dim(ndim, "stencil_to_dense") || bad_var(nvar, "stencil_to_dense"))
     return kErrArg;
@@ -614,10 +818,10 @@ int odil_stencil_to_dense(int dtype, int ndim, const int* nw, int nvar,
   dim3 grid(nblk(g.size() * nent));
   if (dtype == kF64)
     hipLaunchKernelGGL(k_to_dense<double>, grid, dim3(256), 0, s, ent, nent,
-                       nvar, static_cast<const double*>(coef), M, g);
+                       nvar, static_cast<const double*>(coef), M, g, sym);
   else if (dtype == kF32)
     hipLaunchKernelGGL(k_to_dense<float>, grid, dim3(256), 0, s, ent, nent,
-                       nvar, static_cast<const float*>(coef), M, g);
+                       nvar, static_cast<const float*>(coef), M, g, sym);
   else {
     set_error("stencil_to_dense: bad dtype %d", dtype);
     return kErrArg;

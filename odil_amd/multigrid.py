@@ -26,14 +26,30 @@ def _shift3(s):
     return s + (0, ) * (3 - len(s))
 
 
+def is_upper(o, i, s):
+    """Entry (o, i, s) belongs to the stored upper half of a symmetric block
+    operator: o < i, or o == i and s >= 0 (lexicographic)."""
+    return o < i or (o == i and tuple(s) >= (0, ) * len(s))
+
+
+def _mirror(o, i, s):
+    return (i, o, tuple(-x for x in s))
+
+
 class BlockStencil:
     """Block stencil operator from nvar_in fields on grid `nw` to nvar_out
     fields on grid `nwh = (nw - 1) // stride + 1`:
     (A x)_o[I] = sum_{e: out_e = o} coef_e[I] x_{in_e}[stride*I + s_e],
-    zero outside the grid (non-circular, multigrid.py:941-969)."""
+    zero outside the grid (non-circular, multigrid.py:941-969).
+
+    sym=True: a symmetric square stride-1 operator stored as its upper half
+    (`is_upper` entries); the lower half is A_(i,o,-s)(c) = A_(o,i,s)(c-s).
+    Halves the coefficient memory and traffic of J^T J and its Galerkin
+    coarse operators."""
 
     def __init__(self, coef, entries, nw, stride=1, nvar_in=None,
-                 nvar_out=None):
+                 nvar_out=None, sym=False):
+        self.sym = bool(sym)
         self.nw = tuple(int(n) for n in nw)
         self.ndim = len(self.nw)
         self.stride = int(stride)
@@ -58,6 +74,14 @@ class BlockStencil:
             max(e[0] for e in entries) + 1 if entries else 1)
         self.nvar_in = nvar_in if nvar_in is not None else (
             max(e[1] for e in entries) + 1 if entries else 1)
+        if self.sym:
+            if self.stride != 1:
+                raise ValueError('symmetric storage needs stride 1')
+            self.nvar_in = self.nvar_out = max(self.nvar_in, self.nvar_out)
+            bad = [e for e in entries if not is_upper(*e)]
+            if bad:
+                raise ValueError('symmetric storage holds upper entries '
+                                 'only, got {}'.format(bad[:3]))
         self._tables()
 
     @property
@@ -89,12 +113,41 @@ class BlockStencil:
             istart[i + 1] += 1
         istart = np.cumsum(istart).astype(np.int32)
         self._dent_host = dent
+        self._max_entries = int(np.max(np.diff(ostart))) if len(ostart) > 1 \
+            else 0
         t = lambda a: torch.tensor(a, dtype=torch.int32, device=dev)
+        if self.sym:
+            self._sym_tables(t)
         self.t_ent = t(ent)
         self.t_ostart = t(ostart)
         self.t_dent = t(dent)
         self.t_tent = t(tent if len(tent) else np.zeros(1, np.int32))
         self.t_istart = t(istart)
+
+    def _sym_tables(self, t):
+        """Per-output product list of the symmetric apply (k_apply_sym)."""
+        n3 = list(self.nw) + [1] * (3 - self.ndim)
+        lin = lambda v: (v[0] * n3[1] + v[1]) * n3[2] + v[2]
+        per = [[] for _ in range(self.nvar_out)]
+        zero3 = (0, 0, 0)
+        for q, (o, i, s) in enumerate(self.entries):
+            s3 = _shift3(s)
+            per[o].append((q, i, zero3, s3))
+            if not (o == i and all(x == 0 for x in s)):
+                m3 = tuple(-x for x in s3)
+                per[i].append((q, o, m3, m3))
+        rows, ostart = [], [0]
+        reach = [0, 0, 0]
+        for lst in per:
+            for q, var, cs, xs in lst:
+                rows.append([q, var, lin(cs), lin(xs)] + list(cs) + list(xs))
+                reach = [max(r, abs(a), abs(b))
+                         for r, a, b in zip(reach, cs, xs)]
+            ostart.append(len(rows))
+        self.t_items = t(np.array(rows, dtype=np.int32).reshape(-1, 10))
+        self.t_iostart = t(np.array(ostart, dtype=np.int32))
+        self._reach = (ctypes.c_int * 3)(*reach)
+        self._max_items = max(b - a for a, b in zip(ostart[:-1], ostart[1:]))
 
     # -- products --------------------------------------------------------
     def _fields(self, x, n, shape):
@@ -118,11 +171,23 @@ class BlockStencil:
                 for _ in range(n)]
 
     def _call_apply(self, x, y, f, mode, omega):
+        if self.sym:
+            st = _lib.lib().odil_stencil_apply_sym(
+                c_int(_lib.dtype_tag(self.dtype)), c_int(self.ndim),
+                _grid_arr(self.nw), c_int(self.nvar_out), ptr(self.t_items),
+                ptr(self.t_iostart), ptr(self.t_dent), self._reach,
+                c_int(self._max_items), c_int(len(self.entries)),
+                ptr(self.coef), ptrs(x), ptrs(y),
+                ptrs(f) if f is not None else None, c_int(mode),
+                c_double(omega), stream())
+            check(st, 'stencil_apply_sym')
+            return
         st = _lib.lib().odil_stencil_apply(
             c_int(_lib.dtype_tag(self.dtype)), c_int(self.ndim),
             _grid_arr(self.nw), c_int(self.stride), c_int(self.nvar_in),
             c_int(self.nvar_out), ptr(self.t_ent), ptr(self.t_ostart),
-            ptr(self.t_dent), ptr(self.coef), ptrs(x), ptrs(y),
+            ptr(self.t_dent), c_int(self._max_entries), ptr(self.coef),
+            ptrs(x), ptrs(y),
             ptrs(f) if f is not None else None, c_int(mode),
             c_double(omega), stream())
         check(st, 'stencil_apply')
@@ -153,6 +218,13 @@ class BlockStencil:
     def apply_t(self, x, y=None, accumulate=False):
         """y = A^T x (mul_transpose_field); accumulate adds into y."""
         x = self._fields(x, self.nvar_out, self.nwh)
+        if self.sym:
+            if not accumulate or y is None:
+                return self.apply(x, y)
+            t = self.apply(x)
+            for a, b in zip(y, t):
+                a.add_(b)
+            return y
         if y is None:
             y = self._new(self.nvar_in, self.nw)
             accumulate = False
@@ -174,7 +246,8 @@ class BlockStencil:
         st = _lib.lib().odil_stencil_to_dense(
             c_int(_lib.dtype_tag(self.dtype)), c_int(self.ndim),
             _grid_arr(self.nw), c_int(self.nvar_out), ptr(self.t_ent),
-            c_int(len(self.entries)), ptr(self.coef), ptr(M), stream())
+            c_int(len(self.entries)), ptr(self.coef),
+            c_int(1 if self.sym else 0), ptr(M), stream())
         check(st, 'stencil_to_dense')
         return M
 
@@ -182,7 +255,8 @@ class BlockStencil:
         if dtype == self.dtype:
             return self
         return BlockStencil(self.coef.to(dtype), self.entries, self.nw,
-                            self.stride, self.nvar_in, self.nvar_out)
+                            self.stride, self.nvar_in, self.nvar_out,
+                            sym=self.sym)
 
     def tocsr(self):
         """SciPy CSR copy on the host (tests and the CSR export)."""
@@ -212,6 +286,9 @@ class BlockStencil:
             (np.concatenate(vals), (np.concatenate(rows),
                                     np.concatenate(cols))), shape=shape)
         m.sum_duplicates()
+        if self.sym:
+            d = scipy.sparse.diags(m.diagonal())
+            m = (m + m.T - d).tocsr()
         return m
 
     @staticmethod
@@ -224,8 +301,8 @@ class BlockStencil:
 _PLANS = {}
 
 
-def _normal_plan(J):
-    key = ('normal', tuple(J.entries), str(J.device))
+def _normal_plan(J, sym=False):
+    key = ('normal', tuple(J.entries), str(J.device), sym)
     if key not in _PLANS:
         out = {}
         for a, (ea, ka, sa) in enumerate(J.entries):
@@ -234,7 +311,7 @@ def _normal_plan(J):
                     continue
                 d = tuple(y - x for x, y in zip(sa, sb))
                 out.setdefault((ka, kb, d), []).append((a, b) + _shift3(sa))
-        keys = sorted(out)
+        keys = sorted(k for k in out if not sym or is_upper(*k))
         terms = []
         tstart = [0]
         for k in keys:
@@ -248,16 +325,19 @@ def _normal_plan(J):
     return _PLANS[key]
 
 
-def normal_operator(J, dtype=None, out=None):
+def normal_operator(J, dtype=None, out=None, sym=False):
     """N = J^T J in stencil form (the sum over equations of
     `mul_transpose_op`, util.py:577-583 / multigrid.py:1035-1071):
     N_o[m] = sum_{(a,b,sa)} J_a[m - sa] J_b[m - sa], out var = field of a,
     in var = field of b, shift = s_b - s_a. Accumulated in FP64.
     With `out` (a BlockStencil of the same structure) the result is written
-    into its coefficient buffer."""
+    into its coefficient buffer. sym=True returns the upper half only
+    (symmetric storage)."""
     if J.stride != 1:
         raise ValueError('normal_operator needs a stride-1 Jacobian')
-    keys, t_terms, t_start = _normal_plan(J)
+    if out is not None:
+        sym = out.sym
+    keys, t_terms, t_start = _normal_plan(J, sym)
     if out is not None:
         if out.entries != keys or out.nw != J.nw:
             raise ValueError('out has a different structure')
@@ -275,7 +355,7 @@ def normal_operator(J, dtype=None, out=None):
     if out is not None:
         return out
     return BlockStencil(coef, keys, J.nw, nvar_in=J.nvar_in,
-                        nvar_out=J.nvar_in)
+                        nvar_out=J.nvar_in, sym=sym)
 
 
 def restriction_weights(restriction, dim):
@@ -301,25 +381,32 @@ def restriction_weights(restriction, dim):
 
 
 def _galerkin_plan(A, sigma):
-    key = ('galerkin', tuple(A.entries), A.nw, sigma, str(A.device))
+    key = ('galerkin', tuple(A.entries), A.nw, sigma, A.sym, str(A.device))
     if key not in _PLANS:
         dim = A.ndim
         ofs = list(itertools.product([-1, 0, 1], repeat=dim))
         dcs = list(itertools.product([-2, -1, 0, 1, 2], repeat=dim))
+        # Logical entries (o, i, s) -> (stored array q, transposed flag).
         blocks = {}
         for q, (o, i, s) in enumerate(A.entries):
-            blocks.setdefault((o, i), []).append((q, s))
+            blocks.setdefault((o, i), []).append((q, s, 0))
+            if A.sym and not (o == i and all(x == 0 for x in s)):
+                mo, mi, ms = _mirror(o, i, s)
+                blocks.setdefault((mo, mi), []).append((q, ms, 1))
         keys, terms, tstart = [], [], [0]
         for (o, i) in sorted(blocks):
             for dc in dcs:
+                if A.sym and not is_upper(o, i, dc):
+                    continue
                 tl = []
-                for q, s in blocks[(o, i)]:
+                for q, s, tr in blocks[(o, i)]:
                     for r in ofs:
                         t = [r[d] + sigma * (s[d] - 2 * dc[d])
                              for d in range(dim)]
                         if all(abs(x) <= 1 for x in t):
                             tl.append((q, ) + _shift3(r) +
-                                      (sum(abs(x) for x in t), ))
+                                      (sum(abs(x) for x in t), ) +
+                                      _shift3(s) + (tr, ))
                 if tl:
                     keys.append((o, i, dc))
                     terms.extend(tl)
@@ -329,7 +416,7 @@ def _galerkin_plan(A, sigma):
         oent = t([[o, i] + list(_shift3(dc)) for o, i, dc in keys])
         wtab = torch.tensor([1., 0.5, 0.25, 0.125], dtype=torch.float64,
                             device=dev)
-        _PLANS[key] = (keys, oent, t(terms).reshape(-1, 5), t(tstart), wtab)
+        _PLANS[key] = (keys, oent, t(terms).reshape(-1, 9), t(tstart), wtab)
     return _PLANS[key]
 
 
@@ -347,11 +434,16 @@ def galerkin(A, restriction='full', rw_dev=None, mode='reference',
     coefficient c_s(a) with node a - s; it equals reflect(R reflect(A) T)
     with reflect: c_s(x) -> c_{-s}(x), and R A T when the stencil is
     reflection-symmetric.
-    With `out` the result is written into its coefficient buffer."""
+    With `out` the result is written into its coefficient buffer.
+    A symmetric (upper-half) A gives a symmetric coarse operator; this needs
+    mode 'exact' and restriction 'transpose' (R = T^T / 2^dim)."""
     if mode not in ('reference', 'exact'):
         raise ValueError('Unknown galerkin mode ' + str(mode))
     if A.stride != 1 or A.nvar_in != A.nvar_out:
         raise ValueError('galerkin needs a square stride-1 operator')
+    if A.sym and (mode != 'exact' or restriction != 'transpose'):
+        raise ValueError("symmetric Galerkin needs mode='exact' and "
+                         "restriction='transpose'")
     sigma = 1 if mode == 'exact' else -1
     keys, oent, terms, tstart, wtab = _galerkin_plan(A, sigma)
     nwh = tuple(n // 2 + 1 for n in A.nw)
@@ -367,14 +459,14 @@ def galerkin(A, restriction='full', rw_dev=None, mode='reference',
                            device=A.device)
     st = _lib.lib().odil_stencil_galerkin(
         c_int(_lib.dtype_tag(A.dtype)), c_int(A.ndim), _grid_arr(A.nw),
-        ptr(A.t_ent), ptr(oent), ptr(terms), ptr(tstart), c_int(len(keys)),
+        ptr(oent), ptr(terms), ptr(tstart), c_int(len(keys)),
         ptr(rw_dev), ptr(wtab), ptr(A.coef), ptr(coef), c_int(sigma),
         c_int(_rmode(restriction)), stream())
     check(st, 'stencil_galerkin')
     if out is not None:
         return out
     return BlockStencil(coef, keys, nwh, nvar_in=A.nvar_in,
-                        nvar_out=A.nvar_out)
+                        nvar_out=A.nvar_out, sym=A.sym)
 
 
 def restrict(fine, nw, rw_dev, coarse=None, restriction='full'):
@@ -460,13 +552,14 @@ class MultigridOp:
         stay fixed (a captured V-cycle graph stays valid)."""
         if A0.nw != self.nnw[0] or A0.nvar_in != self.nvar:
             raise ValueError('operator grid/fields do not match the levels')
-        same = (self.AA is not None and self.AA[0].entries == A0.entries)
+        same = (self.AA is not None and self.AA[0].entries == A0.entries
+                and self.AA[0].sym == A0.sym)
         if not same:
             AA = [A0.astype(self.dtype)]
             if AA[0] is A0:
                 AA[0] = BlockStencil(A0.coef.clone(), A0.entries, A0.nw,
                                      nvar_in=A0.nvar_in,
-                                     nvar_out=A0.nvar_out)
+                                     nvar_out=A0.nvar_out, sym=A0.sym)
             for level in range(1, self.nlevels):
                 AA.append(galerkin(AA[-1], self.restriction, self.rw,
                                    self.galerkin_mode))
@@ -573,9 +666,11 @@ class MultigridOp:
         if self._buf is None:
             b = []
             for n in self.nnw:
-                mk = lambda: [torch.zeros(n, dtype=self.dtype,
-                                          device=self.device)
-                              for _ in range(self.nvar)]
+                # One allocation per vector (fields x_k = x_0 + k N): the
+                # stencil kernels read such x through a single base pointer.
+                mk = lambda: list(torch.zeros((self.nvar, ) + tuple(n),
+                                              dtype=self.dtype,
+                                              device=self.device).unbind(0))
                 b.append({'u': mk(), 'v': mk(), 'f': mk(), 'r': mk()})
             self._buf = b
         return self._buf

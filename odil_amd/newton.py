@@ -147,6 +147,9 @@ class NewtonMultigrid:
         self.pre, self.post, self.ndirect = smooth_pre, smooth_post, ndirect
         self.matrix_free = matrix_free
         self.use_graph = use_graph
+        # J^T J and its Galerkin products R N T with R = T^T / 2^dim are
+        # symmetric: store upper halves.
+        self.sym = galerkin_mode == 'exact' and restriction == 'transpose'
         self.nvar = len(op.fieldnames)
         if coarse_reg == 'auto':
             # Tikhonov least squares on the coarsest level. The damping sits
@@ -315,12 +318,12 @@ class NewtonMultigrid:
         jac, _ = self.op.jacobian(fields)
         J = BlockStencil.from_jacobian(jac)
         if self.N is None:
-            self.N = normal_operator(J, self.wdtype)
+            self.N = normal_operator(J, self.wdtype, sym=self.sym)
         else:
             normal_operator(J, out=self.N)
         if not self.matrix_free:
             self._N64 = self.N if self.wdtype == torch.float64 else \
-                normal_operator(J, torch.float64)
+                normal_operator(J, torch.float64, sym=self.sym)
         self.mg.update_A(self.N)
         self.mg.prepare(self.ndirect)
         if self.matrix_free:
