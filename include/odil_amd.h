@@ -1,0 +1,330 @@
+/* C ABI of libodil_amd.so: MI355X (gfx950) kernels for the discrete-loss
+ * hot path of ODIL -- fused finite-difference residuals, loss adjoints,
+ * stencil Jacobians, matrix-free J^T J, and the geometric-multigrid
+ * normal-equation solver of the Newton step.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success or a status code below and never
+ *    throws; odil_last_error() returns the message of the last failure of
+ *    the calling process.
+ *  - All array arguments are device pointers owned by the caller, except
+ *    the small host arrays documented as such (grid sizes `nw`, `reach`,
+ *    `ent`/`shifts` outputs of *_jac_entries). No entry point allocates
+ *    device memory.
+ *  - Fields are C-order arrays of one grid (axis 0 slowest, e.g. t for the
+ *    space-time operators, x for the cavity); "fields" arguments are arrays
+ *    of per-field pointers (u, v, p for the cavity).
+ *  - dtype selects the arithmetic: ODIL_F32 or ODIL_F64. Sums of squares
+ *    and dot products are accumulated in FP64 in fixed order (deterministic).
+ *  - `stream` is a hipStream_t (NULL = default stream). Work is queued
+ *    asynchronously; reductions land in device memory.
+ *
+ * Reference paths are relative to the reference repository's code/
+ * directory.
+ */
+#ifndef ODIL_AMD_H
+#define ODIL_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ODIL_F32 = 0, ODIL_F64 = 1 };
+enum {
+  ODIL_OK = 0,
+  ODIL_ERR_ARG = 1,      /* invalid argument (Python: ValueError) */
+  ODIL_ERR_LAUNCH = 2,   /* HIP launch/runtime failure (RuntimeError) */
+  ODIL_ERR_CAPACITY = 3  /* size beyond a documented limit */
+};
+
+/* ---- runtime ----------------------------------------------------------- */
+
+/* Message of the last failed call (thread-unsafe static buffer). Replaces
+ * the Python exceptions the reference raises (util_op.py:539,
+ * linsolver.py:105, multigrid.py:255,521). */
+const char* odil_last_error(void);
+/* Version of this ABI (incremented on incompatible changes). */
+int odil_abi_version(void);
+/* out[r] = sum_b partial[r * nb + b] for r < nrows (fixed order, FP64):
+ * per-equation sums of squares from the per-block partials written by the
+ * residual / loss_grad / jtf kernels (the sum inside mean(f^2) of
+ * Problem._eval_loss, util_op.py:628-654). */
+int odil_reduce_partials(const double* partial, int nrows, int nb,
+                         double* out, void* stream);
+
+/* ---- operators ----------------------------------------------------------
+ * Per operator X in {cavity, wave, poisson, laplace, advection}:
+ *  odil_X_residual   f = operator(fields) for every equation, plus per-block
+ *                    partial sums of f^2 (`partial` [neq][nblocks(0)], may
+ *                    be NULL; `res` may be NULL for the sums only).
+ *                    Replaces Problem._eval_operator / eval_operator
+ *                    (util_op.py:657-674,968-974) of the operator callback.
+ *  odil_X_loss_grad  fused loss + adjoint: grad_k = sum_e (2/N) J_e,k^T f_e,
+ *                    i.e. tape.gradient of loss = sum_e mean(f_e^2) with
+ *                    frozen (stop-gradient) paths excluded, plus the
+ *                    per-block f^2 partials [neq][nblocks(1)]. Replaces
+ *                    Problem._eval_loss + eval_loss_grad (util_op.py:
+ *                    628-654,939-966). `nblocks` (host, may be NULL)
+ *                    receives the partial count.
+ *  odil_X_jac        stencil Jacobian: coefficient fields
+ *                    coef[q] = d f_e / d u_k(. + s) for the entries
+ *                    (e, k, s) listed by odil_X_jac_entries, frozen paths
+ *                    excluded, plus the residual (res may be NULL).
+ *                    Replaces Problem._eval_grad0 (util_op.py:585-617, the
+ *                    split_shifts tape Jacobian) feeding linearize
+ *                    (util_op.py:859-937) and the SparseOperator rows
+ *                    (util.py:544-566).
+ *  odil_X_jac_entries host call: writes the entry table ent[3 q] =
+ *                    (equation, field, shift index) and the shift table
+ *                    shifts[2 s] (2D); returns the number of entries.
+ *  odil_X_nblocks    host call: partial-sum count of kind 0 (residual) or
+ *                    1 (loss_grad / jtf) for the grid.
+ *  odil_X_normal_apply  matrix-free N = J^T J with J recomputed from
+ *                    `state`: mode 0 y = N x, 1 y = b - N x,
+ *                    2 y = x + omega (b - N x) / d (weighted Jacobi),
+ *                    3 y = diag(N). Replaces matr = sum_e mul_transpose_op
+ *                    (util.py:578-585, multigrid.py:1035-1071) applied by
+ *                    SparseOperator.mul_field / smoother_jacobi
+ *                    (multigrid.py:958-969,543-552) on the finest level.
+ *  odil_X_jtf        out = scale J^T f (scale = -1: the Newton right-hand
+ *                    side rhs = -J^T const, util.py:572-577), plus f^2
+ *                    partials.
+ * ------------------------------------------------------------------------ */
+
+/* Lid-driven cavity, steady incompressible Navier-Stokes on a collocated
+ * grid with Rhie-Chow continuity and deferred-correction upwinding
+ * (ns_steady.py:142-394): fields u, v, p on nx x ny cells, mu = 1/Re,
+ * bc 0/1/2 and rhiechow as the reference flags. */
+int odil_cavity_residual(int dtype, int nx, int ny, double dx, double dy,
+                         double mu, int bc, int rhiechow,
+                         const void* const* fields, void* const* res,
+                         double* partial, void* stream);
+int odil_cavity_jac(int dtype, int nx, int ny, double dx, double dy,
+                    double mu, int bc, int rhiechow, const void* const* fields,
+                    void* coef, void* const* res, void* stream);
+int odil_cavity_jac_entries(int* ent, int* shifts);
+int odil_cavity_loss_grad(int dtype, int nx, int ny, double dx, double dy,
+                          double mu, int bc, int rhiechow,
+                          const void* const* fields, void* const* grad,
+                          double* partial, int* nblocks, void* stream);
+int odil_cavity_nblocks(int kind, int dtype, int nx, int ny);
+int odil_cavity_normal_apply(int dtype, int nx, int ny, double dx, double dy,
+                             double mu, int bc, int rhiechow,
+                             const void* const* state, const void* const* x,
+                             const void* const* b, const void* const* d,
+                             void* const* y, double omega, int mode,
+                             void* stream);
+int odil_cavity_jtf(int dtype, int nx, int ny, double dx, double dy,
+                    double mu, int bc, int rhiechow, const void* const* fields,
+                    void* const* out, double scale, double* partial,
+                    int* nblocks, void* stream);
+
+/* 1D wave equation u_tt = u_xx on an nt x nx space-time grid
+ * (wave.py:67-122, operator_fd). init_u, init_ut [nx] and left_u, right_u
+ * [nt] are FP64 device arrays (initial value/velocity, boundary values). */
+int odil_wave_residual(int dtype, int nt, int nx, double dt, double dx,
+                       const double* init_u, const double* init_ut,
+                       const double* left_u, const double* right_u,
+                       const void* const* fields, void* const* res,
+                       double* partial, void* stream);
+int odil_wave_jac(int dtype, int nt, int nx, double dt, double dx,
+                  const double* init_u, const double* init_ut,
+                  const double* left_u, const double* right_u,
+                  const void* const* fields, void* coef, void* const* res,
+                  void* stream);
+int odil_wave_jac_entries(int* ent, int* shifts);
+int odil_wave_loss_grad(int dtype, int nt, int nx, double dt, double dx,
+                        const double* init_u, const double* init_ut,
+                        const double* left_u, const double* right_u,
+                        const void* const* fields, void* const* grad,
+                        double* partial, int* nblocks, void* stream);
+int odil_wave_nblocks(int kind, int dtype, int nt, int nx);
+int odil_wave_normal_apply(int dtype, int nt, int nx, double dt, double dx,
+                           const double* init_u, const double* init_ut,
+                           const double* left_u, const double* right_u,
+                           const void* const* state, const void* const* x,
+                           const void* const* b, const void* const* d,
+                           void* const* y, double omega, int mode,
+                           void* stream);
+int odil_wave_jtf(int dtype, int nt, int nx, double dt, double dx,
+                  const double* init_u, const double* init_ut,
+                  const double* left_u, const double* right_u,
+                  const void* const* fields, void* const* out, double scale,
+                  double* partial, int* nblocks, void* stream);
+
+/* Poisson residual f = lap u - src with Dirichlet zero through quadratic
+ * half-cell extrapolation (the Laplacian of util_op.py:1058-1063 with
+ * extrap_quadh walls, util_op.py:1017-1024; test_multigrid.py:275-296
+ * poses the same problem). src has the field dtype. */
+int odil_poisson_residual(int dtype, int n0, int n1, double dx, double dy,
+                          const void* src, const void* const* fields,
+                          void* const* res, double* partial, void* stream);
+int odil_poisson_jac(int dtype, int n0, int n1, double dx, double dy,
+                     const void* src, const void* const* fields, void* coef,
+                     void* const* res, void* stream);
+int odil_poisson_jac_entries(int* ent, int* shifts);
+int odil_poisson_loss_grad(int dtype, int n0, int n1, double dx, double dy,
+                           const void* src, const void* const* fields,
+                           void* const* grad, double* partial, int* nblocks,
+                           void* stream);
+int odil_poisson_nblocks(int kind, int dtype, int n0, int n1);
+int odil_poisson_normal_apply(int dtype, int n0, int n1, double dx,
+                              double dy, const void* src,
+                              const void* const* state, const void* const* x,
+                              const void* const* b, const void* const* d,
+                              void* const* y, double omega, int mode,
+                              void* stream);
+int odil_poisson_jtf(int dtype, int n0, int n1, double dx, double dy,
+                     const void* src, const void* const* fields,
+                     void* const* out, double scale, double* partial,
+                     int* nblocks, void* stream);
+
+/* operator_laplace (util_op.py:1038-1079): fields u, v; three equations
+ * (lap u, lap v, and an identically zero one) with linear-extrapolation
+ * walls. */
+int odil_laplace_residual(int dtype, int n0, int n1, double dx, double dy,
+                          const void* const* fields, void* const* res,
+                          double* partial, void* stream);
+int odil_laplace_jac(int dtype, int n0, int n1, double dx, double dy,
+                     const void* const* fields, void* coef, void* const* res,
+                     void* stream);
+int odil_laplace_jac_entries(int* ent, int* shifts);
+int odil_laplace_loss_grad(int dtype, int n0, int n1, double dx, double dy,
+                           const void* const* fields, void* const* grad,
+                           double* partial, int* nblocks, void* stream);
+int odil_laplace_nblocks(int kind, int dtype, int n0, int n1);
+int odil_laplace_normal_apply(int dtype, int n0, int n1, double dx,
+                              double dy, const void* const* state,
+                              const void* const* x, const void* const* b,
+                              const void* const* d, void* const* y,
+                              double omega, int mode, void* stream);
+int odil_laplace_jtf(int dtype, int n0, int n1, double dx, double dy,
+                     const void* const* fields, void* const* out,
+                     double scale, double* partial, int* nblocks,
+                     void* stream);
+
+/* 2D linear advection p_t + a . grad p = 0 on an nt x nx x ny space-time
+ * grid, periodic in x and y, first-order upwind on the sign of the constant
+ * velocity (ax, ay), p at t = 0 from p_init [nx][ny] (advection_2d.py:
+ * 103-223, problem 'direct'). One field, one equation. */
+int odil_advection_nblocks(int nt, int nx, int ny);
+int odil_advection_residual(int dtype, int nt, int nx, int ny, double dt,
+                            double dx, double dy, double ax, double ay,
+                            const void* p_init, const void* p, void* res,
+                            double* partial, void* stream);
+int odil_advection_loss_grad(int dtype, int nt, int nx, int ny, double dt,
+                             double dx, double dy, double ax, double ay,
+                             const void* p_init, const void* p, void* grad,
+                             double* partial, void* stream);
+
+/* ---- block stencil operators and multigrid ------------------------------
+ * A block stencil operator maps nvar_in fields on the node grid nw (2D or
+ * 3D, host int[3], unused axes 1) to nvar_out fields on
+ * nwh = (nw - 1) / stride + 1:
+ *   (A x)_o[I] = sum_{e: out_e = o} coef_e[I] x_{in_e}[stride I + s_e],
+ * zero outside the grid: SparseOperator (multigrid.py:877-1214) with
+ * non-circular shifts (apply_shift, multigrid.py:942-947). coef is
+ * [nent][nwh]; `ent` int32 [nent][5] = (out, in, s0, s1, s2) sorted by out,
+ * `ostart` [nvar_out + 1] delimits each output field, `dent` [nvar_out] the
+ * diagonal entry of each field (Jacobi). Index tables are device arrays.
+ * ------------------------------------------------------------------------ */
+
+/* mode 0 y = A x (SparseOperator.mul_field, multigrid.py:958-969);
+ * 1 y = f - A x (MultigridOp.residual, multigrid.py:536-541);
+ * 2 y = x + omega (f - A x) / diag (smoother_jacobi, multigrid.py:543-552).
+ * max_entries = largest per-field entry count (<= 512). */
+int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
+                       int nvar_in, int nvar_out, const int* ent,
+                       const int* ostart, const int* dent, int max_entries,
+                       const void* coef, const void* const* x, void* const* y,
+                       const void* const* f, int mode, double omega,
+                       void* stream);
+/* Same modes for a symmetric square stride-1 operator stored as its upper
+ * half (entries o < i, or o == i and s >= 0). `items` int32 [nitem][10]
+ * lists per output field the direct and mirrored products
+ * (q, var, cs_lin, xs_lin, cs0..2, xs0..2), `ostart` [nvar + 1], `reach`
+ * host int[3] bounds the offsets per axis, max_items <= 512, nent the
+ * coefficient array count. No reference counterpart for the storage: it
+ * applies the same matrices as the reference's full stencils
+ * (mul_transpose_op products and Galerkin levels are symmetric). */
+int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
+                           const int* items, const int* ostart,
+                           const int* dent, const int* reach, int max_items,
+                           int nent, const void* coef, const void* const* x,
+                           void* const* y, const void* const* f, int mode,
+                           double omega, void* stream);
+/* y = A^T x or y += A^T x (mul_transpose_field, multigrid.py:1006-1033).
+ * `tent` lists entry indices grouped by input field, `istart` delimits. */
+int odil_stencil_apply_t(int dtype, int ndim, const int* nw, int stride,
+                         int nvar_in, int nvar_out, const int* ent,
+                         const int* tent, const int* istart, const void* coef,
+                         const void* const* x, void* const* y, int accumulate,
+                         void* stream);
+/* Normal operator N = sum_e J_e^T J_e in stencil form from Jacobian
+ * coefficients jc (mul_transpose_op + add_elementwise, multigrid.py:
+ * 1035-1071,1138-1184, as assembled in util.py:578-585). `terms` int32
+ * [nterm][5] = (a, b, sa0, sa1, sa2), grouped per output entry by
+ * `tstart` [nout + 1]: N_o[m] = sum J_a[m - sa] J_b[m - sa]. FP64
+ * accumulation; dtype_in/dtype_out in {F64->F64, F64->F32, F32->F32}. */
+int odil_stencil_normal(int dtype_in, int dtype_out, int ndim, const int* nw,
+                        const int* terms, const int* tstart, int nout,
+                        const void* jc, void* out, void* stream);
+/* Galerkin coarse operator on node grids (odd sizes):
+ * sigma = +1 exact R A T (Multigrid.update_A, multigrid.py:116-157);
+ * sigma = -1 the arithmetic of MultigridOp.coarsen_sparse_operator
+ * (multigrid.py:391-427). `oent` int32 [nout][5] output entries, `terms`
+ * int32 [nterm][9] = (q, r0, r1, r2, wt_id, s0, s1, s2, tr) per output
+ * entry (tstart), rw [27] restriction weights, wtab interpolation weights;
+ * rmode 0 injects at coarse boundary nodes (get_R, multigrid.py:497-534),
+ * 1 keeps the weights (R = T^T / 2^dim). */
+int odil_stencil_galerkin(int dtype, int ndim, const int* nw, const int* oent,
+                          const int* terms, const int* tstart, int nout,
+                          const double* rw, const double* wtab, const void* fc,
+                          void* cc, int sigma, int rmode, void* stream);
+/* coarse = R fine (multigrid.py:497-534 weights, rmode as above). */
+int odil_mg_restrict(int dtype, int ndim, const int* nw, int nvar,
+                     const double* rw, int rmode, const void* const* fine,
+                     void* const* coarse, void* stream);
+/* fine += T coarse, multilinear interpolation (get_TT, multigrid.py:
+ * 472-494, applied as in MultigridOp.step, multigrid.py:594-596). */
+int odil_mg_prolong_add(int dtype, int ndim, const int* nw, int nvar,
+                        const void* const* coarse, void* const* fine,
+                        void* stream);
+/* Dense FP64 row-major matrix (zeroed by the caller) of a square stride-1
+ * operator (sym != 0: upper-half storage) for the coarsest-level direct
+ * solve (smoother_direct, multigrid.py:554-565; SparseOperator.tocsr,
+ * multigrid.py:906-920). */
+int odil_stencil_to_dense(int dtype, int ndim, const int* nw, int nvar,
+                          const int* ent, int nent, const void* coef, int sym,
+                          double* M, void* stream);
+/* y = M x with M dense FP64 [nvar N][nvar N] (the coarsest-level solve as
+ * a precomputed solution operator). */
+int odil_dense_gemv(int dtype_x, int dtype_y, int nvar, int N,
+                    const double* M, const void* const* x, void* const* y,
+                    void* stream);
+
+/* ---- vector kernels of the Krylov / Newton drivers ----------------------
+ * Scalars live in device memory so a whole solver iteration can be queued
+ * (and captured in a HIP graph) without host round trips. */
+
+/* Number of FP64 partials odil_dot needs in `partial`. */
+int odil_dot_partials(void);
+/* out[0] = sum_k x_k . y_k (FP64, fixed order); keep_prev != 0 copies the
+ * old out[0] to out[1] first. */
+int odil_dot(int dtype, int nvar, long long N, const void* const* x,
+             const void* const* y, double* partial, double* out,
+             int keep_prev, void* stream);
+/* y_k += sign * (num / den) x_k (den == 0: no update). */
+int odil_axpy_ratio(int dtype, int nvar, long long N, void* const* y,
+                    const void* const* x, const double* num,
+                    const double* den, double sign, void* stream);
+/* p_k = z_k + ((num - sub) / den) p_k (sub may be NULL; den == 0: p = z). */
+int odil_xpay_ratio(int dtype, int nvar, long long N, void* const* p,
+                    const void* const* z, const double* num,
+                    const double* sub, const double* den, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ODIL_AMD_H */

@@ -1,0 +1,103 @@
+"""The C-ABI library loads without a GPU, exports exactly the entry points
+declared in include/odil_amd.h, and its host-only calls and argument checks
+behave as documented (no kernel is launched here)."""
+
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, 'include', 'odil_amd.h')
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', ' ', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(odil_\w+)\s*\(', text)))
+
+
+@pytest.fixture(scope='module')
+def lib():
+    from odil_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip('library not built (python -m odil_amd.build)')
+    return _lib.lib()
+
+
+def test_header_compiles_as_c():
+    r = subprocess.run(['gcc', '-fsyntax-only', '-std=c99', '-Wall',
+                        '-Werror', '-x', 'c', HEADER], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_every_declared_symbol_is_exported(lib):
+    names = _declared()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_no_undeclared_exports(lib):
+    from odil_amd import _lib
+    r = subprocess.run(['nm', '-D', '--defined-only', _lib.LIB_PATH],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip('nm unavailable')
+    exported = sorted(set(re.findall(r'\bT (odil_\w+)', r.stdout)))
+    assert exported == _declared()
+
+
+def test_host_calls(lib):
+    assert lib.odil_abi_version() >= 1
+    assert lib.odil_dot_partials() > 0
+    ent = (ctypes.c_int * (3 * 64))()
+    shifts = (ctypes.c_int * (2 * 16))()
+    # cavity: 27 coefficient fields, 5 shifts (centre and the 4 neighbours)
+    assert lib.odil_cavity_jac_entries(ent, shifts) == 27
+    got = {(shifts[2 * s], shifts[2 * s + 1]) for s in range(5)}
+    assert got == {(0, 0), (-1, 0), (1, 0), (0, -1), (0, 1)}
+    # (equation, field) blocks: the momentum equations freeze the advecting
+    # velocity, so fu has no v block and fv no u block.
+    blocks = {}
+    for q in range(27):
+        key = (ent[3 * q], ent[3 * q + 1])
+        blocks[key] = blocks.get(key, 0) + 1
+    assert blocks == {(0, 0): 5, (0, 2): 3, (1, 1): 5, (1, 2): 3,
+                      (2, 0): 3, (2, 1): 3, (2, 2): 5}
+    assert lib.odil_wave_jac_entries(ent, shifts) == 5
+    assert lib.odil_poisson_jac_entries(ent, shifts) == 5
+    assert lib.odil_laplace_jac_entries(ent, shifts) == 10
+    assert lib.odil_cavity_nblocks(0, 1, 1025, 1025) > 0
+    assert lib.odil_advection_nblocks(8, 16, 16) > 0
+
+
+def test_argument_errors_do_not_launch(lib):
+    """Invalid arguments return status 1 with a message, before any HIP
+    call (so this runs on a CPU-only host)."""
+    from odil_amd import _lib
+    lib.odil_last_error.restype = ctypes.c_char_p
+    nw = (ctypes.c_int * 3)(9, 9, 1)
+    st = lib.odil_stencil_apply(ctypes.c_int(1), ctypes.c_int(4), nw,
+                                ctypes.c_int(1), ctypes.c_int(1),
+                                ctypes.c_int(1), None, None, None,
+                                ctypes.c_int(1), None, None, None, None,
+                                ctypes.c_int(0), ctypes.c_double(0.), None)
+    assert st == 1
+    assert b'ndim' in lib.odil_last_error()
+    st = lib.odil_stencil_apply(ctypes.c_int(1), ctypes.c_int(2), nw,
+                                ctypes.c_int(1), ctypes.c_int(1),
+                                ctypes.c_int(1), None, None, None,
+                                ctypes.c_int(100000), None, None, None, None,
+                                ctypes.c_int(0), ctypes.c_double(0.), None)
+    assert st == 3  # capacity
+    st = lib.odil_dot(ctypes.c_int(1), ctypes.c_int(9), ctypes.c_longlong(4),
+                      None, None, None, None, ctypes.c_int(0), None)
+    assert st == 1
+    with pytest.raises(ValueError):
+        _lib.check(1, 'probe')
+    with pytest.raises(_lib.OdilError):
+        _lib.check(3, 'probe')
