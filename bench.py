@@ -44,6 +44,10 @@ def parse():
     p.add_argument('--no_tol_run', action='store_true')
     p.add_argument('--no_cpu', action='store_true')
     p.add_argument('--cpu_n', type=int, default=129)
+    p.add_argument('--copy_probe', action='store_true',
+                   help='first run a plain streaming kernel over a known '
+                   'size (torch.neg on 256 MiB of FP32) whose PMC '
+                   'FETCH_SIZE / WRITE_SIZE calibrate the traffic counters')
     p.add_argument('--no_graph', action='store_true',
                    help='launch the linear solve eagerly instead of from a '
                    'captured HIP graph (kernel tracers see every launch)')
@@ -76,7 +80,88 @@ def steps_per_second(args, precision):
         nm.step(fields)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    return args.steps / (t1 - t0), (t1 - t0) / args.steps
+    return args.steps / (t1 - t0), (t1 - t0) / args.steps, nm
+
+
+def _event_time(fn, reps, use_graph):
+    """Average milliseconds of `fn` over `reps` launches, timed with HIP
+    events on torch's current stream (the stream the kernels run on). With
+    use_graph the launches are captured once and replayed, so host launch
+    gaps do not enter."""
+    import torch
+    for _ in range(3):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    if use_graph:
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            for _ in range(reps):
+                fn()
+        graph.replay()
+        torch.cuda.synchronize()
+        s = torch.cuda.current_stream()
+        e0.record(s)
+        graph.replay()
+        e1.record(s)
+    else:
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def dominant_kernel(args, nm):
+    """The kernel with the largest share of the mixed Newton step: the
+    weighted-Jacobi sweep of the stored (upper-half symmetric) Galerkin
+    operator on multigrid level 1 (513 x 513 nodes, FP32), run on the
+    solver's own operator and level buffers. Algorithmic bytes per launch:
+    every stored coefficient array once, x, f read and y written
+    (nent + 3 * 3) * N1 * 4."""
+    A = nm.mg.AA[1]
+    buf = nm.mg._buffers()[1]
+    n1 = 1
+    for n in A.nw:
+        n1 *= n
+    B = A.coef.element_size()
+    ms = _event_time(lambda: A.jacobi(buf['u'], buf['f'], buf['v'],
+                                      args.omega),
+                     args.kernel_reps, not args.no_graph)
+    nbytes = (len(A.entries) + 3 * A.nvar_out) * n1 * B
+    return {'ms': ms, 'GBps': nbytes / (ms * 1e-3) / 1e9, 'bytes': nbytes,
+            'nent': len(A.entries), 'grid': list(A.nw)}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles',
+                            'traffic_apply_sym_level1_cavity1025.txt')
+
+
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed PMC
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of known
+    size), or None when that summary is absent."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith('hbm_bytes_per_launch:'):
+            return float(line.split(':', 1)[1])
+    return None
+
+
+def copy_probe():
+    """Plain streaming kernel of known size for counter calibration:
+    y = -x over 2^26 FP32 values (256 MiB read, 256 MiB written), 5 times."""
+    import torch
+    x = torch.ones(1 << 26, dtype=torch.float32, device='cuda')
+    y = torch.empty_like(x)
+    for _ in range(5):
+        torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    del x, y
 
 
 def time_to_tol(args, precision):
@@ -94,8 +179,8 @@ def time_to_tol(args, precision):
 
 def kernel_bandwidth(args):
     """Average duration of the cavity residual and fused loss-adjoint kernels
-    at n x n. `kernel_reps` launches are captured in a HIP graph and timed
-    with events on the capture stream, so host launch overhead does not enter.
+    at n x n. `kernel_reps` launches are captured in a HIP graph (eager
+    with --no_graph) and timed with events on the launch stream.
     Algorithmic bytes: residual reads u, v, p and writes fu, fv, fp (6 N B);
     the adjoint reads u, v, p and writes three gradients (6 N B)."""
     import torch
@@ -123,23 +208,7 @@ def kernel_bandwidth(args):
                     op.launch_residual(fields, res, partial)
                 else:
                     op.launch_loss_grad(fields, grads, partial)
-            for _ in range(3):
-                call()
-            torch.cuda.synchronize()
-            graph = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(graph):
-                for _ in range(args.kernel_reps):
-                    call()
-            graph.replay()
-            torch.cuda.synchronize()
-            s = torch.cuda.current_stream()
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record(s)
-            graph.replay()
-            e1.record(s)
-            torch.cuda.synchronize()
-            ms = e0.elapsed_time(e1) / args.kernel_reps
+            ms = _event_time(call, args.kernel_reps, not args.no_graph)
             nbytes = 6 * N * B
             out['{}_{}'.format(kind, name)] = {
                 'ms': ms, 'GBps': nbytes / (ms * 1e-3) / 1e9,
@@ -194,8 +263,12 @@ def main():
     if not torch.cuda.is_available():
         print('no GPU visible', file=sys.stderr)
         sys.exit(1)
-    sps_mixed, ms_mixed = steps_per_second(args, 'mixed')
-    sps_64, ms_64 = steps_per_second(args, 'fp64')
+    if args.copy_probe:
+        copy_probe()
+    sps_mixed, ms_mixed, nm_mixed = steps_per_second(args, 'mixed')
+    dom = dominant_kernel(args, nm_mixed)
+    del nm_mixed
+    sps_64, ms_64, _ = steps_per_second(args, 'fp64')
     tol = {}
     if not args.no_tol_run:
         tol['mixed'], f_mixed = time_to_tol(args, 'mixed')
@@ -208,7 +281,7 @@ def main():
         d.append(torch.max(torch.abs(pm - p6)).item())
         tol['mixed_vs_fp64_maxabs'] = {'u': d[0], 'v': d[1], 'p': d[2]}
     kb = kernel_bandwidth(args)
-    dom = kb['adjoint_fp32']
+    kb['mg_level1_jacobi_fp32'] = dom
     line = {
         'metric': 'Newton steps/s + residual/adjoint GB/s at 1024^2, '
                   'mixed vs FP64, same tol',
@@ -238,10 +311,13 @@ def main():
         'time_to_tol': tol,
         'kernels': kb,
         'roofline': {
-            'bound': 'hbm', 'kernel': 'k_loss_grad<float, CavityOp> '
-                                      '(fused residual + adjoint, fp32)',
+            'bound': 'hbm',
+            'kernel': 'k_apply_sym<float> (level-1 Galerkin Jacobi sweep, '
+                      '{}x{}, {} stored coefficient fields, FP32)'.format(
+                          dom['grid'][0], dom['grid'][1], dom['nent']),
             'achieved': dom['GBps'], 'peak': HBM_PEAK_GBPS, 'unit': 'GB/s',
-            'frac': dom['GBps'] / HBM_PEAK_GBPS, 'traffic': None},
+            'frac': dom['GBps'] / HBM_PEAK_GBPS,
+            'traffic': measured_traffic()},
     }
     if 'mixed' in tol and 'fp64' in tol:
         line['time_to_tol_speedup'] = (tol['fp64']['seconds'] /

@@ -51,7 +51,8 @@ __global__ void __launch_bounds__(256)
                    AdvParams P) {
   __shared__ double red[4];
   const size_t N = (size_t)P.nt * P.nx * P.ny;
-  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const size_t c = (size_t)b * 256 + threadIdx.x;
   double s = 0;
   if (c < N) {
     const int iy = (int)(c % P.ny);
@@ -64,7 +65,7 @@ __global__ void __launch_bounds__(256)
   }
   if (partial) {
     s = block_sum(s, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (threadIdx.x == 0) partial[b] = s;
   }
 }
 
@@ -79,7 +80,8 @@ __global__ void __launch_bounds__(256)
                     AdvParams P, T scale) {
   __shared__ double red[4];
   const size_t N = (size_t)P.nt * P.nx * P.ny;
-  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const size_t c = (size_t)b * 256 + threadIdx.x;
   double s = 0;
   if (c < N) {
     const int iy = (int)(c % P.ny);
@@ -102,7 +104,7 @@ __global__ void __launch_bounds__(256)
   }
   if (partial) {
     s = block_sum(s, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    if (threadIdx.x == 0) partial[b] = s;
   }
 }
 

@@ -28,6 +28,25 @@ void set_error(const char* fmt, ...);
 int check_launch(const char* what);
 
 // Wave-level sum over 64 lanes (CDNA wavefront) using DPP-capable shuffles.
+// Workgroups are dealt to the 8 XCDs of the MI355X round-robin in dispatch
+// order, and each XCD has its own L2. xcd_block maps the dispatch index L of
+// a grid of T workgroups to a logical index such that XCD k serves one
+// contiguous range of logical blocks: neighbouring tiles (whose halos
+// overlap) then share an L2 instead of fetching the halo again from HBM.
+constexpr unsigned kNumXcd = 8;
+__device__ inline unsigned xcd_block(unsigned L, unsigned T) {
+  const unsigned xcd = L % kNumXcd, idx = L / kNumXcd;
+  const unsigned q = T / kNumXcd, r = T % kNumXcd;
+  return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+// Logical (bx, by) of a 2D grid under xcd_block, bx fastest.
+__device__ inline void xcd_block2(unsigned& bx, unsigned& by) {
+  const unsigned g = xcd_block(blockIdx.x + blockIdx.y * gridDim.x,
+                               gridDim.x * gridDim.y);
+  bx = g % gridDim.x;
+  by = g / gridDim.x;
+}
+
 template <class T>
 __device__ inline T wave_sum(T v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -71,14 +71,16 @@ __global__ void __launch_bounds__(256)
                double* __restrict__ partial, int n0, int n1,
                typename Op::Params P) {
   __shared__ double red[4];
-  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+  unsigned bx, by;
+  xcd_block2(bx, by);
+  const int j = bx * 64 + (threadIdx.x & 63);
+  const int i = by * 4 + (threadIdx.x >> 6);
   double ssq[Op::NE];
   for (int e = 0; e < Op::NE; ++e) ssq[e] = 0;
   if (i < n0 && j < n1) {
     T f[Op::NE];
-    if (interior_box(blockIdx.y * 4, blockIdx.y * 4 + 4, blockIdx.x * 64,
-                     blockIdx.x * 64 + 64, n0, n1, Op::R)) {
+    if (interior_box(by * 4, by * 4 + 4, bx * 64, bx * 64 + 64, n0, n1,
+                     Op::R)) {
       GlobalAcc<T, Op::NF, false> acc{in.f, n0, n1, i, j};
       Op::template residual<true>(acc, i, j, n0, n1, P, f);
     } else {
@@ -92,7 +94,7 @@ __global__ void __launch_bounds__(256)
     }
   }
   if (partial) {
-    const int b = blockIdx.y * gridDim.x + blockIdx.x;
+    const int b = by * gridDim.x + bx;
     const int nb = gridDim.x * gridDim.y;
     for (int e = 0; e < Op::NE; ++e) {
       double s = block_sum(ssq[e], red);
@@ -107,13 +109,15 @@ template <class T, class Op>
 __global__ void __launch_bounds__(256)
     k_jac(FieldPtrs<T, Op::NF> in, T* __restrict__ coef, OutPtrs<T, Op::NE> res,
           int n0, int n1, typename Op::Params P) {
-  const int j = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+  unsigned bx, by;
+  xcd_block2(bx, by);
+  const int j = bx * 64 + (threadIdx.x & 63);
+  const int i = by * 4 + (threadIdx.x >> 6);
   if (i >= n0 || j >= n1) return;
   T f[Op::NE];
   T J[Op::NE][Op::NF][Op::NS];
-  if (interior_box(blockIdx.y * 4, blockIdx.y * 4 + 4, blockIdx.x * 64,
-                   blockIdx.x * 64 + 64, n0, n1, Op::R)) {
+  if (interior_box(by * 4, by * 4 + 4, bx * 64, bx * 64 + 64, n0, n1,
+                   Op::R)) {
     GlobalAcc<T, Op::NF, false> acc{in.f, n0, n1, i, j};
     Op::template eval<true>(acc, i, j, n0, n1, P, f, J);
   } else {
@@ -187,8 +191,10 @@ __global__ void __launch_bounds__(256)
   T* ot = reinterpret_cast<T*>(smem_raw);  // [NF][NS-1][E0][E1]
   __shared__ double red[4];
 
-  const int x0 = blockIdx.y * TX;
-  const int y0 = blockIdx.x * TY;
+  unsigned bx, by;
+  xcd_block2(bx, by);
+  const int x0 = by * TX;
+  const int y0 = bx * TY;
   const int tid = threadIdx.x;
   const bool inner = interior_box(x0 - RJ, x0 + TX + RJ, y0 - RJ,
                                   y0 + TY + RJ, n0, n1, Op::R);
@@ -268,7 +274,7 @@ __global__ void __launch_bounds__(256)
     }
   }
   if (partial) {
-    const int bidx = blockIdx.y * gridDim.x + blockIdx.x;
+    const int bidx = by * gridDim.x + bx;
     const int nb = gridDim.x * gridDim.y;
     for (int e = 0; e < NE; ++e) {
       double s = block_sum(ssq[e], red);
@@ -304,8 +310,10 @@ __global__ void __launch_bounds__(256)
   extern __shared__ unsigned char smem_raw[];
   T* ot = reinterpret_cast<T*>(smem_raw);  // [NF][NS-1][E0][E1]
 
-  const int x0 = blockIdx.y * TX;
-  const int y0 = blockIdx.x * TY;
+  unsigned bx, by;
+  xcd_block2(bx, by);
+  const int x0 = by * TX;
+  const int y0 = bx * TY;
   const int tid = threadIdx.x;
   const bool diag = (mode == 3);
   const bool inner = interior_box(x0 - RJ, x0 + TX + RJ, y0 - RJ,

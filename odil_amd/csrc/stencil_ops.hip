@@ -30,12 +30,15 @@ struct Grid3 {
   __host__ __device__ size_t size() const { return (size_t)n0 * n1 * n2; }
 };
 
+// Grids hold fewer than 2^31 points (checked on the host), so index
+// arithmetic is 32-bit (64-bit division is a long instruction sequence).
 __device__ inline void unflat(size_t c, const Grid3& g, int& i0, int& i1,
                               int& i2) {
-  i2 = (int)(c % g.n2);
-  size_t r = c / g.n2;
-  i1 = (int)(r % g.n1);
-  i0 = (int)(r / g.n1);
+  const unsigned cc = (unsigned)c;
+  i2 = (int)(cc % (unsigned)g.n2);
+  const unsigned r = cc / (unsigned)g.n2;
+  i1 = (int)(r % (unsigned)g.n1);
+  i0 = (int)(r / (unsigned)g.n1);
 }
 
 __device__ inline bool inside(int a, int b, int c, const Grid3& g) {
@@ -60,7 +63,12 @@ __global__ void __launch_bounds__(256)
             CVecPtrs<T> x, VecPtrs<T> y, CVecPtrs<T> f, Grid3 nw, Grid3 nwh,
             int stride, int mode, T omega) {
   __shared__ int4 tab[kMaxEntries];  // (in, s0, s1, s2)
-  const int o = blockIdx.y;
+  // Logical block: tile bx, output field o fastest, so every field of one
+  // tile runs on the same XCD (shared x reads in its L2).
+  const unsigned g = xcd_block(blockIdx.x + blockIdx.y * gridDim.x,
+                               gridDim.x * gridDim.y);
+  const int o = g % gridDim.y;
+  const unsigned bx = g / gridDim.y;
   const int e0 = ostart[o];
   const int ne = ostart[o + 1] - e0;
   for (int t = threadIdx.x; t < ne; t += blockDim.x) {
@@ -69,7 +77,7 @@ __global__ void __launch_bounds__(256)
   }
   __syncthreads();
   const size_t Nh = nwh.size();
-  const size_t c = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / LANES;
+  const size_t c = ((size_t)bx * blockDim.x + threadIdx.x) / LANES;
   const int lane = threadIdx.x % LANES;
   if (c >= Nh) return;  // whole groups exit together
   int i0, i1, i2;
@@ -136,20 +144,20 @@ __global__ void __launch_bounds__(256)
                 CVecPtrs<T> x, VecPtrs<T> y, CVecPtrs<T> f, Grid3 nw,
                 Grid3 reach, int mode, T omega) {
   __shared__ int2 fast[kMaxItems];
-  __shared__ int full[kMaxItems * 8];  // (q, var, cs0..2, xs0..2)
-  const int o = blockIdx.y;
+  // Logical block: tile bx, output field o fastest (one XCD per tile).
+  const unsigned g = xcd_block(blockIdx.x + blockIdx.y * gridDim.x,
+                               gridDim.x * gridDim.y);
+  const int o = g % gridDim.y;
+  const unsigned bx = g / gridDim.y;
   const int i0e = ostart[o];
   const int ne = ostart[o + 1] - i0e;
   const int N = (int)nw.size();
   for (int t = threadIdx.x; t < ne; t += blockDim.x) {
     const int* it = items + 10 * (i0e + t);
     fast[t] = make_int2(it[0] * N + it[2], (CONTIG ? it[1] * N : 0) + it[3]);
-    full[8 * t] = it[0];
-    full[8 * t + 1] = it[1];
-    for (int k = 0; k < 6; ++k) full[8 * t + 2 + k] = it[4 + k];
   }
   __syncthreads();
-  const int c = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) / LANES);
+  const int c = (int)(((size_t)bx * blockDim.x + threadIdx.x) / LANES);
   const int lane = threadIdx.x % LANES;
   if (c >= N) return;  // whole groups exit together
   int i0, i1, i2;
@@ -161,7 +169,7 @@ __global__ void __launch_bounds__(256)
   if (inner) {
     if (CONTIG) {
       const T* __restrict__ xb = x.p[0];
-#pragma unroll 8
+#pragma unroll 16
       for (int e = lane; e < ne; e += LANES) {
         const int2 it = fast[e];
         acc += coef[(unsigned)(c + it.x)] * xb[(unsigned)(c + it.y)];
@@ -171,14 +179,15 @@ __global__ void __launch_bounds__(256)
       for (int e = lane; e < ne; e += LANES) {
         const int2 it = fast[e];
         acc += coef[(unsigned)(c + it.x)] *
-               pick(x, full[8 * e + 1])[(unsigned)(c + it.y)];
+               pick(x, items[10 * (i0e + e) + 1])[(unsigned)(c + it.y)];
       }
     }
   } else {
+    // Cells near a face (few): full shifts from the global table.
     for (int e = lane; e < ne; e += LANES) {
-      const int* fl = full + 8 * e;
-      const int a0 = i0 + fl[2], a1 = i1 + fl[3], a2 = i2 + fl[4];
-      const int b0 = i0 + fl[5], b1 = i1 + fl[6], b2 = i2 + fl[7];
+      const int* fl = items + 10 * (i0e + e);
+      const int a0 = i0 + fl[4], a1 = i1 + fl[5], a2 = i2 + fl[6];
+      const int b0 = i0 + fl[7], b1 = i1 + fl[8], b2 = i2 + fl[9];
       if (!inside(a0, a1, a2, nw) || !inside(b0, b1, b2, nw)) continue;
       acc += coef[(size_t)fl[0] * N + ((size_t)a0 * nw.n1 + a1) * nw.n2 + a2] *
              pick(x, fl[1])[((size_t)b0 * nw.n1 + b1) * nw.n2 + b2];
@@ -248,9 +257,13 @@ __global__ void __launch_bounds__(256)
              Grid3 g) {
   // Workgroup (bx, o): output entry o (uniform, so the term table is read
   // with scalar loads), cells m along x.
+  // Logical block: cell tile, output entry fastest, so all entries of one
+  // tile are formed on the same XCD from the same Jacobian lines in its L2.
+  const unsigned lb = xcd_block(blockIdx.x + blockIdx.y * gridDim.x,
+                                gridDim.x * gridDim.y);
+  const int o = lb % gridDim.y;
   const size_t N = g.size();
-  const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int o = blockIdx.y;
+  const size_t m = (size_t)(lb / gridDim.y) * blockDim.x + threadIdx.x;
   if (m >= N) return;
   int m0, m1, m2;
   unflat(m, g, m0, m1, m2);
@@ -289,9 +302,12 @@ __global__ void __launch_bounds__(256)
                Grid3 nwh, int dim, int sigma, int rmode) {
   // Workgroup (bx, o): output entry o (uniform, so the term table is read
   // with scalar loads), coarse nodes I along x.
+  // Logical block: coarse tile, output entry fastest (one XCD per tile).
+  const unsigned lb = xcd_block(blockIdx.x + blockIdx.y * gridDim.x,
+                                gridDim.x * gridDim.y);
+  const int o = lb % gridDim.y;
   const size_t Nc = nwh.size();
-  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int o = blockIdx.y;
+  const size_t I = (size_t)(lb / gridDim.y) * blockDim.x + threadIdx.x;
   if (I >= Nc) return;
   int I0, I1, I2;
   unflat(I, nwh, I0, I1, I2);
@@ -332,7 +348,8 @@ __global__ void __launch_bounds__(256)
     k_restrict(CVecPtrs<T> fine, VecPtrs<T> coarse, int nvar,
                const double* __restrict__ rw, Grid3 nw, Grid3 nwh, int dim,
                int rmode) {
-  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t I =
+      (size_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (I >= nwh.size()) return;
   int I0, I1, I2;
   unflat(I, nwh, I0, I1, I2);
@@ -368,7 +385,8 @@ template <class T>
 __global__ void __launch_bounds__(256)
     k_prolong_add(CVecPtrs<T> coarse, VecPtrs<T> fine, int nvar, Grid3 nw,
                   Grid3 nwh) {
-  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t p =
+      (size_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= nw.size()) return;
   int p0, p1, p2;
   unflat(p, nw, p0, p1, p2);
@@ -479,9 +497,21 @@ bool bad_var(int n, const char* what) {
   return false;
 }
 
-bool bad_dim(int ndim, const char* what) {
+bool bad_dim(int ndim, const int* nw, const char* what) {
   if (ndim != 2 && ndim != 3) {
     set_error("%s: ndim must be 2 or 3, got %d", what, ndim);
+    return true;
+  }
+  double cells = 1;
+  for (int d = 0; d < ndim; ++d) {
+    if (nw[d] < 1) {
+      set_error("%s: grid size %d along axis %d", what, nw[d], d);
+      return true;
+    }
+    cells *= nw[d];
+  }
+  if (cells >= 2147483648.0) {
+    set_error("%s: grids need fewer than 2^31 points", what);
     return true;
   }
   return false;
@@ -503,7 +533,7 @@ int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
                        const void* coef, const void* const* x, void* const* y,
                        const void* const* f, int mode, double omega,
                        void* stream) {
-  if (bad_dim(ndim, "stencil_apply") || bad_var(nvar_in, "stencil_apply") ||
+  if (bad_dim(ndim, nw, "stencil_apply") || bad_var(nvar_in, "stencil_apply") ||
       bad_var(nvar_out, "stencil_apply"))
     return kErrArg;
   if (stride != 1 && stride != 2) {
@@ -565,15 +595,17 @@ int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
 // [nvar + 1] list the products reaching each output field (see
 // k_apply_sym); `reach` (host, 3 ints) bounds |cs|, |xs| per axis;
 // `max_items` is the longest per-field list (at most 512); `nent` the number
-// of coefficient arrays. x fields laid out as one allocation (x_k = x_0 +
+// of coefficient arrays; `lanes` threads per cell (1, 2, 4, 8, 16, or 0 for
+// a size-based default). x fields laid out as one allocation (x_k = x_0 +
 // k N) take the fast path.
 int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
                            const int* items, const int* ostart,
                            const int* dent, const int* reach, int max_items,
-                           int nent, const void* coef, const void* const* x,
-                           void* const* y, const void* const* f, int mode,
-                           double omega, void* stream) {
-  if (bad_dim(ndim, "stencil_apply_sym") ||
+                           int nent, int lanes, const void* coef,
+                           const void* const* x, void* const* y,
+                           const void* const* f, int mode, double omega,
+                           void* stream) {
+  if (bad_dim(ndim, nw, "stencil_apply_sym") ||
       bad_var(nvar, "stencil_apply_sym"))
     return kErrArg;
   if (mode == 2 && !dent) {
@@ -607,9 +639,15 @@ int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
     contig = contig && static_cast<const char*>(x[k]) ==
                            static_cast<const char*>(x[0]) + k * g.size() * esz;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const size_t groups = g.size() * (size_t)nvar;
-  int lanes = 1;
-  while (lanes < 16 && groups * lanes < (1u << 20)) lanes *= 2;
+  if (lanes == 0) {  // default: enough threads to fill the chip
+    const size_t groups = g.size() * (size_t)nvar;
+    lanes = 1;
+    while (lanes < 16 && groups * lanes < (1u << 20)) lanes *= 2;
+  }
+  if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16) {
+    set_error("stencil_apply_sym: lanes must be 0, 1, 2, 4, 8 or 16");
+    return kErrArg;
+  }
   if (dtype != kF64 && dtype != kF32) {
     set_error("stencil_apply_sym: bad dtype %d", dtype);
     return kErrArg;
@@ -651,7 +689,7 @@ int odil_stencil_apply_t(int dtype, int ndim, const int* nw, int stride,
                          const int* tent, const int* istart, const void* coef,
                          const void* const* x, void* const* y, int accumulate,
                          void* stream) {
-  if (bad_dim(ndim, "stencil_apply_t") ||
+  if (bad_dim(ndim, nw, "stencil_apply_t") ||
       bad_var(nvar_in, "stencil_apply_t") ||
       bad_var(nvar_out, "stencil_apply_t"))
     return kErrArg;
@@ -687,7 +725,7 @@ int odil_stencil_apply_t(int dtype, int ndim, const int* nw, int stride,
 int odil_stencil_normal(int dtype_in, int dtype_out, int ndim, const int* nw,
                         const int* terms, const int* tstart, int nout,
                         const void* jc, void* out, void* stream) {
-  if (bad_dim(ndim, "stencil_normal")) return kErrArg;
+  if (bad_dim(ndim, nw, "stencil_normal")) return kErrArg;
   if (nout < 1 || nout > 65535) {
     set_error("stencil_normal: bad number of output entries %d", nout);
     return kErrArg;
@@ -727,7 +765,7 @@ int odil_stencil_galerkin(int dtype, int ndim, const int* nw, const int* oent,
     set_error("stencil_galerkin: sigma must be +1 or -1");
     return kErrArg;
   }
-  if (bad_dim(ndim, "stencil_galerkin")) return kErrArg;
+  if (bad_dim(ndim, nw, "stencil_galerkin")) return kErrArg;
   Grid3 g = mk_grid(ndim, nw);
   if (g.n0 % 2 == 0 || g.n1 % 2 == 0 || (ndim == 3 && g.n2 % 2 == 0)) {
     set_error("stencil_galerkin: node grids need odd sizes");
@@ -758,7 +796,7 @@ int odil_stencil_galerkin(int dtype, int ndim, const int* nw, const int* oent,
 int odil_mg_restrict(int dtype, int ndim, const int* nw, int nvar,
                      const double* rw, int rmode, const void* const* fine,
                      void* const* coarse, void* stream) {
-  if (bad_dim(ndim, "mg_restrict") || bad_var(nvar, "mg_restrict"))
+  if (bad_dim(ndim, nw, "mg_restrict") || bad_var(nvar, "mg_restrict"))
     return kErrArg;
   Grid3 g = mk_grid(ndim, nw);
   Grid3 gh{g.n0 / 2 + 1, g.n1 / 2 + 1, ndim == 3 ? g.n2 / 2 + 1 : 1};
@@ -781,7 +819,7 @@ int odil_mg_restrict(int dtype, int ndim, const int* nw, int nvar,
 int odil_mg_prolong_add(int dtype, int ndim, const int* nw, int nvar,
                         const void* const* coarse, void* const* fine,
                         void* stream) {
-  if (bad_dim(ndim, "mg_prolong_add") || bad_var(nvar, "mg_prolong_add"))
+  if (bad_dim(ndim, nw, "mg_prolong_add") || bad_var(nvar, "mg_prolong_add"))
     return kErrArg;
   Grid3 g = mk_grid(ndim, nw);
   if (g.n0 % 2 == 0 || g.n1 % 2 == 0 || (ndim == 3 && g.n2 % 2 == 0)) {
@@ -811,7 +849,7 @@ int odil_mg_prolong_add(int dtype, int ndim, const int* nw, int nvar,
 int odil_stencil_to_dense(int dtype, int ndim, const int* nw, int nvar,
                           const int* ent, int nent, const void* coef, int sym,
                           double* M, void* stream) {
-  if (bad_dim(ndim, "stencil_to_dense") || bad_var(nvar, "stencil_to_dense"))
+  if (bad_dim(ndim, nw, "stencil_to_dense") || bad_var(nvar, "stencil_to_dense"))
     return kErrArg;
   Grid3 g = mk_grid(ndim, nw);
   hipStream_t s = static_cast<hipStream_t>(stream);

@@ -50,6 +50,7 @@ class BlockStencil:
     def __init__(self, coef, entries, nw, stride=1, nvar_in=None,
                  nvar_out=None, sym=False):
         self.sym = bool(sym)
+        self.lanes = 0  # threads per cell of the symmetric sweep, 0 = auto
         self.nw = tuple(int(n) for n in nw)
         self.ndim = len(self.nw)
         self.stride = int(stride)
@@ -177,6 +178,7 @@ class BlockStencil:
                 _grid_arr(self.nw), c_int(self.nvar_out), ptr(self.t_items),
                 ptr(self.t_iostart), ptr(self.t_dent), self._reach,
                 c_int(self._max_items), c_int(len(self.entries)),
+                c_int(self.lanes),
                 ptr(self.coef), ptrs(x), ptrs(y),
                 ptrs(f) if f is not None else None, c_int(mode),
                 c_double(omega), stream())
@@ -214,6 +216,24 @@ class BlockStencil:
             raise ValueError('Jacobi needs a diagonal entry in every block')
         self._call_apply(x, y, f, 2, omega)
         return y
+
+    def tune(self, x, f, y, omega=0.5):
+        """Pick the threads per cell of the symmetric sweep by timing the
+        Jacobi sweep on these buffers (cached per operator shape)."""
+        if not self.sym:
+            return self
+        from odil_amd import autotune
+        contig = all(a.data_ptr() == x[0].data_ptr() +
+                     k * x[0].numel() * x[0].element_size()
+                     for k, a in enumerate(x))
+        key = ('apply_sym', self.nw, len(self.entries), self._max_items,
+               str(self.dtype), contig)
+
+        def run(lanes):
+            self.lanes = lanes
+            self._call_apply(x, y, f, 2, omega)
+        self.lanes = autotune.pick(key, [0, 1, 2, 4, 8, 16], run)
+        return self
 
     def apply_t(self, x, y=None, accumulate=False):
         """y = A^T x (mul_transpose_field); accumulate adds into y."""
@@ -254,9 +274,11 @@ class BlockStencil:
     def astype(self, dtype):
         if dtype == self.dtype:
             return self
-        return BlockStencil(self.coef.to(dtype), self.entries, self.nw,
-                            self.stride, self.nvar_in, self.nvar_out,
-                            sym=self.sym)
+        out = BlockStencil(self.coef.to(dtype), self.entries, self.nw,
+                           self.stride, self.nvar_in, self.nvar_out,
+                           sym=self.sym)
+        out.lanes = self.lanes
+        return out
 
     def tocsr(self):
         """SciPy CSR copy on the host (tests and the CSR export)."""
@@ -565,6 +587,7 @@ class MultigridOp:
                                    self.galerkin_mode))
             self.AA = AA
             self._coarse_inv = {}
+            self._tune()
         else:
             self.AA[0].coef.copy_(A0.coef)
             for level in range(1, self.nlevels):
@@ -573,6 +596,15 @@ class MultigridOp:
             for level in list(self._coarse_inv):
                 self._coarse_inv[level].copy_(self._invert(level))
         return self
+
+    def _tune(self):
+        """Autotune the sweep launch shape of every stored level on the
+        level buffers."""
+        buf = self._buffers()
+        for level, A in enumerate(self.AA):
+            if A.sym:
+                b = buf[level]
+                A.tune(b['u'], b['f'], b['v'])
 
     def _invert(self, level):
         """Dense FP64 solution operator of the coarsest level.

@@ -110,3 +110,42 @@ def test_fixture_state_residual(dev):
     for e in range(3):
         _close(res[e].cpu(), ff[e], 1e-12, scale=0.01 * 128**2)
         assert np.sqrt(ssq[e].item() / 128**2) < 1e-5
+
+
+# -- end to end: Newton with the GMG normal-equation solver ------------------
+
+@pytest.fixture(scope='module')
+def oracle_cavity65():
+    """Converged Re=100 cavity on 65x65 cells from the oracle's Newton with
+    the reference's direct normal-equation solve (linsolver.py:35-45), 40
+    steps from zero (max RMS residual ~2e-12)."""
+    from oracle import newton
+    n = 65
+    dom = ops.cavity_domain(n)
+    zero = {k: np.zeros((n, n)) for k in 'uvp'}
+    packed = newton.optimize_newton(ops.make_cavity_operator(0.01), dom,
+                                    zero, 40)
+    return tape.unpack_fields(dom, packed)
+
+
+@pytest.mark.parametrize('precision,tol,rtol', [('fp64', 1e-11, 1e-8),
+                                                ('mixed', 1e-9, 1e-6)])
+def test_newton_multigrid_matches_oracle_direct(dev, oracle_cavity65,
+                                                precision, tol, rtol):
+    """The GPU Newton-multigrid solve (FCG on J^T J, V(2,2) preconditioner,
+    5 levels) reaches the same converged cavity solution as the oracle's
+    direct Newton; pressure is compared up to its constant (null mode)."""
+    from odil_amd.newton import NewtonMultigrid
+    from odil_amd.operators import Cavity
+    n = 65
+    op = Cavity((n, n), 0.01)
+    nm = NewtonMultigrid(op, nlvl=5, maxiter=10, precision=precision)
+    fields = [torch.zeros(n, n, dtype=torch.float64, device=dev)
+              for _ in range(3)]
+    hist = nm.run(fields, 80, tol=tol)
+    assert max(hist[-1][1]) <= tol, hist[-1]
+    ref = oracle_cavity65
+    for k, name in enumerate('uv'):
+        _close(fields[k].cpu().numpy(), ref[name], rtol)
+    p = fields[2].cpu().numpy()
+    _close(p - p.mean(), ref['p'] - ref['p'].mean(), rtol)
