@@ -301,6 +301,15 @@ int odil_mg_prolong_add(int dtype, int ndim, const int* nw, int nvar,
 int odil_stencil_to_dense(int dtype, int ndim, const int* nw, int nvar,
                           const int* ent, int nent, const void* coef, int sym,
                           double* M, void* stream);
+/* CSR of a stride-1 block stencil operator (J from odil_X_jac) in the layout
+ * of Problem.linearize (util_op.py:877-894): row e N + c, column
+ * k N + flat((c + s) mod shape) (np.roll wrap), int32 rowptr [nout N + 1],
+ * col and val [nent N] with every row of field e holding its cnt_e entries
+ * in entry order (explicit zeros kept). `ent`/`ostart` as above. */
+int odil_stencil_to_csr(int dtype, int ndim, const int* nw, int nout,
+                        const int* ent, const int* ostart, int nent,
+                        const void* coef, int* rowptr, int* col, void* val,
+                        void* stream);
 /* y = M x with M dense FP64 [nvar N][nvar N] (the coarsest-level solve as
  * a precomputed solution operator). */
 int odil_dense_gemv(int dtype_x, int dtype_y, int nvar, int N,
@@ -326,6 +335,20 @@ int odil_axpy_ratio(int dtype, int nvar, long long N, void* const* y,
 int odil_xpay_ratio(int dtype, int nvar, long long N, void* const* p,
                     const void* const* z, const double* num,
                     const double* sub, const double* den, void* stream);
+
+/* ---- optimizer kernels -------------------------------------------------- */
+
+/* One fused Adam step in place (AdamNativeOptimizer.run, optimizer.py:
+ * 303-314): m += (g - m)(1 - beta1), v += (g^2 - v)(1 - beta2),
+ * x -= alpha m / (sqrt(v) + eps), alpha = lr sqrt(1 - beta2^t) /
+ * (1 - beta1^t) from the caller. */
+int odil_adam_step(int dtype, long long N, void* x, const void* g, void* m,
+                   void* v, double alpha, double beta1, double beta2,
+                   double eps, void* stream);
+/* out = x + a y (line-search trial points and gradient descent,
+ * GdOptimizer, optimizer.py:241-268); out may alias x or y. */
+int odil_waxpy(int dtype, long long N, void* out, const void* x,
+               const void* y, double a, void* stream);
 
 #ifdef __cplusplus
 }

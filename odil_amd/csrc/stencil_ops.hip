@@ -434,6 +434,41 @@ __global__ void __launch_bounds__(256)
   if (sym && row != col) M[col * (nvar * N) + row] = v;
 }
 
+// CSR of a stride-1 block stencil operator in the layout of the reference's
+// Problem.linearize (util_op.py:877-894): row e N + c, column
+// k N + flat((c + s) mod shape) (periodic wrap, as np.roll), value
+// coef_q[c]. Entries are sorted by output field; row (e, c) holds the
+// cnt_e = ostart[e + 1] - ostart[e] entries of field e in entry order, so
+// rowptr[e N + c] = N ostart[e] + c cnt_e. One thread per (e, c).
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_to_csr(const int* __restrict__ ent, const int* __restrict__ ostart,
+             int nout, const T* __restrict__ coef, int* __restrict__ rowptr,
+             int* __restrict__ col, T* __restrict__ val, Grid3 g) {
+  const size_t N = g.size();
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * (size_t)nout) return;
+  const int e = (int)(t / N);
+  const size_t c = t - (size_t)e * N;
+  int c0, c1, c2;
+  unflat(c, g, c0, c1, c2);
+  const int q0 = ostart[e], cnt = ostart[e + 1] - q0;
+  const size_t base = N * (size_t)q0 + c * (size_t)cnt;
+  rowptr[t] = (int)base;
+  if (t == N * (size_t)nout - 1) rowptr[t + 1] = (int)(N * (size_t)ostart[nout]);
+  for (int j = 0; j < cnt; ++j) {
+    const int* en = ent + 5 * (q0 + j);
+    int a0 = (c0 + en[2]) % g.n0, a1 = (c1 + en[3]) % g.n1,
+        a2 = (c2 + en[4]) % g.n2;
+    a0 += a0 < 0 ? g.n0 : 0;
+    a1 += a1 < 0 ? g.n1 : 0;
+    a2 += a2 < 0 ? g.n2 : 0;
+    col[base + j] = (int)((size_t)en[1] * N +
+                          ((size_t)a0 * g.n1 + a1) * g.n2 + a2);
+    val[base + j] = coef[(size_t)(q0 + j) * N + c];
+  }
+}
+
 // y = M x for a dense row-major FP64 matrix, one wavefront per row; x and y
 // may be FP32 or FP64 (the coarse solve stays FP64 inside).
 template <class Tx, class Ty>
@@ -865,6 +900,40 @@ int odil_stencil_to_dense(int dtype, int ndim, const int* nw, int nvar,
     return kErrArg;
   }
   return check_launch("stencil_to_dense");
+}
+
+// CSR (int32 rowptr [nout N + 1], col / val [nent N]) of a stride-1 block
+// stencil operator in the reference linearize layout; `ent` must be sorted
+// by output field with `ostart` delimiting the fields.
+int odil_stencil_to_csr(int dtype, int ndim, const int* nw, int nout,
+                        const int* ent, const int* ostart, int nent,
+                        const void* coef, int* rowptr, int* col, void* val,
+                        void* stream) {
+  if (bad_dim(ndim, nw, "stencil_to_csr")) return kErrArg;
+  if (nout < 1) {
+    set_error("stencil_to_csr: nout must be positive");
+    return kErrArg;
+  }
+  Grid3 g = mk_grid(ndim, nw);
+  if ((double)g.size() * (nent > nout ? nent : nout) >= 2147483647.0) {
+    set_error("stencil_to_csr: more than 2^31 - 1 non-zeros");
+    return kErrCapacity;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  dim3 grid(nblk(g.size() * (size_t)nout));
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_to_csr<double>, grid, dim3(256), 0, s, ent, ostart,
+                       nout, static_cast<const double*>(coef), rowptr, col,
+                       static_cast<double*>(val), g);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_to_csr<float>, grid, dim3(256), 0, s, ent, ostart,
+                       nout, static_cast<const float*>(coef), rowptr, col,
+                       static_cast<float*>(val), g);
+  else {
+    set_error("stencil_to_csr: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  return check_launch("stencil_to_csr");
 }
 
 // y = M x with M dense FP64 (nvar*N square); x, y split into nvar fields of

@@ -85,6 +85,37 @@ __global__ void __launch_bounds__(256)
       p.p[k][i] = z.p[k][i] + b * p.p[k][i];
 }
 
+// One fused Adam step (AdamNativeOptimizer.run, optimizer.py:303-314):
+//   m += (g - m)(1 - b1); v += (g^2 - v)(1 - b2); x -= alpha m / (sqrt v + eps)
+// with alpha = lr sqrt(1 - b2^t) / (1 - b1^t) computed by the caller.
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_adam(T* __restrict__ x, const T* __restrict__ g, T* __restrict__ m,
+           T* __restrict__ v, size_t N, T alpha, T b1, T b2, T eps) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += stride) {
+    const T gi = g[i];
+    T mi = m[i], vi = v[i];
+    mi += (gi - mi) * (T(1) - b1);
+    vi += (gi * gi - vi) * (T(1) - b2);
+    m[i] = mi;
+    v[i] = vi;
+    x[i] -= mi * alpha / (sqrt(vi) + eps);
+  }
+}
+
+// out = x + a y (host scalar a; out may alias x or y).
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_waxpy(T* out, const T* x, const T* y, size_t N, double a) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const T aa = T(a);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += stride)
+    out[i] = x[i] + aa * y[i];
+}
+
 }  // namespace odil
 
 using namespace odil;
@@ -186,6 +217,57 @@ int odil_xpay_ratio(int dtype, int nvar, long long N, void* const* p,
     return kErrArg;
   }
   return check_launch("xpay_ratio");
+}
+
+// One Adam step on N values in place (x, m, v updated; g read).
+int odil_adam_step(int dtype, long long N, void* x, const void* g, void* m,
+                   void* v, double alpha, double beta1, double beta2,
+                   double eps, void* stream) {
+  if (N < 0 || !x || !g || !m || !v) {
+    set_error("adam_step: bad arguments");
+    return kErrArg;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_adam<double>, dim3(grid_for(N)), dim3(256), 0, s,
+                       static_cast<double*>(x), static_cast<const double*>(g),
+                       static_cast<double*>(m), static_cast<double*>(v),
+                       (size_t)N, alpha, beta1, beta2, eps);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_adam<float>, dim3(grid_for(N)), dim3(256), 0, s,
+                       static_cast<float*>(x), static_cast<const float*>(g),
+                       static_cast<float*>(m), static_cast<float*>(v),
+                       (size_t)N, (float)alpha, (float)beta1, (float)beta2,
+                       (float)eps);
+  else {
+    set_error("adam_step: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  return check_launch("adam_step");
+}
+
+// out = x + a y on N values.
+int odil_waxpy(int dtype, long long N, void* out, const void* x,
+               const void* y, double a, void* stream) {
+  if (N < 0 || !out || !x || !y) {
+    set_error("waxpy: bad arguments");
+    return kErrArg;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_waxpy<double>, dim3(grid_for(N)), dim3(256), 0, s,
+                       static_cast<double*>(out),
+                       static_cast<const double*>(x),
+                       static_cast<const double*>(y), (size_t)N, a);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_waxpy<float>, dim3(grid_for(N)), dim3(256), 0, s,
+                       static_cast<float*>(out), static_cast<const float*>(x),
+                       static_cast<const float*>(y), (size_t)N, a);
+  else {
+    set_error("waxpy: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  return check_launch("waxpy");
 }
 
 }  // extern "C"

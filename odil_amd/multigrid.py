@@ -271,6 +271,32 @@ class BlockStencil:
         check(st, 'stencil_to_dense')
         return M
 
+    def to_csr_arrays(self):
+        """Device CSR (rowptr, col, val) in the reference linearize layout
+        (util_op.py:877-894): row out N + c, column in N + flat((c + s) mod
+        shape), entries of each row in entry order, explicit zeros kept.
+        Stride-1 operators only."""
+        if self.stride != 1:
+            raise ValueError('to_csr_arrays needs a stride-1 operator')
+        if self.sym:
+            raise ValueError('to_csr_arrays needs full (non-symmetric) '
+                             'storage')
+        N = int(np.prod(self.nw))
+        dev = self.device
+        rowptr = torch.empty(self.nvar_out * N + 1, dtype=torch.int32,
+                             device=dev)
+        col = torch.empty(len(self.entries) * N, dtype=torch.int32,
+                          device=dev)
+        val = torch.empty(len(self.entries) * N, dtype=self.dtype,
+                          device=dev)
+        st = _lib.lib().odil_stencil_to_csr(
+            c_int(_lib.dtype_tag(self.dtype)), c_int(self.ndim),
+            _grid_arr(self.nw), c_int(self.nvar_out), ptr(self.t_ent),
+            ptr(self.t_ostart), c_int(len(self.entries)), ptr(self.coef),
+            ptr(rowptr), ptr(col), ptr(val), stream())
+        check(st, 'stencil_to_csr')
+        return rowptr, col, val
+
     def astype(self, dtype):
         if dtype == self.dtype:
             return self

@@ -1,0 +1,235 @@
+"""Reference-shaped Problem API, Newton with the GPU direct solve, and the
+gradient optimizers, against the oracle and the reference's stored data."""
+
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import operators as ops
+from oracle import tape
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), 'golden')
+
+
+def _cavity_problem(n0, n1, dev, mu=0.01):
+    from odil_amd.operators import Cavity
+    from odil_amd.problem import Domain, Problem
+    dom = Domain(shape=(n0, n1), varnames=('x', 'y'),
+                 fieldnames=['u', 'v', 'p'], device=dev)
+    return Problem(Cavity((n0, n1), mu), dom)
+
+
+def _wave_problem(nt, nx, dev):
+    from odil_amd.operators import Wave
+    from odil_amd.problem import Domain, Problem
+    odom, _, exact = ops.wave_reference_setup(nt, nx)
+    t1, x1 = odom.cell_center_1d(0), odom.cell_center_1d(1)
+    left, _ = ops.wave_exact(t1, t1 * 0 - 1)
+    right, _ = ops.wave_exact(t1, t1 * 0 + 1)
+    iu, iut = ops.wave_exact(x1 * 0, x1)
+    dom = Domain(shape=(nt, nx), lower=(0, -1), upper=(1, 1),
+                 varnames=('t', 'x'), fieldnames=['u'], device=dev)
+    return Problem(Wave((nt, nx), iu, iut, left, right, device=dev),
+                   dom), exact
+
+
+def test_linearize_matches_oracle_csr(dev):
+    """Device CSR of the split-shift Jacobian: its raw layout follows the
+    linearize rule (util_op.py:877-894: row e N + c, column
+    k N + roll(col_diag, -s)[c]) exactly, and as a matrix it equals the
+    oracle's linearize."""
+    from odil_amd.problem import State
+    rng = np.random.default_rng(1)
+    n0, n1 = 11, 13
+    N = n0 * n1
+    st = {k: rng.uniform(-1, 1, (n0, n1)) for k in 'uvp'}
+    pb = _cavity_problem(n0, n1, dev)
+    J, _ = pb.jacobian(State(dict(st)))
+    rowptr, col, val = (a.cpu().numpy() for a in J.to_csr_arrays())
+    col_diag = np.arange(N).reshape(n0, n1)
+    coef = J.coef.cpu().numpy()
+    for e in range(3):
+        qs = [q for q, en in enumerate(J.entries) if en[0] == e]
+        for c in (0, 5, N // 2, N - 1):
+            r = e * N + c
+            lo = rowptr[r]
+            assert rowptr[r + 1] - lo == len(qs)
+            for j, q in enumerate(qs):
+                _, k, sh = J.entries[q]
+                want = k * N + np.roll(col_diag, np.negative(sh),
+                                       axis=(0, 1)).ravel()[c]
+                assert col[lo + j] == want
+                assert val[lo + j] == coef[q].ravel()[c]
+    const, m = pb.linearize(State(dict(st)))
+    oconst, om = tape.linearize(ops.make_cavity_operator(0.01),
+                                ops.cavity_domain(n0, n1), st)
+    assert m.shape == om.shape
+    assert abs(m - om.tocsr()).max() <= 1e-12 * abs(om).max()
+    for a, b in zip(const, oconst):
+        np.testing.assert_allclose(a, b, rtol=1e-12,
+                                   atol=1e-12 * abs(b).max())
+    # the device CSR tensor gives the same products
+    _, md = pb.linearize(State(dict(st)), device=True)
+    x = rng.normal(size=m.shape[1])
+    y = (md @ torch.tensor(x, device=dev).unsqueeze(1)).squeeze(1)
+    np.testing.assert_allclose(y.cpu().numpy(), m @ x, rtol=1e-12,
+                               atol=1e-12 * abs(m @ x).max())
+
+
+def test_eval_loss_grad_and_packing(dev):
+    from odil_amd.problem import State
+    rng = np.random.default_rng(2)
+    n0, n1 = 17, 12
+    st = {k: rng.uniform(-1, 1, (n0, n1)) for k in 'uvp'}
+    pb = _cavity_problem(n0, n1, dev)
+    loss, grads, wgrads, split = pb.eval_loss_grad(State(dict(st)))
+    ol, og, osplit = tape.eval_loss_grad(ops.make_cavity_operator(0.01),
+                                         ops.cavity_domain(n0, n1), st)
+    assert wgrads == {}
+    np.testing.assert_allclose(loss.item(), ol, rtol=1e-12)
+    np.testing.assert_allclose([s.item() for s in split], osplit,
+                               rtol=1e-11)
+    for k in 'uvp':
+        np.testing.assert_allclose(grads[k].cpu().numpy(), og[k],
+                                   rtol=1e-11,
+                                   atol=1e-12 * abs(og[k]).max())
+    s = State(dict(st))
+    packed = pb.pack_state(s)
+    assert packed.numel() == 3 * n0 * n1
+    back = pb.unpack_fields(packed)
+    for k in 'uvp':
+        np.testing.assert_array_equal(back[k].cpu().numpy(), st[k])
+    opacked = tape.pack_fields(ops.cavity_domain(n0, n1), st)
+    np.testing.assert_array_equal(packed.cpu().numpy(), opacked)
+
+
+@pytest.mark.parametrize('nx', [3, 8, 16, 33, 64])
+def test_newton_direct_reproduces_reference_wave_error(dev, nx):
+    """Two GPU Newton steps (stencil Jacobian, J^T J, Cholesky on the GPU)
+    from zero give the reference's stored discretisation error of the wave
+    case (cases/wave_cost/precomputed/fd_newton/lines.pickle)."""
+    from odil_amd.drivers import optimize_newton
+    from odil_amd.problem import State
+    g = json.load(open(os.path.join(GOLD, 'wave_fd_newton.json')))
+    ref = g['error'][g['nx'].index(nx)]
+    pb, exact = _wave_problem(nx, nx, dev)
+    st = State()
+    optimize_newton(pb, st, 2)
+    u = st.fields['u'].cpu().numpy()
+    err = np.mean((exact - u)**2)**0.5
+    assert abs(err - ref) <= 1e-10 * ref, (err, ref)
+
+
+def test_newton_direct_wave_gaussian_to_tolerance(dev):
+    """The 64x64 Gaussian wave case (FP64) reaches residual 1e-10 and
+    equals the oracle's direct Newton solution."""
+    from odil_amd.drivers import optimize_newton
+    from odil_amd.operators import Wave
+    from odil_amd.problem import Domain, Problem, State
+    from oracle import newton
+    odom, oop, bnd = ops.wave_gaussian_setup(64, 64, seed=0)
+    dom = Domain(shape=(64, 64), lower=(0, -1), upper=(1, 1),
+                 varnames=('t', 'x'), fieldnames=['u'], device=dev)
+    pb = Problem(Wave((64, 64), *bnd, device=dev), dom)
+    st = State()
+    hist = optimize_newton(pb, st, 5, tol=1e-10)
+    assert max(hist[-1][1]) <= 1e-10
+    packed = newton.optimize_newton(oop, odom, {'u': np.zeros((64, 64))}, 2)
+    ref = tape.unpack_fields(odom, packed)['u']
+    u = st.fields['u'].cpu().numpy()
+    assert np.abs(u - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_adam_kernel_matches_reference_update(dev):
+    """Fused Adam step equals the update of optimizer.py:303-314 restated
+    in NumPy (FP64), over several steps with a varying gradient."""
+    from odil_amd.optimizer import AdamNativeOptimizer
+    rng = np.random.default_rng(3)
+    n = 1000
+    a = rng.uniform(0.5, 2, n)
+    b = rng.normal(size=n)
+
+    def loss_grad_np(x):
+        return 0.5 * np.sum(a * x**2) - b @ x, a * x - b
+
+    def loss_grad(x, epoch):
+        l, gr = loss_grad_np(x.cpu().numpy())
+        return (torch.tensor(l), torch.tensor(gr, device=dev),
+                [torch.tensor(0.)])
+
+    x0 = rng.normal(size=n)
+    xg, _ = AdamNativeOptimizer().run(torch.tensor(x0, device=dev),
+                                      loss_grad, epochs=25, lr=1e-2)
+    x, m, v = x0.copy(), np.zeros(n), np.zeros(n)
+    for t in range(1, 26):
+        _, gr = loss_grad_np(x)
+        alpha = 1e-2 * np.sqrt(1 - 0.999**t) / (1 - 0.9**t)
+        m += (gr - m) * (1 - 0.9)
+        v += (gr**2 - v) * (1 - 0.999)
+        x -= m * alpha / (np.sqrt(v) + 1e-7)
+    np.testing.assert_allclose(xg.cpu().numpy(), x, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize('optname', ['lbfgs', 'lbfgsb'])
+def test_lbfgs_solves_poisson_loss(dev, optname):
+    """L-BFGS on the Poisson discrete loss (a convex quadratic) reaches the
+    exact discrete solution (direct solve of the same equations)."""
+    from odil_amd.drivers import optimize_opt
+    from odil_amd.operators import Poisson
+    from odil_amd.problem import Domain, Problem, State
+    n = 16
+    src = ops.gaussian_mixture_source(n, seed=4)
+    dom = Domain(shape=(n, n), varnames=('x', 'y'), fieldnames=['u'],
+                 device=dev)
+    pb = Problem(Poisson((n, n), torch.tensor(src, device=dev)), dom)
+    st, info = optimize_opt(pb, State(), optname, 800, m=10)
+    loss, _, _, _ = pb.eval_loss_grad(st)
+    assert loss.item() <= 1e-12 * np.mean(src**2), loss.item()
+    const, m = tape.linearize(ops.make_poisson_operator(src),
+                              tape.Domain(ndim=2, shape=(n, n),
+                                          varnames=('x', 'y'),
+                                          fieldnames=['u']),
+                              {'u': np.zeros((n, n))})
+    import scipy.sparse.linalg
+    ref = scipy.sparse.linalg.spsolve(m.tocsc(), -const[0].ravel())
+    u = st.fields['u'].cpu().numpy().ravel()
+    assert np.abs(u - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+def test_advection_fp32_lbfgs_and_gradient_fd(dev):
+    """Advection (FP32 residual and adjoint): the FP32 adjoint's
+    directional derivative matches central finite differences of the FP64
+    oracle loss to 1e-4 relative, and L-BFGS (m = 10) on the device
+    reduces the loss like SciPy's L-BFGS-B does on the FP64 oracle loss
+    (0.015x after 60 iterations, 3e-5x after 200: a transport problem, so
+    progress is about one time row per iteration)."""
+    from odil_amd.drivers import optimize_opt
+    from odil_amd.operators import Advection
+    from odil_amd.problem import Domain, Problem, State
+    shape = (32, 32, 32)
+    odom, oop, (ax, ay, p0) = ops.advection_setup(*shape, seed=7)
+    op = Advection(shape, ax, ay, torch.tensor(p0, dtype=torch.float32,
+                                                device=dev))
+    dom = Domain(shape=shape, varnames=('t', 'x', 'y'), fieldnames=['p'],
+                 dtype=torch.float32, device=dev)
+    pb = Problem(op, dom)
+    rng = np.random.default_rng(8)
+    p = rng.normal(size=shape)
+    d = rng.normal(size=shape)
+    _, g, _, _ = pb.eval_loss_grad(State({'p': p}))
+    gd = float(np.sum(g['p'].double().cpu().numpy() * d))
+    h = 1e-3
+    lp, _, _ = tape.eval_loss_grad(oop, odom, {'p': p + h * d})
+    lm, _, _ = tape.eval_loss_grad(oop, odom, {'p': p - h * d})
+    fd = (lp - lm) / (2 * h)
+    assert abs(gd - fd) <= 1e-4 * abs(fd), (gd, fd)
+    st = State()
+    l0 = pb.eval_loss_grad(st)[0].item()
+    st, info = optimize_opt(pb, st, 'lbfgs', 200, m=10)
+    l1 = pb.eval_loss_grad(st)[0].item()
+    assert l1 <= 1e-3 * l0, (l0, l1)
