@@ -203,6 +203,31 @@ int odil_laplace_jtf(int dtype, int n0, int n1, double dx, double dy,
                      double scale, double* partial, int* nblocks,
                      void* stream);
 
+/* Inverse wave problem (speed recovery from sensor traces; no counterpart
+ * in the reference, which has the forward wave.py case only). Loss =
+ * mean(f_wave^2) + mean(f_data^2) + k_reg^2 mean((dc/dx)^2) with f_wave the
+ * odil_wave residual with c(x)^2 u_xx (c [nx], field dtype) and
+ * f_data = k_data (u - data[t][s]) on the columns with sensor_of[x] = s >= 0
+ * (int32 [nx], -1 elsewhere; data [nt][nsensor], field dtype), the
+ * reference's imposed-point equation form (ns_steady.py:396-411). Writes
+ * grad_u [nt][nx], grad_c [nx] and split[3] (device FP64: the three loss
+ * terms); `work` holds odil_wave_inverse_work(dtype, nt, nx) doubles. */
+int odil_wave_inverse_work(int dtype, int nt, int nx);
+int odil_wave_inverse_loss_grad(int dtype, int nt, int nx, double dt,
+                                double dx, const double* init_u,
+                                const double* init_ut, const double* left_u,
+                                const double* right_u, const void* c,
+                                const int* sensor_of, const void* data,
+                                int nsensor, double k_data, double k_reg,
+                                const void* u, void* grad_u, void* grad_c,
+                                double* work, double* split, void* stream);
+/* Solution of f_wave = 0 marched row by row (one workgroup): synthetic
+ * sensor data of the inverse problem; c may be NULL (c = 1). */
+int odil_wave_march(int dtype, int nt, int nx, double dt, double dx,
+                    const double* init_u, const double* init_ut,
+                    const double* left_u, const double* right_u,
+                    const void* c, void* u, void* stream);
+
 /* 2D linear advection p_t + a . grad p = 0 on an nt x nx x ny space-time
  * grid, periodic in x and y, first-order upwind on the sign of the constant
  * velocity (ax, ay), p at t = 0 from p_init [nx][ny] (advection_2d.py:

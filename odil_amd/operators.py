@@ -417,3 +417,102 @@ class Advection:
               'reduce_partials')
         split = ssq / self.ncell
         return split.sum(), [g], split
+
+
+class WaveInverse:
+    """Inverse wave problem: unknown field u on the (nt, nx) space-time grid
+    and unknown speed c(x) [nx]. Loss = mean(f_wave^2) + mean(f_data^2) +
+    k_reg^2 mean((dc/dx)^2): the wave residual with c(x)^2 u_xx (c = 1 is
+    wave.py:67-122), the misfit k_data (u - data) on the sensor columns
+    (imposed-point form of ns_steady.py:396-411) and a Tikhonov term on the
+    speed gradient. `loss_grad` is one call of
+    `odil_wave_inverse_loss_grad`."""
+
+    def __init__(self, shape, init_u, init_ut, left_u, right_u, sensors,
+                 data, k_data=1.0, k_reg=1e-2, lower=(0., -1.),
+                 upper=(1., 1.), device=None):
+        self.shape = tuple(int(s) for s in shape)
+        nt, nx = self.shape
+        self.ncell = nt * nx
+        device = device or torch.device('cuda')
+        self.device = device
+        self.dt = (upper[0] - lower[0]) / nt
+        self.dx = (upper[1] - lower[1]) / nx
+        self.bnd = [_dev64(a, device) for a in (init_u, init_ut, left_u,
+                                                 right_u)]
+        sensors = np.asarray(sensors, dtype=np.int64)
+        sof = np.full(nx, -1, dtype=np.int32)
+        sof[sensors] = np.arange(len(sensors), dtype=np.int32)
+        self.sensor_of = torch.tensor(sof, device=device)
+        self.nsensor = len(sensors)
+        self._data64 = torch.as_tensor(np.asarray(data), dtype=torch.float64,
+                                       device=device).contiguous()
+        if tuple(self._data64.shape) != (nt, self.nsensor):
+            raise ValueError('data must have shape (nt, nsensor)')
+        self._data = {torch.float64: self._data64}
+        self.k_data, self.k_reg = float(k_data), float(k_reg)
+
+    def _data_for(self, dtype):
+        if dtype not in self._data:
+            self._data[dtype] = self._data64.to(dtype)
+        return self._data[dtype]
+
+    def loss_grad(self, u, c, grad_u=None, grad_c=None):
+        """Returns (loss, grad_u, grad_c, split): loss and split (3 terms)
+        as float64 device tensors."""
+        _lib.require_device(u)
+        _lib.require_device(c)
+        nt, nx = self.shape
+        if tuple(u.shape) != self.shape or c.numel() != nx \
+                or c.dtype != u.dtype:
+            raise ValueError('u must be {} and c ({},) of one dtype'.format(
+                self.shape, nx))
+        dtag = _lib.dtype_tag(u.dtype)
+        gu = torch.empty_like(u) if grad_u is None else grad_u
+        gc = torch.empty_like(c) if grad_c is None else grad_c
+        nwork = _lib.lib().odil_wave_inverse_work(c_int(dtag), c_int(nt),
+                                                  c_int(nx))
+        work = torch.empty(nwork, dtype=torch.float64, device=u.device)
+        split = torch.empty(3, dtype=torch.float64, device=u.device)
+        st = _lib.lib().odil_wave_inverse_loss_grad(
+            c_int(dtag), c_int(nt), c_int(nx), c_double(self.dt),
+            c_double(self.dx), *[ptr(a) for a in self.bnd], ptr(c),
+            ptr(self.sensor_of), ptr(self._data_for(u.dtype)),
+            c_int(self.nsensor), c_double(self.k_data), c_double(self.k_reg),
+            ptr(u), ptr(gu), ptr(gc), ptr(work), ptr(split), _lib.stream())
+        check(st, 'wave_inverse_loss_grad')
+        return split.sum(), gu, gc, split
+
+    def march(self, c, dtype=torch.float64):
+        """Solution of f_wave = 0 for speed c (None: c = 1)."""
+        nt, nx = self.shape
+        u = torch.empty(self.shape, dtype=dtype, device=self.device)
+        if c is not None:
+            c = c.to(dtype).contiguous()
+        st = _lib.lib().odil_wave_march(
+            c_int(_lib.dtype_tag(dtype)), c_int(nt), c_int(nx),
+            c_double(self.dt), c_double(self.dx),
+            *[ptr(a) for a in self.bnd], ptr(c), ptr(u), _lib.stream())
+        check(st, 'wave_march')
+        return u
+
+    # -- packed state [u, c] for the gradient optimizers ------------------
+    def pack(self, u, c):
+        return torch.cat([u.reshape(-1), c.reshape(-1)])
+
+    def unpack(self, x):
+        n = self.ncell
+        return x[:n].view(self.shape), x[n:]
+
+    def make_loss_grad(self):
+        """loss_grad(x, epoch) -> (loss, grad, residual_list) over the
+        packed [u, c] (the optimizer interface, optimizer.py:18-28)."""
+
+        def loss_grad(x, epoch):
+            u, c = self.unpack(x)
+            g = torch.empty_like(x)
+            gu, gc = self.unpack(g)
+            loss, _, _, split = self.loss_grad(u, c, gu, gc)
+            return loss, g, [torch.sqrt(v) for v in split]
+
+        return loss_grad

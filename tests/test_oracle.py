@@ -251,3 +251,35 @@ def test_transpose_restriction_is_scaled_interpolation_transpose():
         R = omg.get_R(nw, 'transpose').tocsr()
         T = omg.get_TT(nw).tocsr()
         assert abs(R - T / 2**len(nw)).max() < 1e-15
+
+
+# -- inverse wave oracle ----------------------------------------------------
+def test_inverse_wave_oracle_pins():
+    """c = 1 reproduces the reference wave operator exactly; the forward
+    march zeroes the residual; u and c gradients match central FD."""
+    import torch
+    from oracle import inverse as inv
+    case = inv.wave_inverse_setup(24, 20, nsensor=6, seed=1)
+    rng = np.random.default_rng(0)
+    u = rng.normal(size=(24, 20))
+    fw = inv.wave_c_residual(torch.tensor(u), torch.ones(20,
+                                                         dtype=torch.float64),
+                             case).numpy()
+    op = ops.make_wave_operator(case['init_u'], case['init_ut'],
+                                case['left_u'], case['right_u'])
+    dom = tape.Domain(ndim=2, shape=(24, 20), lower=(0, -1), upper=(1, 1),
+                      varnames=('t', 'x'), fieldnames=['u'])
+    ff = tape.eval_operator(op, dom, {'u': u})
+    np.testing.assert_array_equal(fw, ff[0])
+    fw = inv.wave_c_residual(torch.tensor(case['u_true']),
+                             torch.tensor(case['c_true']), case).numpy()
+    assert np.abs(fw).max() <= 1e-12 * np.abs(case['u_true']).max() * 24**2
+    c = case['c_true'] * (1 + 0.1 * rng.normal(size=20))
+    _, _, gu, gc = inv.inverse_loss_grad(u, c, case, k_data=3.0, k_reg=0.1)
+    du, dc, h = rng.normal(size=u.shape), rng.normal(size=20), 1e-6
+    lp = inv.inverse_loss_grad(u + h * du, c + h * dc, case, 3.0, 0.1)[0]
+    lm = inv.inverse_loss_grad(u - h * du, c - h * dc, case, 3.0, 0.1)[0]
+    fd = (lp - lm) / (2 * h)
+    assert abs(np.sum(gu * du) + np.sum(gc * dc) - fd) <= 1e-7 * abs(fd)
+    with pytest.raises(ValueError):
+        inv.wave_inverse_setup(70, 130)  # explicit march CFL violated
