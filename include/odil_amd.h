@@ -242,6 +242,26 @@ int odil_advection_loss_grad(int dtype, int nt, int nx, int ny, double dt,
                              const void* p_init, const void* p, void* grad,
                              double* partial, void* stream);
 
+/* Batched Poisson (many independent instances of uneven size in one
+ * persistent launch; the reference solves one problem per process).
+ * Instance i owns cells [off[i], off[i] + n0[i] n1[i]) of the packed u, src,
+ * grad buffers (field dtype), spacings dx[i], dy[i] (device arrays:
+ * off int64, n0/n1 int32, dx/dy FP64). `tiles` int32 [ntiles][4] =
+ * (instance, x0, y0, 0) lists every tx x ty tile, grouped by instance with
+ * tstart[i] (int32 [ninst + 1]) the first tile of instance i; nblocks is
+ * the persistent grid size. Writes grad_i = (2 / N_i) J_i^T f_i,
+ * loss[i] = mean(f_i^2) (FP64) and per-tile sums of f^2 in partial
+ * [ntiles]. The per-instance residual is odil_poisson_residual's. */
+int odil_poisson_batch_shapes(int dtype, int* shapes, int cap);
+int odil_poisson_batch_loss_grad(int dtype, int ninst, const long long* off,
+                                 const int* n0, const int* n1,
+                                 const double* dx, const double* dy,
+                                 const int* tiles, int ntiles,
+                                 const int* tstart, int tx, int ty,
+                                 int nblocks, const void* u, const void* src,
+                                 void* grad, double* partial, double* loss,
+                                 void* stream);
+
 /* ---- block stencil operators and multigrid ------------------------------
  * A block stencil operator maps nvar_in fields on the node grid nw (2D or
  * 3D, host int[3], unused axes 1) to nvar_out fields on

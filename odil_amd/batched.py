@@ -1,0 +1,285 @@
+"""Many independent Poisson instances of uneven size on one GPU.
+
+* `PoissonBatch` packs the instances (u, src, grad) into single buffers and
+  evaluates every instance's residual, loss mean(f^2) and adjoint
+  (2/N) J^T f in ONE persistent launch over a worklist of tiles
+  (`odil_poisson_batch_loss_grad`); the tile shape is chosen by the
+  autotuner on the batch itself.
+* `PoissonBatchSolver` solves every instance's normal equations
+  J^T J u = J^T src (one Newton step of the linear problem, the
+  `optimize_multigridop_base` step of util.py:509-604) by flexible CG with
+  a GMG V-cycle preconditioner. The Poisson Jacobian does not depend on u or
+  src, so all instances of one size share one multigrid hierarchy; the
+  whole batch (every instance, every iteration, ~10^4 kernels) is captured
+  once into a single HIP graph and replayed.
+
+Node-based multigrid needs 2^k + 1 points per axis (multigrid.py:362-370),
+so `batch_sizes` draws sizes log-uniformly between 64 and 1024 and snaps
+them to the nearest 2^k + 1.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from odil_amd import _lib, autotune
+from odil_amd._lib import c_int, check, ptr, stream
+from odil_amd.multigrid import MultigridOp, normal_operator, BlockStencil
+from odil_amd.newton import DeviceScalars, FusedNormal, VectorOps
+from odil_amd.operators import Poisson
+
+
+def batch_sizes(count, seed, lo=64, hi=1024):
+    """`count` grid sizes n = round(lo (hi/lo)^U), U ~ U(0, 1), snapped to
+    the nearest 2^k + 1 in log scale."""
+    rng = np.random.default_rng(seed)
+    raw = lo * (hi / lo)**rng.uniform(0, 1, count)
+    ks = np.arange(int(np.log2(lo)), int(np.log2(hi)) + 1)
+    cand = 2.0**ks + 1
+    idx = np.abs(np.log(raw[:, None]) - np.log(cand[None, :])).argmin(1)
+    return [int(cand[i]) for i in idx]
+
+
+def gaussian_mixture(n, seed, nblobs=3):
+    """Synthetic source on the unit square: `nblobs` Gaussians with centres
+    U(0.2, 0.8)^2, widths U(0.05, 0.15), amplitudes U(-1, 1)."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(n) + 0.5) / n
+    xx, yy = np.meshgrid(x, x, indexing='ij')
+    s = np.zeros((n, n))
+    for _ in range(nblobs):
+        cx, cy = rng.uniform(0.2, 0.8, 2)
+        w = rng.uniform(0.05, 0.15)
+        a = rng.uniform(-1, 1)
+        s += a * np.exp(-((xx - cx)**2 + (yy - cy)**2) / (2 * w**2))
+    return s
+
+
+class PoissonBatch:
+    """Instances i = 0..count-1 of f = lap u - src on n_i x n_i cells of the
+    unit square, Dirichlet u = 0, packed back to back in C order."""
+
+    def __init__(self, sizes, sources, dtype=torch.float32, device=None):
+        self.device = device or torch.device('cuda')
+        self.dtype = dtype
+        self.sizes = [(int(n), int(n)) if np.isscalar(n) else tuple(n)
+                      for n in sizes]
+        self.count = len(self.sizes)
+        ncell = [a * b for a, b in self.sizes]
+        self.offsets = np.concatenate([[0], np.cumsum(ncell)]).astype(np.int64)
+        self.total = int(self.offsets[-1])
+        self.src = torch.empty(self.total, dtype=dtype, device=self.device)
+        for i, s in enumerate(sources):
+            self.view(self.src, i).copy_(torch.as_tensor(np.asarray(s)))
+        self.u = torch.zeros(self.total, dtype=dtype, device=self.device)
+        self.grad = torch.empty_like(self.u)
+        t = lambda a, dt: torch.tensor(a, dtype=dt, device=self.device)
+        self.t_off = t(self.offsets[:-1], torch.int64)
+        self.t_n0 = t([a for a, _ in self.sizes], torch.int32)
+        self.t_n1 = t([b for _, b in self.sizes], torch.int32)
+        self.t_dx = t([1. / a for a, _ in self.sizes], torch.float64)
+        self.t_dy = t([1. / b for _, b in self.sizes], torch.float64)
+        self.loss = torch.empty(self.count, dtype=torch.float64,
+                                device=self.device)
+        self._tiles = {}
+        self.shape = None
+        self.nblocks = 4 * torch.cuda.get_device_properties(
+            self.device).multi_processor_count
+
+    def view(self, buf, i):
+        a, b = self.sizes[i]
+        return buf[int(self.offsets[i]):int(self.offsets[i + 1])].view(a, b)
+
+    def shapes(self):
+        arr = (ctypes.c_int * 32)()
+        n = _lib.lib().odil_poisson_batch_shapes(
+            c_int(_lib.dtype_tag(self.dtype)), arr, c_int(16))
+        return [(arr[2 * k], arr[2 * k + 1]) for k in range(n)]
+
+    def tiles(self, tx, ty):
+        """Tile worklist (instance, x0, y0, 0) grouped by instance."""
+        key = (tx, ty)
+        if key not in self._tiles:
+            rows, tstart = [], [0]
+            for i, (a, b) in enumerate(self.sizes):
+                for x0 in range(0, a, tx):
+                    for y0 in range(0, b, ty):
+                        rows.append((i, x0, y0, 0))
+                tstart.append(len(rows))
+            tl = torch.tensor(np.array(rows, dtype=np.int32),
+                              device=self.device)
+            ts = torch.tensor(np.array(tstart, dtype=np.int32),
+                              device=self.device)
+            part = torch.empty(len(rows), dtype=torch.float64,
+                               device=self.device)
+            self._tiles[key] = (tl, ts, part)
+        return self._tiles[key]
+
+    def launch(self, shape=None, u=None, grad=None):
+        """One persistent launch: loss of every instance into self.loss and
+        the adjoints into `grad` (default self.grad)."""
+        tx, ty = shape or self.shape or self.shapes()[0]
+        tl, ts, part = self.tiles(tx, ty)
+        u = self.u if u is None else u
+        grad = self.grad if grad is None else grad
+        st = _lib.lib().odil_poisson_batch_loss_grad(
+            c_int(_lib.dtype_tag(self.dtype)), c_int(self.count),
+            ptr(self.t_off), ptr(self.t_n0), ptr(self.t_n1), ptr(self.t_dx),
+            ptr(self.t_dy), ptr(tl), c_int(tl.shape[0]), ptr(ts), c_int(tx),
+            c_int(ty), c_int(self.nblocks), ptr(u), ptr(self.src), ptr(grad),
+            ptr(part), ptr(self.loss), stream())
+        check(st, 'poisson_batch_loss_grad')
+        return self.loss, grad
+
+    def tune(self):
+        """Pick the tile shape of the batched launch by timing it."""
+        key = ('poisson_batch', tuple(self.sizes), str(self.dtype))
+        self.shape = autotune.pick(key, self.shapes(),
+                                   lambda s: self.launch(s))
+        return self.shape
+
+    def instance_ops(self):
+        """Single-grid `Poisson` operators viewing each instance."""
+        return [Poisson(self.sizes[i], self.view(self.src, i))
+                for i in range(self.count)]
+
+    def loss_grad_loop(self, ops=None):
+        """Reference point: one single-grid launch per instance."""
+        ops = ops or self.instance_ops()
+        out = []
+        for i, op in enumerate(ops):
+            nb = op.nblocks(1, _lib.dtype_tag(self.dtype))
+            if not hasattr(op, '_part'):
+                op._part = torch.empty((1, nb), dtype=torch.float64,
+                                       device=self.device)
+            op.launch_loss_grad([self.view(self.u, i)],
+                                [self.view(self.grad, i)], op._part)
+            out.append(op._part)
+        return out
+
+    def algorithmic_bytes(self):
+        """Read u and src, write grad, per cell."""
+        return 3 * self.total * self.u.element_size()
+
+
+class _SizeClass:
+    """Shared multigrid hierarchy and work vectors of one grid size."""
+
+    def __init__(self, shape, wdtype, nlvl, device, coarse_reg):
+        f64 = torch.float64
+        z = torch.zeros(shape, dtype=f64, device=device)
+        self.op = Poisson(shape, z)
+        jac, _ = self.op.jacobian([z])
+        N = normal_operator(BlockStencil.from_jacobian(jac), wdtype,
+                            sym=True)
+        self.mg = MultigridOp(shape, nvar=1, nlevels=nlvl,
+                              restriction='transpose', dtype=wdtype,
+                              device=device, coarse_reg=coarse_reg,
+                              galerkin_mode='exact')
+        self.mg.update_A(N)
+        self.state64 = [z]
+        self.state_w = [torch.zeros(shape, dtype=wdtype, device=device)]
+        self.fine = FusedNormal(self.op, self.state_w)
+        self.fine.refresh()
+        self.mg.fine = self.fine
+        mk = lambda dt: [torch.zeros(shape, dtype=dt, device=device)]
+        self.r, self.z, self.p, self.q, self.zo, self.x = (
+            mk(f64), mk(f64), mk(f64), mk(f64), mk(f64), mk(f64))
+        self.b_w, self.x_w = mk(wdtype), mk(wdtype)
+        self.sc = DeviceScalars(8, device)
+
+
+class PoissonBatchSolver:
+    """Normal-equation solve of every instance: `maxiter` flexible-CG
+    iterations on J^T J (FP64, matrix-free) preconditioned by one V(2,2)
+    cycle of the shared hierarchy of the instance's size (FP32 in 'mixed').
+    `solve()` replays one HIP graph holding the whole batch."""
+
+    def __init__(self, batch, precision='mixed', maxiter=10, omega=0.5,
+                 nlvl=None, ndirect=3, use_graph=True):
+        self.batch = batch
+        self.maxiter = int(maxiter)
+        self.omega = float(omega)
+        self.ndirect = ndirect
+        self.use_graph = use_graph
+        self.wdtype = torch.float32 if precision == 'mixed' else \
+            torch.float64
+        dev = batch.device
+        self.vec = VectorOps(dev)
+        self.classes = {}
+        for shp in sorted(set(batch.sizes)):
+            self.classes[shp] = _SizeClass(
+                shp, self.wdtype, nlvl, dev,
+                1e-12 if precision == 'mixed' else 1e-14)
+            self.classes[shp].mg.prepare(ndirect)
+        # FP64 solution and source per instance (the batch buffers may be
+        # FP32); the instance operator only supplies the residual.
+        self.u64 = [torch.zeros(s, dtype=torch.float64, device=dev)
+                    for s in batch.sizes]
+        self.ops = [Poisson(s, batch.view(batch.src, i).double())
+                    for i, s in enumerate(batch.sizes)]
+        self._graph = None
+
+    def _precond(self, k):
+        same = self.wdtype == torch.float64
+        rw = k.r if same else k.b_w
+        zw = k.z if same else k.x_w
+        if not same:
+            rw[0].copy_(k.r[0])
+        zw[0].zero_()
+        k.mg.vcycle(zw, rw, omega=self.omega, ndirect=self.ndirect)
+        if not same:
+            k.z[0].copy_(zw[0])
+
+    def _solve_instance(self, i):
+        k = self.classes[self.batch.sizes[i]]
+        op, u = self.ops[i], self.u64[i]
+        v, s = self.vec, k.sc
+        op.jtf([u], k.r, -1.0)             # r = -J^T f(u)
+        k.x[0].zero_()
+        self._precond(k)
+        k.p[0].copy_(k.z[0])
+        v.dot(k.r, k.z, s.p(0))
+        for _ in range(self.maxiter):
+            k.op.normal_apply(k.state64, k.p, k.q, mode=0)
+            v.dot(k.p, k.q, s.p(2))
+            v.axpy(k.x, k.p, s.p(0), s.p(2), 1.)
+            v.axpy(k.r, k.q, s.p(0), s.p(2), -1.)
+            k.zo[0].copy_(k.z[0])
+            self._precond(k)
+            v.dot(k.r, k.z, s.p(0), keep_prev=True)
+            v.dot(k.r, k.zo, s.p(3))
+            v.xpay(k.p, k.z, s.p(0), s.p(1), sub=s.p(3))
+        u.add_(k.x[0])
+
+    def _body(self):
+        for i in range(self.batch.count):
+            self._solve_instance(i)
+
+    def solve(self):
+        """One solve of every instance (u += correction)."""
+        if not self.use_graph:
+            self._body()
+            return
+        if self._graph is None:
+            self._body()
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._body()
+            self._graph = g
+            return
+        self._graph.replay()
+
+    def reset(self):
+        for u in self.u64:
+            u.zero_()
+
+    def residual_rms(self):
+        """Per-instance RMS residual of the current solutions."""
+        out = []
+        for op, u in zip(self.ops, self.u64):
+            _, ssq = op.residual([u], want_fields=False)
+            out.append((ssq[0].item() / op.ncell)**0.5)
+        return out

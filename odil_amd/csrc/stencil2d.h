@@ -171,40 +171,35 @@ __device__ inline void eval_global(const FieldPtrs<T, Op::NF>& in, int gi,
   }
 }
 
-// Fused loss + adjoint. One workgroup owns a TX x TY tile of output cells.
+// Fused loss + adjoint of one TX x TY tile of output cells at (x0, y0),
+// computed by the whole 256-thread workgroup:
 //  1. evaluate f and J on the tile grown by RJ (RJ = Jacobian reach),
 //     reading the state from global memory; every off-centre product
-//     J[e][k][s] f_e goes to LDS, the centre product of inner cells stays in
-//     registers;
+//     J[e][k][s] f_e goes to LDS `ot`, the centre product of inner cells
+//     stays in registers;
 //  2. gather grad_k(c) = scale * sum_s out_{k,s}(c - shift_s).
+// Adds this thread's f_e^2 to ssq[e]. Starts with a barrier so a workgroup
+// may run it on several tiles in turn.
 template <class T, class Op, int TX, int TY>
-__global__ void __launch_bounds__(256)
-    k_loss_grad(FieldPtrs<T, Op::NF> in, OutPtrs<T, Op::NF> grad,
-                double* __restrict__ partial, int n0, int n1, T scale,
-                typename Op::Params P) {
+__device__ inline void loss_grad_tile(const FieldPtrs<T, Op::NF>& in,
+                                      const OutPtrs<T, Op::NF>& grad, int n0,
+                                      int n1, int x0, int y0, T scale,
+                                      const typename Op::Params& P, T* ot,
+                                      double* ssq) {
   constexpr int NF = Op::NF, NE = Op::NE, NS = Op::NS, RJ = Op::RJ;
   constexpr int E0 = TX + 2 * RJ, E1 = TY + 2 * RJ;
   constexpr int NT = 256;
   constexpr int NPT = (TX * TY) / NT;  // inner cells per thread
   static_assert((TX * TY) % NT == 0, "tile must be a multiple of 256");
-  extern __shared__ unsigned char smem_raw[];
-  T* ot = reinterpret_cast<T*>(smem_raw);  // [NF][NS-1][E0][E1]
-  __shared__ double red[4];
-
-  unsigned bx, by;
-  xcd_block2(bx, by);
-  const int x0 = by * TX;
-  const int y0 = bx * TY;
   const int tid = threadIdx.x;
   const bool inner = interior_box(x0 - RJ, x0 + TX + RJ, y0 - RJ,
                                   y0 + TY + RJ, n0, n1, Op::R);
   auto OT = [&](int k, int s, int ea, int eb) -> T& {
     return ot[((k * (NS - 1) + s - 1) * E0 + ea) * E1 + eb];
   };
+  __syncthreads();  // previous tile's gathers are done with `ot`
 
   T cen[NPT][NF];
-  double ssq[NE];
-  for (int e = 0; e < NE; ++e) ssq[e] = 0;
   for (int p = 0; p < NPT; ++p) {
     const int q = tid + p * NT;
     const int a = q / TY, b = q - a * TY;
@@ -273,12 +268,31 @@ __global__ void __launch_bounds__(256)
       if (grad.f[k]) grad.f[k][c] = g * scale;
     }
   }
+}
+
+// Fused loss + adjoint over the grid: one workgroup per TX x TY tile
+// (XCD-aware order), per-block partial sums of f_e^2 in `partial`.
+template <class T, class Op, int TX, int TY>
+__global__ void __launch_bounds__(256)
+    k_loss_grad(FieldPtrs<T, Op::NF> in, OutPtrs<T, Op::NF> grad,
+                double* __restrict__ partial, int n0, int n1, T scale,
+                typename Op::Params P) {
+  constexpr int NE = Op::NE;
+  extern __shared__ unsigned char smem_raw[];
+  T* ot = reinterpret_cast<T*>(smem_raw);  // [NF][NS-1][E0][E1]
+  __shared__ double red[4];
+  unsigned bx, by;
+  xcd_block2(bx, by);
+  double ssq[NE];
+  for (int e = 0; e < NE; ++e) ssq[e] = 0;
+  loss_grad_tile<T, Op, TX, TY>(in, grad, n0, n1, by * TX, bx * TY, scale, P,
+                                ot, ssq);
   if (partial) {
     const int bidx = by * gridDim.x + bx;
     const int nb = gridDim.x * gridDim.y;
     for (int e = 0; e < NE; ++e) {
       double s = block_sum(ssq[e], red);
-      if (tid == 0) partial[(size_t)e * nb + bidx] = s;
+      if (threadIdx.x == 0) partial[(size_t)e * nb + bidx] = s;
     }
   }
 }
