@@ -9,9 +9,11 @@
   J^T J u = J^T src (one Newton step of the linear problem, the
   `optimize_multigridop_base` step of util.py:509-604) by flexible CG with
   a GMG V-cycle preconditioner. The Poisson Jacobian does not depend on u or
-  src, so all instances of one size share one multigrid hierarchy; the
-  whole batch (every instance, every iteration, ~10^4 kernels) is captured
-  once into a single HIP graph and replayed.
+  src, so all instances of one size share one multigrid hierarchy. The
+  batch solve (every instance, every iteration: ~700 kernels per instance)
+  is captured into HIP graphs of at most `graph_instances` instances each,
+  so graph size stays bounded as the batch grows, and the graphs replay
+  back to back with no host work in between.
 
 Node-based multigrid needs 2^k + 1 points per axis (multigrid.py:362-370),
 so `batch_sizes` draws sizes log-uniformly between 64 and 1024 and snaps
@@ -194,11 +196,13 @@ class PoissonBatchSolver:
     """Normal-equation solve of every instance: `maxiter` flexible-CG
     iterations on J^T J (FP64, matrix-free) preconditioned by one V(2,2)
     cycle of the shared hierarchy of the instance's size (FP32 in 'mixed').
-    `solve()` replays one HIP graph holding the whole batch."""
+    `solve()` replays the batch's HIP graphs (at most `graph_instances`
+    instances per graph)."""
 
     def __init__(self, batch, precision='mixed', maxiter=10, omega=0.5,
-                 nlvl=None, ndirect=3, use_graph=True):
+                 nlvl=None, ndirect=3, use_graph=True, graph_instances=8):
         self.batch = batch
+        self.graph_instances = max(1, int(graph_instances))
         self.maxiter = int(maxiter)
         self.omega = float(omega)
         self.ndirect = ndirect
@@ -219,7 +223,7 @@ class PoissonBatchSolver:
                     for s in batch.sizes]
         self.ops = [Poisson(s, batch.view(batch.src, i).double())
                     for i, s in enumerate(batch.sizes)]
-        self._graph = None
+        self._graphs = None
 
     def _precond(self, k):
         same = self.wdtype == torch.float64
@@ -253,24 +257,31 @@ class PoissonBatchSolver:
             v.xpay(k.p, k.z, s.p(0), s.p(1), sub=s.p(3))
         u.add_(k.x[0])
 
-    def _body(self):
-        for i in range(self.batch.count):
-            self._solve_instance(i)
+    def _chunks(self):
+        n, k = self.batch.count, self.graph_instances
+        return [range(a, min(a + k, n)) for a in range(0, n, k)]
 
     def solve(self):
         """One solve of every instance (u += correction)."""
         if not self.use_graph:
-            self._body()
+            for i in range(self.batch.count):
+                self._solve_instance(i)
             return
-        if self._graph is None:
-            self._body()
-            torch.cuda.synchronize()
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                self._body()
-            self._graph = g
+        if self._graphs is None:
+            graphs = []
+            for chunk in self._chunks():
+                for i in chunk:          # eager run (the first solve)
+                    self._solve_instance(i)
+                torch.cuda.synchronize()
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):  # recorded, not executed
+                    for i in chunk:
+                        self._solve_instance(i)
+                graphs.append(g)
+            self._graphs = graphs
             return
-        self._graph.replay()
+        for g in self._graphs:
+            g.replay()
 
     def reset(self):
         for u in self.u64:

@@ -1,0 +1,273 @@
+"""Measurements of the configurations beside the cavity headline:
+`python -m odil_amd.cases W A I P` prints one JSON line per case.
+
+W  wave 64 x 64 (FP64, seeded 3-Gaussian initial state), Newton with the
+   GPU direct normal-equation solve to RMS residual 1e-10, seeds 0-9.
+A  advection 128^3 (FP32): fused residual + adjoint throughput, and device
+   L-BFGS (m = 10) on the discrete loss.
+I  inverse wave 512 x 512, 64 sensors: loss + u/c adjoint throughput,
+   Adam (lr 1e-3) then device L-BFGS, in FP32 and FP64.
+P  256 Poisson instances of uneven size (65^2 .. 1025^2, FP32): one
+   persistent residual + adjoint launch (autotuned tiles) against a
+   per-instance launch loop, and the batched FCG + V-cycle solve replayed
+   from HIP graphs against eager per-instance solves.
+All inputs are synthetic and seeded; timings use HIP events on the launch
+stream.
+"""
+
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def _event_ms(fn, reps, graph=True):
+    for _ in range(2):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    if graph:
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(reps):
+                fn()
+        g.replay()
+        torch.cuda.synchronize()
+        s = torch.cuda.current_stream()
+        e0.record(s)
+        g.replay()
+        e1.record(s)
+    else:
+        e0.record(s)
+        for _ in range(reps):
+            fn()
+        e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _wave_gaussian(n, seed):
+    """3-Gaussian initial state, u_t = 0, walls 0 (the W configuration)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-0.5, 0.5, 3)
+    w = rng.uniform(0.05, 0.15, 3)
+    a = rng.uniform(0.5, 1.0, 3)
+    x = -1 + 2. / n * (np.arange(n) + 0.5)
+    u0 = sum(a[k] * np.exp(-(x - c[k])**2 / (2 * w[k]**2)) for k in range(3))
+    return u0, np.zeros(n), np.zeros(n), np.zeros(n)
+
+
+def case_w():
+    from odil_amd.drivers import optimize_newton
+    from odil_amd.operators import Wave
+    from odil_amd.problem import Domain, Problem, State
+    n = 64
+    rates, steps, rms = [], [], []
+    for seed in range(10):
+        dom = Domain(shape=(n, n), lower=(0, -1), upper=(1, 1),
+                     varnames=('t', 'x'), fieldnames=['u'])
+        pb = Problem(Wave((n, n), *_wave_gaussian(n, seed)), dom)
+        st = State()
+        optimize_newton(pb, st, 1)  # warm-up (code objects, allocator)
+        st = State()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hist = optimize_newton(pb, st, 10, tol=1e-10)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        k = hist[-1][0]
+        steps.append(k)
+        rates.append(k / dt if k else float('nan'))
+        rms.append(max(hist[-1][1]))
+    return {'case': 'W', 'workload': 'wave 64x64 FP64, Newton + GPU direct '
+            'J^T J solve to RMS residual 1e-10, seeds 0-9',
+            'newton_steps_per_s': float(np.mean(rates)),
+            'newton_steps_to_tol': steps, 'final_rms_max': max(rms)}
+
+
+def case_a():
+    from odil_amd.drivers import optimize_opt
+    from odil_amd.operators import Advection
+    from odil_amd.problem import Domain, Problem, State
+    shape = (128, 128, 128)
+    rng = np.random.default_rng(0)
+    ax, ay = rng.uniform(-1, 1, 2)
+    n = shape[1]
+    x = (np.arange(n) + 0.5) / n
+    xx, yy = np.meshgrid(x, x, indexing='ij')
+    cx, cy = rng.uniform(0.3, 0.7, 2)
+    p0 = np.exp(-((xx - cx)**2 + (yy - cy)**2) / (2 * 0.1**2))
+    out = {'case': 'A', 'workload': 'advection 128^3, periodic x/y, '
+           'a = ({:.3f}, {:.3f})'.format(ax, ay)}
+    N = int(np.prod(shape))
+    for dt, name in ((torch.float32, 'fp32'), (torch.float64, 'fp64')):
+        op = Advection(shape, ax, ay, torch.tensor(p0, dtype=dt,
+                                                   device='cuda'))
+        p = torch.tensor(rng.normal(size=shape), dtype=dt, device='cuda')
+        g = torch.empty_like(p)
+        ms = _event_ms(lambda: op.loss_grad([p], [g]), 20, graph=False)
+        B = p.element_size()
+        out['loss_grad_{}_ms'.format(name)] = ms
+        out['loss_grad_{}_GBps'.format(name)] = 2 * N * B / (ms * 1e6)
+    op = Advection(shape, ax, ay, torch.tensor(p0, dtype=torch.float32,
+                                                device='cuda'))
+    dom = Domain(shape=shape, varnames=('t', 'x', 'y'), fieldnames=['p'],
+                 dtype=torch.float32)
+    pb = Problem(op, dom)
+    st = State()
+    l0 = pb.eval_loss_grad(st)[0].item()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, info = optimize_opt(pb, st, 'lbfgs', 200, m=10)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    out.update({'lbfgs_iters': info.epochs, 'lbfgs_evals': info.evals,
+                'lbfgs_iters_per_s': info.epochs / (t1 - t0),
+                'loss0': l0, 'loss_after': pb.eval_loss_grad(st)[0].item()})
+    return out
+
+
+def _inverse_case(nt, nx, ns, seed):
+    """Seeded inverse-wave case (same construction as the test oracle's,
+    restated here so the package does not depend on test code)."""
+    rng = np.random.default_rng(seed)
+    dt, dx = 1. / nt, 2. / nx
+    x1 = -1 + dx * (np.arange(nx) + 0.5)
+    cc = rng.uniform(-0.5, 0.5, 3)
+    ww = rng.uniform(0.05, 0.15, 3)
+    aa = rng.uniform(0.5, 1.0, 3)
+    init_u = sum(aa[k] * np.exp(-(x1 - cc[k])**2 / (2 * ww[k]**2))
+                 for k in range(3))
+    s = rng.uniform(-0.25, 0.25, 4)
+    xk = rng.uniform(-0.8, 0.8, 4)
+    c_true = np.clip(1 + sum(s[k] * np.tanh((x1 - xk[k]) / 0.05)
+                             for k in range(4)), 0.5, 1.5)
+    sensors = np.unique(np.round(np.linspace(0, nx - 1, ns)).astype(int))
+    return init_u, c_true, sensors
+
+
+def case_i(adam_steps=5000, lbfgs_iters=1000):
+    from odil_amd.operators import WaveInverse
+    from odil_amd.optimizer import make_optimizer
+    nt = nx = 512
+    init_u, c_true, sensors = _inverse_case(nt, nx, 64, 0)
+    z = np.zeros(nx)
+    zt = np.zeros(nt)
+    # synthetic data: the FP64 forward march with the true speed
+    op64 = WaveInverse((nt, nx), init_u, z, zt, zt, sensors,
+                       np.zeros((nt, len(sensors))), device='cuda')
+    u_true = op64.march(torch.tensor(c_true, device='cuda'))
+    data = u_true[:, torch.tensor(sensors, device='cuda')].cpu().numpy()
+    out = {'case': 'I', 'workload': 'inverse wave 512x512, 64 sensors, '
+           'k_data 100, Tikhonov 1e-4 on dc/dx; Adam lr 1e-3 x {} then '
+           'L-BFGS(10) x {}'.format(adam_steps, lbfgs_iters)}
+    res = {}
+    for dt, name in ((torch.float32, 'fp32'), (torch.float64, 'fp64')):
+        op = WaveInverse((nt, nx), init_u, z, zt, zt, sensors, data,
+                         k_data=100.0, k_reg=1e-2, device='cuda')
+        u = torch.zeros((nt, nx), dtype=dt, device='cuda')
+        c = torch.ones(nx, dtype=dt, device='cuda')
+        gu, gc = torch.empty_like(u), torch.empty_like(c)
+        ms = _event_ms(lambda: op.loss_grad(u, c, gu, gc), 20, graph=False)
+        out['loss_grad_{}_ms'.format(name)] = ms
+        out['loss_grad_{}_evals_per_s'.format(name)] = 1e3 / ms
+        x = op.pack(u, c)
+        lg = op.make_loss_grad()
+        l0 = lg(x, 0)[0].item()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x, _ = make_optimizer('adamn').run(x, lg, epochs=adam_steps, lr=1e-3)
+        la = lg(x, 0)[0].item()
+        x, info = make_optimizer('lbfgs', m=10).run(x, lg,
+                                                    epochs=lbfgs_iters)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _, cc = op.unpack(x)
+        cc = cc.double().cpu().numpy()
+        res[name] = cc
+        out[name] = {'seconds': t1 - t0, 'loss0': l0, 'loss_after_adam': la,
+                     'loss_final': lg(x, 0)[0].item(),
+                     'c_rms_error': float(np.sqrt(np.mean((cc - c_true)**2))),
+                     'c_rms_error_initial':
+                         float(np.sqrt(np.mean((1 - c_true)**2)))}
+    out['c_fp32_vs_fp64_max'] = float(np.abs(res['fp32'] - res['fp64']).max())
+    return out
+
+
+def _progress(msg):
+    print('# ' + msg, file=sys.stderr, flush=True)
+
+
+def case_p(count=256, seed=0, solve=True):
+    from odil_amd.batched import (PoissonBatch, PoissonBatchSolver,
+                                  batch_sizes, gaussian_mixture)
+    sizes = batch_sizes(count, seed)
+    srcs = [gaussian_mixture(n, seed=1000 + i) for i, n in enumerate(sizes)]
+    _progress('P: building batch of {} instances'.format(count))
+    b = PoissonBatch(sizes, srcs, dtype=torch.float32)
+    out = {'case': 'P', 'workload': '{} Poisson instances, sizes {}, FP32'
+           .format(count, {n: sizes.count(n) for n in sorted(set(sizes))}),
+           'cells': b.total}
+    _progress('P: autotuning the batched launch')
+    shape = b.tune()
+    ops = b.instance_ops()
+    _progress('P: timing the batched launch, tile {}'.format(shape))
+    ms_b = _event_ms(lambda: b.launch(shape), 10, graph=True)
+    _progress('P: timing the per-instance loop')
+    ms_l = _event_ms(lambda: b.loss_grad_loop(ops), 10, graph=False)
+    nb = b.algorithmic_bytes()
+    out.update({'tile': list(shape), 'batched_ms': ms_b,
+                'batched_instances_per_s': count / (ms_b * 1e-3),
+                'batched_GBps': nb / (ms_b * 1e6),
+                'batched_frac_hbm': nb / (ms_b * 1e6) / HBM_PEAK_GBPS,
+                'loop_ms': ms_l, 'loop_instances_per_s': count / (ms_l * 1e-3),
+                'loop_GBps': nb / (ms_l * 1e6)})
+    if solve:
+        for use_graph in (False, True):
+            _progress('P: solver setup, graphs={}'.format(use_graph))
+            t0 = time.perf_counter()
+            sol = PoissonBatchSolver(b, precision='mixed', maxiter=10,
+                                     use_graph=use_graph)
+            _progress('P: first solve (graphs={}: eager + capture)'.format(
+                use_graph))
+            sol.solve()           # graph: eager run + capture
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            _progress('P: timed solve')
+            sol.reset()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            sol.solve()
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            key = 'solve_graph' if use_graph else 'solve_eager'
+            out[key + '_setup_s'] = t1 - t0
+            out[key + '_s'] = t3 - t2
+            out[key + '_instances_per_s'] = count / (t3 - t2)
+            if use_graph:
+                rms = sol.residual_rms()
+                out['solve_rms_max_rel'] = float(max(
+                    r / np.sqrt(np.mean(s**2)) for r, s in zip(rms, srcs)))
+            del sol
+            torch.cuda.empty_cache()
+    return out
+
+
+def main(argv):
+    names = [a.upper() for a in argv] or ['W', 'A', 'I', 'P']
+    fns = {'W': case_w, 'A': case_a, 'I': case_i, 'P': case_p}
+    for n in names:
+        t0 = time.perf_counter()
+        r = fns[n]()
+        r['wall_s'] = time.perf_counter() - t0
+        print(json.dumps(r), flush=True)
+
+
+if __name__ == '__main__':
+    main(sys.argv[1:])
