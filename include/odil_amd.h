@@ -290,15 +290,18 @@ int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
  * (q, var, cs_lin, xs_lin, cs0..2, xs0..2), `ostart` [nvar + 1], `reach`
  * host int[3] bounds the offsets per axis, max_items <= 512, nent the
  * coefficient array count, lanes the threads per cell (1, 2, 4, 8, 16; 0
- * picks by grid size; an autotuner chooses it per operator shape). Fields
- * of x in one allocation (x_k = x_0 + k N) take a faster path. No reference
+ * picks by grid size; -1: one thread per 16 bytes of consecutive cells with
+ * vector loads, x in one allocation only), per the 256-thread tiles one
+ * workgroup sweeps (1..64; the autotuner chooses lanes and per for each
+ * operator shape). Fields of x in one allocation (x_k = x_0 + k N) take
+ * faster paths. No reference
  * counterpart for the storage: it
  * applies the same matrices as the reference's full stencils
  * (mul_transpose_op products and Galerkin levels are symmetric). */
 int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
                            const int* items, const int* ostart,
                            const int* dent, const int* reach, int max_items,
-                           int nent, int lanes, const void* coef,
+                           int nent, int lanes, int per, const void* coef,
                            const void* const* x, void* const* y,
                            const void* const* f, int mode, double omega,
                            void* stream);

@@ -20,7 +20,21 @@ struct CavityParams {
   double dx, dy, mu;
   int bc;        // 0: no-slip + lid, 1: no-penetration, 2: extrapolation
   int rhiechow;  // 1: Rhie-Chow continuity, 0: central divergence
+  // Grid constants computed once on the host (FP64): 1/dx, 1/dy,
+  // 1/(2dx), 1/(2dy), mu (2/dx^2 + 2/dy^2), mu/dx^2, mu/dy^2.
+  double idx, idy, h2x, h2y, gdiag, lxx, lyy;
 };
+
+inline CavityParams cavity_params(double dx, double dy, double mu, int bc,
+                                  int rhiechow) {
+  return CavityParams{dx,           dy,
+                      mu,           bc,
+                      rhiechow,     1.0 / dx,
+                      1.0 / dy,     1.0 / (2.0 * dx),
+                      1.0 / (2.0 * dy),
+                      mu * (2.0 / (dx * dx) + 2.0 / (dy * dy)),
+                      mu / (dx * dx), mu / (dy * dy)};
+}
 
 // Halo rules per field and face (ns_steady.py:219-257).
 struct FaceRule {
@@ -132,12 +146,16 @@ struct CavityOp {
                                  0, 1, 2, 3, 4, 0, 3, 4,        //
                                  0, 1, 2, 0, 3, 4, 0, 1, 2, 3, 4};
 
-  // 1/g with a hardware reciprocal in FP32 (1 ulp), exact in FP64.
+  // 1/g with the hardware reciprocal: FP32 as is (1 ulp); FP64 refined by
+  // two Newton-Raphson steps y <- y + y (1 - g y) (to ~1 ulp).
   __device__ static inline T recip(T g) {
-    if constexpr (sizeof(T) == 4)
+    if constexpr (sizeof(T) == 4) {
       return __builtin_amdgcn_rcpf(g);
-    else
-      return T(1) / g;
+    } else {
+      T y = __builtin_amdgcn_rcp(g);
+      y = fma(y, fma(-g, y, T(1)), y);
+      return fma(y, fma(-g, y, T(1)), y);
+    }
   }
 
   // INT: the cell and its distance-2 neighbours are inside the domain and
@@ -146,7 +164,7 @@ struct CavityOp {
   __device__ static inline void compute(const Acc& q, int i, int j, int nx,
                                         int ny, const Params& P, T* f,
                                         T (*J)[NF][NS], bool want_jac) {
-    const T dx = T(P.dx), dy = T(P.dy), mu = T(P.mu);
+    const T mu = T(P.mu);
     const FaceRule ru = cavity_rule(P.bc, 0), rv = cavity_rule(P.bc, 1);
     const FaceRule rp{LINEAR, LINEAR, 0, 0, 0, 0};
     const Nbr<T> U = neighbours<INT, T>(q, 0, i, j, nx, ny, ru);
@@ -155,8 +173,8 @@ struct CavityOp {
     const bool iw0 = !INT && i == 0, iw1 = !INT && i == nx - 1;
     const bool jw0 = !INT && j == 0, jw1 = !INT && j == ny - 1;
     const T a = U.c, b = V.c;  // frozen velocities at the cell
-    const T idx = T(1) / dx, idy = T(1) / dy;
-    const T h2x = T(1) / (T(2) * dx), h2y = T(1) / (T(2) * dy);
+    const T idx = T(P.idx), idy = T(P.idy);
+    const T h2x = T(P.h2x), h2y = T(P.h2y);
 
     // Second-order upwind of the frozen field, first order at walls.
     auto hi2x = [&](const Nbr<T>& s) {
@@ -184,7 +202,7 @@ struct CavityOp {
       f[0] = a * hi2x(U) + b * hi2y(U) - mu * lap(U) + (Pn.xp - Pn.xm) * h2x;
       f[1] = a * hi2x(V) + b * hi2y(V) - mu * lap(V) + (Pn.yp - Pn.ym) * h2y;
     }
-    const T gdiag = mu * (T(2) / (dx * dx) + T(2) / (dy * dy));
+    const T gdiag = T(P.gdiag);
     // Rhie-Chow coefficient g from frozen velocities (roll: periodic index,
     // only used where the face flux is kept).
     auto gat = [&](int di, int dj) {
@@ -238,7 +256,7 @@ struct CavityOp {
 
     // Live Jacobian on effective neighbours, then folded onto raw values.
     T e[5];
-    const T lxx = mu / (dx * dx), lyy = mu / (dy * dy);
+    const T lxx = T(P.lxx), lyy = T(P.lyy);
     // fu <- u and fv <- v share the operator uf d/dx + vf d/dy - mu lap.
     e[0] = T(2) * lxx + T(2) * lyy;
     e[1] = -lxx;
@@ -355,7 +373,7 @@ int odil_cavity_residual(int dtype, int nx, int ny, double dx, double dy,
     set_error("cavity_residual: need nx, ny >= 3 and fields");
     return kErrArg;
   }
-  CavityParams P{dx, dy, mu, bc, rhiechow};
+  CavityParams P = cavity_params(dx, dy, mu, bc, rhiechow);
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (dtype == kF64)
     return launch_residual<double, CavityOp<double>>(fields, res, partial, nx,
@@ -376,7 +394,7 @@ int odil_cavity_jac(int dtype, int nx, int ny, double dx, double dy,
     set_error("cavity_jac: need nx, ny >= 3, fields and coef");
     return kErrArg;
   }
-  CavityParams P{dx, dy, mu, bc, rhiechow};
+  CavityParams P = cavity_params(dx, dy, mu, bc, rhiechow);
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (dtype == kF64)
     return launch_jac<double, CavityOp<double>>(fields, coef, res, nx, ny, P,
@@ -412,7 +430,7 @@ int odil_cavity_loss_grad(int dtype, int nx, int ny, double dx, double dy,
     set_error("cavity_loss_grad: need nx, ny >= 3, fields and grad");
     return kErrArg;
   }
-  CavityParams P{dx, dy, mu, bc, rhiechow};
+  CavityParams P = cavity_params(dx, dy, mu, bc, rhiechow);
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (dtype == kF64)
     return cavity_dispatch_grad<double>(fields, grad, partial, nx, ny, P, s,
@@ -445,7 +463,7 @@ int odil_cavity_normal_apply(int dtype, int nx, int ny, double dx, double dy,
     set_error("cavity_normal_apply: need nx, ny >= 3");
     return kErrArg;
   }
-  CavityParams P{dx, dy, mu, bc, rhiechow};
+  CavityParams P = cavity_params(dx, dy, mu, bc, rhiechow);
   return dispatch_normal_apply<CavityOp<double>, CavityOp<float>>(
       dtype, state, x, b, d, y, nx, ny, omega, mode, P,
       static_cast<hipStream_t>(stream));
@@ -461,7 +479,7 @@ int odil_cavity_jtf(int dtype, int nx, int ny, double dx, double dy,
     set_error("cavity_jtf: bad arguments");
     return kErrArg;
   }
-  CavityParams P{dx, dy, mu, bc, rhiechow};
+  CavityParams P = cavity_params(dx, dy, mu, bc, rhiechow);
   return dispatch_jtf<CavityOp<double>, CavityOp<float>>(
       dtype, fields, out, partial, nx, ny, scale, P,
       static_cast<hipStream_t>(stream), nblocks);

@@ -37,6 +37,25 @@ struct GlobalAcc {
   }
 };
 
+// Unclamped accessor of cells whose stencil stays inside the grid: one
+// base pointer per field at the cell, neighbours at 32-bit offsets
+// di * n1 + dj (same-row neighbours fold into the load's immediate).
+template <class T, int NF>
+struct GlobalAcc<T, NF, false> {
+  const T* b[NF];
+  int n1;
+  __device__ inline GlobalAcc(const T* const* f, int n0_, int n1_, int i,
+                              int j)
+      : n1(n1_) {
+    const size_t c = (size_t)i * n1_ + j;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) b[k] = f[k] + c;
+  }
+  __device__ inline T operator()(int k, int di, int dj) const {
+    return b[k][di * n1 + dj];
+  }
+};
+
 // True when every cell of rows [a0, a1) x cols [b0, b1) is at least R cells
 // from the domain edge, so operators may take their interior fast path
 // (no wall rules, no clamping).

@@ -142,9 +142,10 @@ __global__ void __launch_bounds__(256)
     k_apply_sym(const int* __restrict__ items, const int* __restrict__ ostart,
                 const int* __restrict__ dent, const T* __restrict__ coef,
                 CVecPtrs<T> x, VecPtrs<T> y, CVecPtrs<T> f, Grid3 nw,
-                Grid3 reach, int mode, T omega) {
+                Grid3 reach, int mode, T omega, int per) {
   __shared__ int2 fast[kMaxItems];
-  // Logical block: tile bx, output field o fastest (one XCD per tile).
+  // Logical block: a chunk of `per` consecutive tiles, output field o
+  // fastest (one XCD per chunk); the item list is staged once per chunk.
   const unsigned g = xcd_block(blockIdx.x + blockIdx.y * gridDim.x,
                                gridDim.x * gridDim.y);
   const int o = g % gridDim.y;
@@ -157,53 +158,161 @@ __global__ void __launch_bounds__(256)
     fast[t] = make_int2(it[0] * N + it[2], (CONTIG ? it[1] * N : 0) + it[3]);
   }
   __syncthreads();
-  const int c = (int)(((size_t)bx * blockDim.x + threadIdx.x) / LANES);
   const int lane = threadIdx.x % LANES;
-  if (c >= N) return;  // whole groups exit together
-  int i0, i1, i2;
-  unflat(c, nw, i0, i1, i2);
-  const bool inner = i0 >= reach.n0 && i0 < nw.n0 - reach.n0 &&
-                     i1 >= reach.n1 && i1 < nw.n1 - reach.n1 &&
-                     i2 >= reach.n2 && i2 < nw.n2 - reach.n2;
-  T acc = 0;
-  if (inner) {
-    if (CONTIG) {
-      const T* __restrict__ xb = x.p[0];
+  for (int r = 0; r < per; ++r) {
+    const int c = (int)((((size_t)bx * per + r) * blockDim.x + threadIdx.x) /
+                        LANES);
+    if (c >= N) return;  // whole groups exit together; later tiles too
+    int i0, i1, i2;
+    unflat(c, nw, i0, i1, i2);
+    const bool inner = i0 >= reach.n0 && i0 < nw.n0 - reach.n0 &&
+                       i1 >= reach.n1 && i1 < nw.n1 - reach.n1 &&
+                       i2 >= reach.n2 && i2 < nw.n2 - reach.n2;
+    T acc = 0;
+    if (inner) {
+      if (CONTIG) {
+        const T* __restrict__ xb = x.p[0];
 #pragma unroll 16
-      for (int e = lane; e < ne; e += LANES) {
-        const int2 it = fast[e];
-        acc += coef[(unsigned)(c + it.x)] * xb[(unsigned)(c + it.y)];
+        for (int e = lane; e < ne; e += LANES) {
+          const int2 it = fast[e];
+          acc += coef[(unsigned)(c + it.x)] * xb[(unsigned)(c + it.y)];
+        }
+      } else {
+#pragma unroll 4
+        for (int e = lane; e < ne; e += LANES) {
+          const int2 it = fast[e];
+          acc += coef[(unsigned)(c + it.x)] *
+                 pick(x, items[10 * (i0e + e) + 1])[(unsigned)(c + it.y)];
+        }
       }
     } else {
-#pragma unroll 4
+      // Cells near a face (few): full shifts from the global table.
       for (int e = lane; e < ne; e += LANES) {
-        const int2 it = fast[e];
-        acc += coef[(unsigned)(c + it.x)] *
-               pick(x, items[10 * (i0e + e) + 1])[(unsigned)(c + it.y)];
+        const int* fl = items + 10 * (i0e + e);
+        const int a0 = i0 + fl[4], a1 = i1 + fl[5], a2 = i2 + fl[6];
+        const int b0 = i0 + fl[7], b1 = i1 + fl[8], b2 = i2 + fl[9];
+        if (!inside(a0, a1, a2, nw) || !inside(b0, b1, b2, nw)) continue;
+        acc +=
+            coef[(size_t)fl[0] * N + ((size_t)a0 * nw.n1 + a1) * nw.n2 + a2] *
+            pick(x, fl[1])[((size_t)b0 * nw.n1 + b1) * nw.n2 + b2];
       }
     }
-  } else {
-    // Cells near a face (few): full shifts from the global table.
-    for (int e = lane; e < ne; e += LANES) {
-      const int* fl = items + 10 * (i0e + e);
-      const int a0 = i0 + fl[4], a1 = i1 + fl[5], a2 = i2 + fl[6];
-      const int b0 = i0 + fl[7], b1 = i1 + fl[8], b2 = i2 + fl[9];
-      if (!inside(a0, a1, a2, nw) || !inside(b0, b1, b2, nw)) continue;
-      acc += coef[(size_t)fl[0] * N + ((size_t)a0 * nw.n1 + a1) * nw.n2 + a2] *
-             pick(x, fl[1])[((size_t)b0 * nw.n1 + b1) * nw.n2 + b2];
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1)
+      acc += __shfl_xor(acc, off, LANES);
+    if (lane != 0) continue;
+    if (mode == 0) {
+      y.p[o][c] = acc;
+    } else if (mode == 1) {
+      y.p[o][c] = f.p[o][c] - acc;
+    } else {
+      const T d = coef[(size_t)dent[o] * N + c];
+      y.p[o][c] = x.p[o][c] + omega * (f.p[o][c] - acc) / d;
     }
   }
-#pragma unroll
-  for (int off = LANES / 2; off > 0; off >>= 1)
-    acc += __shfl_xor(acc, off, LANES);
-  if (lane != 0) return;
-  if (mode == 0) {
-    y.p[o][c] = acc;
-  } else if (mode == 1) {
-    y.p[o][c] = f.p[o][c] - acc;
-  } else {
-    const T d = coef[(size_t)dent[o] * N + c];
-    y.p[o][c] = x.p[o][c] + omega * (f.p[o][c] - acc) / d;
+}
+
+// Vector form of k_apply_sym for contiguous x (x_k = x_0 + k N): each
+// thread owns V = 16 / sizeof(T) consecutive nodes and reads every
+// coefficient and x span with one 16-byte load (unaligned), cutting the
+// memory instructions and address arithmetic by V. A span that is wholly
+// interior (all V nodes at least `reach` from every face, in one row) takes
+// the vector path; other spans take the per-node bounds-checked path.
+template <class VT, class T>
+__device__ inline VT vld(const T* p) {
+  VT v;
+  __builtin_memcpy(&v, p, sizeof(VT));
+  return v;
+}
+template <class VT, class T>
+__device__ inline void vst(T* p, const VT& v) {
+  __builtin_memcpy(p, &v, sizeof(VT));
+}
+
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_apply_sym_vec(const int* __restrict__ items,
+                    const int* __restrict__ ostart,
+                    const int* __restrict__ dent, const T* __restrict__ coef,
+                    CVecPtrs<T> x, VecPtrs<T> y, CVecPtrs<T> f, Grid3 nw,
+                    Grid3 reach, int mode, T omega) {
+  constexpr int V = 16 / sizeof(T);
+  typedef T VT __attribute__((ext_vector_type(V)));
+  __shared__ int2 fast[kMaxItems];
+  const unsigned g = xcd_block(blockIdx.x + blockIdx.y * gridDim.x,
+                               gridDim.x * gridDim.y);
+  const int o = g % gridDim.y;
+  const unsigned bx = g / gridDim.y;
+  const int i0e = ostart[o];
+  const int ne = ostart[o + 1] - i0e;
+  const int N = (int)nw.size();
+  for (int t = threadIdx.x; t < ne; t += blockDim.x) {
+    const int* it = items + 10 * (i0e + t);
+    fast[t] = make_int2(it[0] * N + it[2], it[1] * N + it[3]);
+  }
+  __syncthreads();
+  const int c0 = (int)(((size_t)bx * blockDim.x + threadIdx.x) * V);
+  if (c0 >= N) return;
+  const int nc = min(V, N - c0);
+  int a0, a1, a2, b0, b1, b2;
+  unflat(c0, nw, a0, a1, a2);
+  unflat(c0 + nc - 1, nw, b0, b1, b2);
+  // The span lies in one row of the contiguous (last non-trivial) axis and
+  // every node of it is at least `reach` from every face.
+  bool inner;
+  if (nw.n2 > 1)
+    inner = nc == V && a0 == b0 && a1 == b1 && a0 >= reach.n0 &&
+            a0 < nw.n0 - reach.n0 && a1 >= reach.n1 &&
+            a1 < nw.n1 - reach.n1 && a2 >= reach.n2 &&
+            b2 < nw.n2 - reach.n2;
+  else
+    inner = nc == V && a0 == b0 && a0 >= reach.n0 &&
+            a0 < nw.n0 - reach.n0 && a1 >= reach.n1 &&
+            b1 < nw.n1 - reach.n1;
+  const T* __restrict__ xb = x.p[0];
+  if (inner) {
+    VT acc = 0;
+#pragma unroll 8
+    for (int e = 0; e < ne; ++e) {
+      const int2 it = fast[e];
+      acc += vld<VT>(coef + (unsigned)(c0 + it.x)) *
+             vld<VT>(xb + (unsigned)(c0 + it.y));
+    }
+    VT out;
+    if (mode == 0) {
+      out = acc;
+    } else if (mode == 1) {
+      out = vld<VT>(f.p[o] + c0) - acc;
+    } else {
+      const VT d = vld<VT>(coef + (size_t)dent[o] * N + c0);
+      out = vld<VT>(xb + (size_t)o * N + c0) +
+            omega * (vld<VT>(f.p[o] + c0) - acc) / d;
+    }
+    vst(y.p[o] + c0, out);
+    return;
+  }
+  for (int v = 0; v < nc; ++v) {
+    const int c = c0 + v;
+    int i0, i1, i2;
+    unflat(c, nw, i0, i1, i2);
+    T acc = 0;
+    for (int e = 0; e < ne; ++e) {
+      const int* fl = items + 10 * (i0e + e);
+      const int p0 = i0 + fl[4], p1 = i1 + fl[5], p2 = i2 + fl[6];
+      const int q0 = i0 + fl[7], q1 = i1 + fl[8], q2 = i2 + fl[9];
+      if (!inside(p0, p1, p2, nw) || !inside(q0, q1, q2, nw)) continue;
+      acc += coef[(size_t)fl[0] * N + ((size_t)p0 * nw.n1 + p1) * nw.n2 + p2] *
+             xb[(size_t)fl[1] * N + ((size_t)q0 * nw.n1 + q1) * nw.n2 + q2];
+    }
+    T out;
+    if (mode == 0)
+      out = acc;
+    else if (mode == 1)
+      out = f.p[o][c] - acc;
+    else
+      out = xb[(size_t)o * N + c] +
+            omega * (f.p[o][c] - acc) / coef[(size_t)dent[o] * N + c];
+    y.p[o][c] = out;
   }
 }
 
@@ -631,12 +740,15 @@ int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
 // k_apply_sym); `reach` (host, 3 ints) bounds |cs|, |xs| per axis;
 // `max_items` is the longest per-field list (at most 512); `nent` the number
 // of coefficient arrays; `lanes` threads per cell (1, 2, 4, 8, 16, or 0 for
-// a size-based default). x fields laid out as one allocation (x_k = x_0 +
-// k N) take the fast path.
+// a size-based default) or -1: one thread per 16 bytes of consecutive cells
+// with vector loads (x in one allocation only); `per` tiles of 256 threads
+// per workgroup (1..64, scalar paths; the item list is staged once per
+// workgroup). x fields laid out as one allocation (x_k = x_0 + k N) take the
+// fast paths.
 int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
                            const int* items, const int* ostart,
                            const int* dent, const int* reach, int max_items,
-                           int nent, int lanes, const void* coef,
+                           int nent, int lanes, int per, const void* coef,
                            const void* const* x, void* const* y,
                            const void* const* f, int mode, double omega,
                            void* stream) {
@@ -649,6 +761,10 @@ int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
   }
   if ((mode == 1 || mode == 2) && !f) {
     set_error("stencil_apply_sym: mode %d needs f", mode);
+    return kErrArg;
+  }
+  if (per < 1 || per > 64) {
+    set_error("stencil_apply_sym: tiles per workgroup must be in [1, 64]");
     return kErrArg;
   }
   if (max_items < 0 || max_items > kMaxItems) {
@@ -679,20 +795,41 @@ int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
     lanes = 1;
     while (lanes < 16 && groups * lanes < (1u << 20)) lanes *= 2;
   }
-  if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16) {
-    set_error("stencil_apply_sym: lanes must be 0, 1, 2, 4, 8 or 16");
+  if (lanes != -1 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 &&
+      lanes != 16) {
+    set_error("stencil_apply_sym: lanes must be -1, 0, 1, 2, 4, 8 or 16");
     return kErrArg;
   }
   if (dtype != kF64 && dtype != kF32) {
     set_error("stencil_apply_sym: bad dtype %d", dtype);
     return kErrArg;
   }
-  dim3 grid(nblk(g.size() * lanes), nvar);
+  if (lanes == -1) {
+    if (!contig) {
+      set_error("stencil_apply_sym: the vector path needs x in one "
+                "allocation");
+      return kErrArg;
+    }
+    const size_t V = dtype == kF64 ? 2 : 4;
+    dim3 vgrid(nblk((g.size() + V - 1) / V), nvar);
+    if (dtype == kF64)
+      hipLaunchKernelGGL(k_apply_sym_vec<double>, vgrid, dim3(256), 0, s,
+                         items, ostart, dent, static_cast<const double*>(coef),
+                         cvp<double>(x, nvar), vp<double>(y, nvar),
+                         cvp<double>(f, nvar), g, rc, mode, omega);
+    else
+      hipLaunchKernelGGL(k_apply_sym_vec<float>, vgrid, dim3(256), 0, s,
+                         items, ostart, dent, static_cast<const float*>(coef),
+                         cvp<float>(x, nvar), vp<float>(y, nvar),
+                         cvp<float>(f, nvar), g, rc, mode, (float)omega);
+    return check_launch("stencil_apply_sym");
+  }
+  dim3 grid(ceil_div((int)nblk(g.size() * lanes), per), nvar);
 #define ODIL_APPLY2(T, L, C)                                                 \
   hipLaunchKernelGGL((k_apply_sym<T, L, C>), grid, dim3(256), 0, s, items,   \
                      ostart, dent, static_cast<const T*>(coef),              \
                      cvp<T>(x, nvar), vp<T>(y, nvar), cvp<T>(f, nvar), g,    \
-                     rc, mode, (T)omega)
+                     rc, mode, (T)omega, per)
 #define ODIL_APPLY(T, L)      \
   if (contig)                 \
     ODIL_APPLY2(T, L, true);  \

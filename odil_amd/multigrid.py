@@ -51,6 +51,7 @@ class BlockStencil:
                  nvar_out=None, sym=False):
         self.sym = bool(sym)
         self.lanes = 0  # threads per cell of the symmetric sweep, 0 = auto
+        self.per = 1    # 256-thread tiles per workgroup of that sweep
         self.nw = tuple(int(n) for n in nw)
         self.ndim = len(self.nw)
         self.stride = int(stride)
@@ -178,7 +179,7 @@ class BlockStencil:
                 _grid_arr(self.nw), c_int(self.nvar_out), ptr(self.t_items),
                 ptr(self.t_iostart), ptr(self.t_dent), self._reach,
                 c_int(self._max_items), c_int(len(self.entries)),
-                c_int(self.lanes),
+                c_int(self.lanes), c_int(self.per if self.lanes != -1 else 1),
                 ptr(self.coef), ptrs(x), ptrs(y),
                 ptrs(f) if f is not None else None, c_int(mode),
                 c_double(omega), stream())
@@ -229,10 +230,13 @@ class BlockStencil:
         key = ('apply_sym', self.nw, len(self.entries), self._max_items,
                str(self.dtype), contig)
 
-        def run(lanes):
-            self.lanes = lanes
+        def run(c):
+            self.lanes, self.per = c
             self._call_apply(x, y, f, 2, omega)
-        self.lanes = autotune.pick(key, [0, 1, 2, 4, 8, 16], run)
+        cand = [(L, p) for L in (0, 1, 2, 4, 8, 16) for p in (1, 2, 4, 8)]
+        if contig:
+            cand.append((-1, 1))
+        self.lanes, self.per = autotune.pick(key, cand, run)
         return self
 
     def apply_t(self, x, y=None, accumulate=False):
@@ -303,7 +307,7 @@ class BlockStencil:
         out = BlockStencil(self.coef.to(dtype), self.entries, self.nw,
                            self.stride, self.nvar_in, self.nvar_out,
                            sym=self.sym)
-        out.lanes = self.lanes
+        out.lanes, out.per = self.lanes, self.per
         return out
 
     def tocsr(self):

@@ -130,7 +130,8 @@ class NewtonMultigrid:
                  ndirect=3, maxiter=10, restriction='transpose',
                  galerkin_mode='exact', precision='fp64', linsolver='pcg',
                  rtol=0., check_every=10, coarse_reg='auto',
-                 matrix_free=True, use_graph=True, device=None):
+                 matrix_free=True, use_graph=True, setup_every=1,
+                 device=None):
         if precision not in ('fp64', 'mixed'):
             raise ValueError('precision must be fp64 or mixed')
         if linsolver not in ('vcycle', 'pcg'):
@@ -147,6 +148,12 @@ class NewtonMultigrid:
         self.pre, self.post, self.ndirect = smooth_pre, smooth_post, ndirect
         self.matrix_free = matrix_free
         self.use_graph = use_graph
+        # Rebuild the stored hierarchy (J, J^T J, Galerkin levels, coarsest
+        # solution operator) every `setup_every` Newton steps; the Krylov
+        # operator and the finest level are always those of the current
+        # state, so a lagged hierarchy only changes the preconditioner.
+        self.setup_every = max(1, int(setup_every))
+        self._nstep = 0
         # J^T J and its Galerkin products R N T with R = T^T / 2^dim are
         # symmetric: store upper halves.
         self.sym = galerkin_mode == 'exact' and restriction == 'transpose'
@@ -315,17 +322,20 @@ class NewtonMultigrid:
         if tol is not None and max(rms) <= tol:
             self.last_linear = 0
             return rms
-        jac, _ = self.op.jacobian(fields)
-        J = BlockStencil.from_jacobian(jac)
-        if self.N is None:
-            self.N = normal_operator(J, self.wdtype, sym=self.sym)
-        else:
-            normal_operator(J, out=self.N)
-        if not self.matrix_free:
-            self._N64 = self.N if self.wdtype == torch.float64 else \
-                normal_operator(J, torch.float64, sym=self.sym)
-        self.mg.update_A(self.N)
-        self.mg.prepare(self.ndirect)
+        if (self._nstep % self.setup_every == 0 or not self.mg.AA
+                or not self.matrix_free):
+            jac, _ = self.op.jacobian(fields)
+            J = BlockStencil.from_jacobian(jac)
+            if self.N is None:
+                self.N = normal_operator(J, self.wdtype, sym=self.sym)
+            else:
+                normal_operator(J, out=self.N)
+            if not self.matrix_free:
+                self._N64 = self.N if self.wdtype == torch.float64 else \
+                    normal_operator(J, torch.float64, sym=self.sym)
+            self.mg.update_A(self.N)
+            self.mg.prepare(self.ndirect)
+        self._nstep += 1
         if self.matrix_free:
             if self.state_w is not fields:
                 for a, b in zip(self.state_w, fields):

@@ -319,7 +319,8 @@ def _rand_sym(dev, dtype, nw, nvar, seed):
 
 
 @pytest.mark.parametrize('nw,nvar', [((13, 10), 3), ((40, 37), 2),
-                                     ((7, 6, 9), 2)])
+                                     ((7, 6, 9), 2), ((33, 67), 3),
+                                     ((9, 8, 23), 2)])
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
 def test_sym_apply(dev, nw, nvar, dtype):
     A, ent, coef = _rand_sym(dev, dtype, nw, nvar, seed=len(nw) + nvar)
@@ -342,6 +343,24 @@ def test_sym_apply(dev, nw, nvar, dtype):
         d = A.coef[q].double().cpu().numpy()
         want = xn[k] + 0.6 * (fn[k] - ref[k]) / d
         assert np.abs(y[k].double().cpu().numpy() - want).max() <= tol * sc
+    # every launch shape on contiguous x (one allocation), incl. the vector
+    # path (-1: 16 bytes of consecutive cells per thread)
+    xc = list(torch.tensor(np.stack(xn), dtype=dtype, device=dev).unbind(0))
+    fc = list(torch.tensor(np.stack(fn), dtype=dtype, device=dev).unbind(0))
+    for lanes, per in [(-1, 1)] + [(L, p) for L in (1, 2, 4, 8, 16)
+                                    for p in (1, 3)]:
+        A.lanes, A.per = lanes, per
+        for a, r in zip(A.apply(xc), ref):
+            assert np.abs(a.double().cpu().numpy() - r).max() <= tol * sc
+        yc = [torch.empty_like(a) for a in xc]
+        A.jacobi(xc, fc, yc, 0.6)
+        for k in range(nvar):
+            q = A.entries.index((k, k, (0, ) * len(nw)))
+            d = A.coef[q].double().cpu().numpy()
+            want = xn[k] + 0.6 * (fn[k] - ref[k]) / d
+            assert np.abs(yc[k].double().cpu().numpy() - want).max() <= \
+                tol * sc, lanes
+    A.lanes, A.per = 0, 1
     # the mirrored CSR and the dense matrix are symmetric and match
     M = A.tocsr()
     assert abs(M - M.T).max() <= 1e-15 * abs(M).max()
