@@ -4,10 +4,11 @@
 axis), mixed precision vs FP64, plus residual / adjoint kernel bandwidth.
 
 One step = one Newton iteration: residual and right-hand side -J^T f (FP64),
-stencil Jacobian and stored normal operator J^T J for the Galerkin hierarchy
-(7 levels, down to 17x17), coarsest FP64 least-squares operator, `maxiter`
-iterations of flexible CG on J^T J (FP64, matrix-free) preconditioned by one
-V(2,2) cycle, state update. Mixed precision runs the preconditioner in FP32.
+`maxiter` iterations of flexible CG on J^T J (FP64, matrix-free)
+preconditioned by one V(2,2) cycle, state update; every `setup_every`-th step
+also rebuilds the stencil Jacobian, the stored normal operator J^T J, the
+Galerkin hierarchy (7 levels, down to 17x17) and the coarsest FP64
+least-squares operator. Mixed precision runs the preconditioner in FP32.
 Inputs start resident in HBM (zero state, Re = 100, lid u = 1).
 
 Prints one JSON line (see the keys below). `--gpus N` with N > 1 is not
@@ -38,6 +39,10 @@ def parse():
     p.add_argument('--linsolver', default='pcg', choices=['vcycle', 'pcg'])
     p.add_argument('--omega', type=float, default=0.5)
     p.add_argument('--nlvl', type=int, default=7)
+    p.add_argument('--setup_every', type=int, default=4,
+                   help='rebuild the stored multigrid hierarchy every k '
+                   'Newton steps (the Krylov operator and finest level are '
+                   'always current)')
     p.add_argument('--tol', type=float, default=1e-8)
     p.add_argument('--max_epochs', type=int, default=150)
     p.add_argument('--kernel_reps', type=int, default=50)
@@ -63,7 +68,8 @@ def make_solver(args, precision):
                          omega=args.omega, precision=precision,
                          linsolver=args.linsolver, galerkin_mode='exact',
                          restriction='transpose',
-                         use_graph=not args.no_graph)
+                         use_graph=not args.no_graph,
+                         setup_every=args.setup_every)
     fields = [torch.zeros(args.n, args.n, dtype=torch.float64,
                           device='cuda') for _ in range(3)]
     return op, nm, fields
@@ -299,10 +305,11 @@ def main():
         'config': {
             'workload': 'lid-driven cavity {0}x{0}, Re={1:g}, Newton with '
                         'GMG normal-equation solve ({5}, {3} iterations/step, '
-                        '{2} levels, V(2,2), omega={4})'.format(
+                        '{2} levels, V(2,2), omega={4}, hierarchy rebuilt '
+                        'every {6} steps)'.format(
                             args.n, args.re, args.nlvl, args.maxiter,
                             args.omega, 'FCG+V-cycle' if args.linsolver ==
-                            'pcg' else 'V-cycles'),
+                            'pcg' else 'V-cycles', args.setup_every),
             'grid': [args.n, args.n],
             'parallelism': 'single GPU'},
         'fp64_steps_per_s': sps_64,
