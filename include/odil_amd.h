@@ -288,16 +288,15 @@ int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
  * half (entries o < i, or o == i and s >= 0). `items` int32 [nitem][10]
  * lists per output field the direct and mirrored products
  * (q, var, cs_lin, xs_lin, cs0..2, xs0..2), `ostart` [nvar + 1], `reach`
- * host int[3] bounds the offsets per axis, max_items <= 512, nent the
- * coefficient array count, lanes the threads per cell (1, 2, 4, 8, 16; 0
- * picks by grid size; -1: one thread per 16 bytes of consecutive cells with
- * vector loads, x in one allocation only), per the 256-thread tiles one
+ * host int[3] bounds the offsets per axis (at most 511), max_items <= 512,
+ * nent the coefficient array count, lanes the threads per cell (1, 2, 4, 8,
+ * 16; 0 picks by grid size; -1: one thread per 16 bytes of consecutive cells
+ * with vector loads, x in one allocation only), per the 256-thread tiles one
  * workgroup sweeps (1..64; the autotuner chooses lanes and per for each
  * operator shape). Fields of x in one allocation (x_k = x_0 + k N) take
- * faster paths. No reference
- * counterpart for the storage: it
- * applies the same matrices as the reference's full stencils
- * (mul_transpose_op products and Galerkin levels are symmetric). */
+ * faster paths. No reference counterpart for the storage: it applies the
+ * same matrices as the reference's full stencils (mul_transpose_op products
+ * and Galerkin levels are symmetric). */
 int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
                            const int* items, const int* ostart,
                            const int* dent, const int* reach, int max_items,

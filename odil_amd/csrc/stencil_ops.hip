@@ -126,7 +126,41 @@ __global__ void __launch_bounds__(256)
 // reads coef[c + a] and xb[c + b] with 32-bit index arithmetic. CONTIG: the
 // x fields are one allocation, x_k = x_0 + k N (otherwise the field pointer
 // is selected per item). A group of LANES threads owns one cell.
+// The per-axis shifts are staged beside the flat offsets, packed (three
+// 10-bit fields biased by 512, |shift| <= 511 checked on the host; var in
+// bits 30-31 of the x word), so that a cell near a face runs the same
+// unrolled, load-batched loop as an inner cell: a product whose coefficient
+// or x point falls outside the grid reads a valid address (element 0) and is
+// masked to zero. Branching out per product instead serialises one memory
+// round trip per item in every wavefront that holds such a cell.
 constexpr int kMaxItems = 512;
+constexpr int kMaxShift = 511;
+
+__device__ inline int pack_shift(int a, int b, int c) {
+  return (a + 512) | ((b + 512) << 10) | ((c + 512) << 20);
+}
+
+// Is (i0, i1, i2) + packed shift p inside the grid?
+__device__ inline bool shifted_inside(int i0, int i1, int i2, int p,
+                                      const Grid3& nw) {
+  // Bitwise & (no short circuit) keeps the item loop free of branches.
+  return ((unsigned)(i0 + (p & 1023) - 512) < (unsigned)nw.n0) &
+         ((unsigned)(i1 + ((p >> 10) & 1023) - 512) < (unsigned)nw.n1) &
+         ((unsigned)(i2 + ((p >> 20) & 1023) - 512) < (unsigned)nw.n2);
+}
+
+// Stage items [i0e, i0e + ne) of the global table into LDS (see above).
+template <bool CONTIG>
+__device__ inline void stage_items(const int* __restrict__ items, int i0e,
+                                   int ne, int N, int2* fast, int2* shf) {
+  for (int t = threadIdx.x; t < ne; t += blockDim.x) {
+    const int* it = items + 10 * (i0e + t);
+    fast[t] = make_int2(it[0] * N + it[2], (CONTIG ? it[1] * N : 0) + it[3]);
+    shf[t] = make_int2(pack_shift(it[4], it[5], it[6]),
+                       pack_shift(it[7], it[8], it[9]) |
+                           (int)((unsigned)it[1] << 30));
+  }
+}
 
 template <class T>
 __device__ inline const T* pick(const CVecPtrs<T>& x, int k) {
@@ -144,6 +178,7 @@ __global__ void __launch_bounds__(256)
                 CVecPtrs<T> x, VecPtrs<T> y, CVecPtrs<T> f, Grid3 nw,
                 Grid3 reach, int mode, T omega, int per) {
   __shared__ int2 fast[kMaxItems];
+  __shared__ int2 shf[kMaxItems];
   // Logical block: a chunk of `per` consecutive tiles, output field o
   // fastest (one XCD per chunk); the item list is staged once per chunk.
   const unsigned g = xcd_block(blockIdx.x + blockIdx.y * gridDim.x,
@@ -153,10 +188,7 @@ __global__ void __launch_bounds__(256)
   const int i0e = ostart[o];
   const int ne = ostart[o + 1] - i0e;
   const int N = (int)nw.size();
-  for (int t = threadIdx.x; t < ne; t += blockDim.x) {
-    const int* it = items + 10 * (i0e + t);
-    fast[t] = make_int2(it[0] * N + it[2], (CONTIG ? it[1] * N : 0) + it[3]);
-  }
+  stage_items<CONTIG>(items, i0e, ne, N, fast, shf);
   __syncthreads();
   const int lane = threadIdx.x % LANES;
   for (int r = 0; r < per; ++r) {
@@ -182,19 +214,21 @@ __global__ void __launch_bounds__(256)
         for (int e = lane; e < ne; e += LANES) {
           const int2 it = fast[e];
           acc += coef[(unsigned)(c + it.x)] *
-                 pick(x, items[10 * (i0e + e) + 1])[(unsigned)(c + it.y)];
+                 pick(x, (unsigned)shf[e].y >> 30)[(unsigned)(c + it.y)];
         }
       }
     } else {
-      // Cells near a face (few): full shifts from the global table.
+      // Cells near a face: products outside the grid masked to zero.
+#pragma unroll 4
       for (int e = lane; e < ne; e += LANES) {
-        const int* fl = items + 10 * (i0e + e);
-        const int a0 = i0 + fl[4], a1 = i1 + fl[5], a2 = i2 + fl[6];
-        const int b0 = i0 + fl[7], b1 = i1 + fl[8], b2 = i2 + fl[9];
-        if (!inside(a0, a1, a2, nw) || !inside(b0, b1, b2, nw)) continue;
-        acc +=
-            coef[(size_t)fl[0] * N + ((size_t)a0 * nw.n1 + a1) * nw.n2 + a2] *
-            pick(x, fl[1])[((size_t)b0 * nw.n1 + b1) * nw.n2 + b2];
+        const int2 it = fast[e];
+        const int2 sh = shf[e];
+        const bool ok = shifted_inside(i0, i1, i2, sh.x, nw) &
+                        shifted_inside(i0, i1, i2, sh.y, nw);
+        const T* xp = CONTIG ? x.p[0] : pick(x, (unsigned)sh.y >> 30);
+        const T a = coef[ok ? (unsigned)(c + it.x) : 0u];
+        const T b = xp[ok ? (unsigned)(c + it.y) : 0u];
+        acc += ok ? a * b : T(0);
       }
     }
 #pragma unroll
@@ -239,6 +273,7 @@ __global__ void __launch_bounds__(256)
   constexpr int V = 16 / sizeof(T);
   typedef T VT __attribute__((ext_vector_type(V)));
   __shared__ int2 fast[kMaxItems];
+  __shared__ int2 shf[kMaxItems];
   const unsigned g = xcd_block(blockIdx.x + blockIdx.y * gridDim.x,
                                gridDim.x * gridDim.y);
   const int o = g % gridDim.y;
@@ -246,10 +281,7 @@ __global__ void __launch_bounds__(256)
   const int i0e = ostart[o];
   const int ne = ostart[o + 1] - i0e;
   const int N = (int)nw.size();
-  for (int t = threadIdx.x; t < ne; t += blockDim.x) {
-    const int* it = items + 10 * (i0e + t);
-    fast[t] = make_int2(it[0] * N + it[2], it[1] * N + it[3]);
-  }
+  stage_items<true>(items, i0e, ne, N, fast, shf);
   __syncthreads();
   const int c0 = (int)(((size_t)bx * blockDim.x + threadIdx.x) * V);
   if (c0 >= N) return;
@@ -291,19 +323,32 @@ __global__ void __launch_bounds__(256)
     vst(y.p[o] + c0, out);
     return;
   }
+  // Span near a face or across a row end: per node, products outside the
+  // grid masked to zero (nodes past the end of the grid are skipped).
+  int ii0[V], ii1[V], ii2[V];
+  T accs[V];
+#pragma unroll
+  for (int v = 0; v < V; ++v) {
+    unflat(min(c0 + v, N - 1), nw, ii0[v], ii1[v], ii2[v]);
+    accs[v] = 0;
+  }
+#pragma unroll 2
+  for (int e = 0; e < ne; ++e) {
+    const int2 it = fast[e];
+    const int2 sh = shf[e];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const bool ok = (v < nc) &
+                      shifted_inside(ii0[v], ii1[v], ii2[v], sh.x, nw) &
+                      shifted_inside(ii0[v], ii1[v], ii2[v], sh.y, nw);
+      const T a = coef[ok ? (unsigned)(c0 + v + it.x) : 0u];
+      const T b = xb[ok ? (unsigned)(c0 + v + it.y) : 0u];
+      accs[v] += ok ? a * b : T(0);
+    }
+  }
   for (int v = 0; v < nc; ++v) {
     const int c = c0 + v;
-    int i0, i1, i2;
-    unflat(c, nw, i0, i1, i2);
-    T acc = 0;
-    for (int e = 0; e < ne; ++e) {
-      const int* fl = items + 10 * (i0e + e);
-      const int p0 = i0 + fl[4], p1 = i1 + fl[5], p2 = i2 + fl[6];
-      const int q0 = i0 + fl[7], q1 = i1 + fl[8], q2 = i2 + fl[9];
-      if (!inside(p0, p1, p2, nw) || !inside(q0, q1, q2, nw)) continue;
-      acc += coef[(size_t)fl[0] * N + ((size_t)p0 * nw.n1 + p1) * nw.n2 + p2] *
-             xb[(size_t)fl[1] * N + ((size_t)q0 * nw.n1 + q1) * nw.n2 + q2];
-    }
+    const T acc = accs[v];
     T out;
     if (mode == 0)
       out = acc;
@@ -775,6 +820,10 @@ int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
   if (!reach || reach[0] < 0 || reach[1] < 0 || reach[2] < 0 ||
       (ndim == 2 && reach[2] != 0)) {
     set_error("stencil_apply_sym: bad reach");
+    return kErrArg;
+  }
+  if (reach[0] > kMaxShift || reach[1] > kMaxShift || reach[2] > kMaxShift) {
+    set_error("stencil_apply_sym: shifts beyond %d per axis", kMaxShift);
     return kErrArg;
   }
   Grid3 g = mk_grid(ndim, nw);
