@@ -201,7 +201,9 @@ __global__ void __launch_bounds__(256)
                        i1 >= reach.n1 && i1 < nw.n1 - reach.n1 &&
                        i2 >= reach.n2 && i2 < nw.n2 - reach.n2;
     T acc = 0;
-    if (inner) {
+    // One loop per wavefront: one that holds any cell near a face runs the
+    // masked loop for all its cells instead of both loops in turn.
+    if (__all(inner)) {
       if (CONTIG) {
         const T* __restrict__ xb = x.p[0];
 #pragma unroll 16
@@ -496,67 +498,94 @@ __global__ void __launch_bounds__(256)
   cc[(size_t)o * Nc + I] = (T)acc;
 }
 
-// Restriction of nvar fields (R.mul_field, multigrid.py:496-534).
-template <class T>
+// Restriction of NV fields (R.mul_field, multigrid.py:496-534). The 3^DIM
+// taps are fully unrolled with every field's load issued before the sums: a
+// tap with zero weight or outside the fine grid reads element 0 and adds an
+// exact zero (same sum order as skipping it), so no load waits on a branch.
+template <class T, int NV, int DIM>
 __global__ void __launch_bounds__(256)
-    k_restrict(CVecPtrs<T> fine, VecPtrs<T> coarse, int nvar,
-               const double* __restrict__ rw, Grid3 nw, Grid3 nwh, int dim,
+    k_restrict(CVecPtrs<T> fine, VecPtrs<T> coarse,
+               const double* __restrict__ rw, Grid3 nw, Grid3 nwh,
                int rmode) {
-  const size_t I =
-      (size_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const unsigned I = xcd_block(blockIdx.x, gridDim.x) * blockDim.x +
+                     threadIdx.x;
   if (I >= nwh.size()) return;
   int I0, I1, I2;
   unflat(I, nwh, I0, I1, I2);
   const bool inner = (I0 > 0 && I0 < nwh.n0 - 1) &&
                      (I1 > 0 && I1 < nwh.n1 - 1) &&
-                     (dim < 3 || (I2 > 0 && I2 < nwh.n2 - 1));
+                     (DIM < 3 || (I2 > 0 && I2 < nwh.n2 - 1));
   const int c0 = 2 * I0, c1 = 2 * I1, c2 = 2 * I2;
-  for (int k = 0; k < nvar; ++k) {
-    const T* f = fine.p[k];
-    T acc;
-    if (!inner && rmode == 0) {
-      acc = f[((size_t)c0 * nw.n1 + c1) * nw.n2 + c2];
-    } else {
-      acc = 0;
-      const int r2lo = dim < 3 ? 0 : -1, r2hi = dim < 3 ? 0 : 1;
-      for (int r0 = -1; r0 <= 1; ++r0)
-        for (int r1 = -1; r1 <= 1; ++r1)
-          for (int r2 = r2lo; r2 <= r2hi; ++r2) {
-            const double w = rw[(r0 + 1) * 9 + (r1 + 1) * 3 + (r2 + 1)];
-            if (w != 0.0 && inside(c0 + r0, c1 + r1, c2 + r2, nw))
-              acc += T(w) *
-                     f[((size_t)(c0 + r0) * nw.n1 + (c1 + r1)) * nw.n2 +
-                       (c2 + r2)];
+  T acc[NV];
+  if (!inner && rmode == 0) {
+    const unsigned c = ((unsigned)c0 * nw.n1 + c1) * nw.n2 + c2;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = fine.p[k][c];
+  } else {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0;
+    constexpr int R2 = DIM == 3 ? 1 : 0;
+#pragma unroll
+    for (int r0 = -1; r0 <= 1; ++r0)
+#pragma unroll
+      for (int r1 = -1; r1 <= 1; ++r1)
+#pragma unroll
+        for (int r2 = -R2; r2 <= R2; ++r2) {
+          const double w = rw[(r0 + 1) * 9 + (r1 + 1) * 3 + (r2 + 1)];
+          const bool ok = (w != 0.0) & inside(c0 + r0, c1 + r1, c2 + r2, nw);
+          const unsigned q =
+              ok ? ((unsigned)(c0 + r0) * nw.n1 + (c1 + r1)) * nw.n2 +
+                       (c2 + r2)
+                 : 0u;
+#pragma unroll
+          for (int k = 0; k < NV; ++k) {
+            const T v = fine.p[k][q];
+            acc[k] += ok ? T(w) * v : T(0);
           }
-    }
-    coarse.p[k][I] = acc;
+        }
   }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) coarse.p[k][I] = acc[k];
 }
 
 // Prolongation with add: u_fine += T e_coarse, multilinear interpolation on
-// node-based grids (TT.mul_transpose_field, multigrid.py:471-494).
-template <class T>
+// node-based grids (TT.mul_transpose_field, multigrid.py:471-494). The
+// 2^DIM coarse neighbours are read unconditionally (a tap that the node's
+// parity excludes reads its base node and adds an exact zero).
+template <class T, int NV, int DIM>
 __global__ void __launch_bounds__(256)
-    k_prolong_add(CVecPtrs<T> coarse, VecPtrs<T> fine, int nvar, Grid3 nw,
+    k_prolong_add(CVecPtrs<T> coarse, VecPtrs<T> fine, Grid3 nw,
                   Grid3 nwh) {
-  const size_t p =
-      (size_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const unsigned p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x +
+                     threadIdx.x;
   if (p >= nw.size()) return;
   int p0, p1, p2;
   unflat(p, nw, p0, p1, p2);
   const int a0 = p0 >> 1, a1 = p1 >> 1, a2 = p2 >> 1;
   const int o0 = p0 & 1, o1 = p1 & 1, o2 = p2 & 1;
   const T w = T(1) / T(1 << (o0 + o1 + o2));
-  for (int k = 0; k < nvar; ++k) {
-    const T* e = coarse.p[k];
-    T acc = 0;
-    for (int d0 = 0; d0 <= o0; ++d0)
-      for (int d1 = 0; d1 <= o1; ++d1)
-        for (int d2 = 0; d2 <= o2; ++d2)
-          acc += e[((size_t)(a0 + d0) * nwh.n1 + (a1 + d1)) * nwh.n2 +
-                   (a2 + d2)];
-    fine.p[k][p] += w * acc;
-  }
+  const unsigned base = ((unsigned)a0 * nwh.n1 + a1) * nwh.n2 + a2;
+  T acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0;
+  constexpr int D2 = DIM == 3 ? 1 : 0;
+#pragma unroll
+  for (int d0 = 0; d0 <= 1; ++d0)
+#pragma unroll
+    for (int d1 = 0; d1 <= 1; ++d1)
+#pragma unroll
+      for (int d2 = 0; d2 <= D2; ++d2) {
+        const bool ok = (d0 <= o0) & (d1 <= o1) & (d2 <= o2);
+        const unsigned q =
+            ok ? base + ((unsigned)d0 * nwh.n1 + d1) * nwh.n2 + d2 : base;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          const T v = coarse.p[k][q];
+          acc[k] += ok ? v : T(0);
+        }
+      }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) fine.p[k][p] += w * acc[k];
 }
 
 // Dense matrix of a square block stencil operator (stride 1), row-major,
@@ -1022,18 +1051,28 @@ int odil_mg_restrict(int dtype, int ndim, const int* nw, int nvar,
   Grid3 g = mk_grid(ndim, nw);
   Grid3 gh{g.n0 / 2 + 1, g.n1 / 2 + 1, ndim == 3 ? g.n2 / 2 + 1 : 1};
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (dtype == kF64)
-    hipLaunchKernelGGL(k_restrict<double>, dim3(nblk(gh.size())), dim3(256), 0,
-                       s, cvp<double>(fine, nvar), vp<double>(coarse, nvar),
-                       nvar, rw, g, gh, ndim, rmode);
-  else if (dtype == kF32)
-    hipLaunchKernelGGL(k_restrict<float>, dim3(nblk(gh.size())), dim3(256), 0,
-                       s, cvp<float>(fine, nvar), vp<float>(coarse, nvar),
-                       nvar, rw, g, gh, ndim, rmode);
-  else {
+  if (dtype != kF64 && dtype != kF32) {
     set_error("mg_restrict: bad dtype %d", dtype);
     return kErrArg;
   }
+#define ODIL_RESTRICT(T, NV, D)                                              \
+  hipLaunchKernelGGL((k_restrict<T, NV, D>), dim3(nblk(gh.size())),          \
+                     dim3(256), 0, s, cvp<T>(fine, nvar), vp<T>(coarse, nvar), \
+                     rw, g, gh, rmode)
+#define ODIL_RESTRICT_NV(T, D)        \
+  switch (nvar) {                     \
+    case 1: ODIL_RESTRICT(T, 1, D); break; \
+    case 2: ODIL_RESTRICT(T, 2, D); break; \
+    case 3: ODIL_RESTRICT(T, 3, D); break; \
+    default: ODIL_RESTRICT(T, 4, D); break; \
+  }
+  if (dtype == kF64) {
+    if (ndim == 3) ODIL_RESTRICT_NV(double, 3) else ODIL_RESTRICT_NV(double, 2)
+  } else {
+    if (ndim == 3) ODIL_RESTRICT_NV(float, 3) else ODIL_RESTRICT_NV(float, 2)
+  }
+#undef ODIL_RESTRICT_NV
+#undef ODIL_RESTRICT
   return check_launch("mg_restrict");
 }
 
@@ -1049,18 +1088,28 @@ int odil_mg_prolong_add(int dtype, int ndim, const int* nw, int nvar,
   }
   Grid3 gh{g.n0 / 2 + 1, g.n1 / 2 + 1, ndim == 3 ? g.n2 / 2 + 1 : 1};
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (dtype == kF64)
-    hipLaunchKernelGGL(k_prolong_add<double>, dim3(nblk(g.size())), dim3(256),
-                       0, s, cvp<double>(coarse, nvar), vp<double>(fine, nvar),
-                       nvar, g, gh);
-  else if (dtype == kF32)
-    hipLaunchKernelGGL(k_prolong_add<float>, dim3(nblk(g.size())), dim3(256),
-                       0, s, cvp<float>(coarse, nvar), vp<float>(fine, nvar),
-                       nvar, g, gh);
-  else {
+  if (dtype != kF64 && dtype != kF32) {
     set_error("mg_prolong_add: bad dtype %d", dtype);
     return kErrArg;
   }
+#define ODIL_PROLONG(T, NV, D)                                               \
+  hipLaunchKernelGGL((k_prolong_add<T, NV, D>), dim3(nblk(g.size())),        \
+                     dim3(256), 0, s, cvp<T>(coarse, nvar), vp<T>(fine, nvar), \
+                     g, gh)
+#define ODIL_PROLONG_NV(T, D)        \
+  switch (nvar) {                    \
+    case 1: ODIL_PROLONG(T, 1, D); break; \
+    case 2: ODIL_PROLONG(T, 2, D); break; \
+    case 3: ODIL_PROLONG(T, 3, D); break; \
+    default: ODIL_PROLONG(T, 4, D); break; \
+  }
+  if (dtype == kF64) {
+    if (ndim == 3) ODIL_PROLONG_NV(double, 3) else ODIL_PROLONG_NV(double, 2)
+  } else {
+    if (ndim == 3) ODIL_PROLONG_NV(float, 3) else ODIL_PROLONG_NV(float, 2)
+  }
+#undef ODIL_PROLONG_NV
+#undef ODIL_PROLONG
   return check_launch("mg_prolong_add");
 }
 

@@ -132,23 +132,27 @@ def test_galerkin_3d(dev):
 
 @pytest.mark.parametrize('nw', [(17, 33), (9, 9, 17)])
 @pytest.mark.parametrize('restriction', ['full', 'transpose'])
-def test_restrict_prolong(dev, nw, restriction):
+@pytest.mark.parametrize('nvar,dtype', [(1, torch.float64), (2, torch.float64),
+                                        (3, torch.float32), (4, torch.float64)])
+def test_restrict_prolong(dev, nw, restriction, nvar, dtype):
     from odil_amd.multigrid import prolong_add, restrict, restriction_weights
+    tol = 1e-14 if dtype == torch.float64 else 2e-6
     rw = torch.tensor(restriction_weights(restriction, len(nw)), device=dev)
-    x = _vec(dev, torch.float64, 2, nw, 4)
+    x = _vec(dev, dtype, nvar, nw, 4)
     xc = restrict(x, nw, rw, restriction=restriction)
     R = omg.get_R(nw, restriction).tocsr()
-    for k in range(2):
-        ref = R @ x[k].cpu().numpy().ravel()
-        assert np.abs(xc[k].cpu().numpy().ravel() - ref).max() < 1e-14
+    for k in range(nvar):
+        ref = R @ x[k].cpu().double().numpy().ravel()
+        assert np.abs(xc[k].cpu().double().numpy().ravel() - ref).max() < tol
     nwh = tuple(n // 2 + 1 for n in nw)
-    e = _vec(dev, torch.float64, 2, nwh, 5)
+    e = _vec(dev, dtype, nvar, nwh, 5)
     u = [a.clone() for a in x]
     prolong_add(e, u, nw)
     T = omg.get_TT(nw).tocsr().T
-    for k in range(2):
-        ref = x[k].cpu().numpy().ravel() + T @ e[k].cpu().numpy().ravel()
-        assert np.abs(u[k].cpu().numpy().ravel() - ref).max() < 1e-14
+    for k in range(nvar):
+        ref = (x[k].cpu().double().numpy().ravel() +
+               T @ e[k].cpu().double().numpy().ravel())
+        assert np.abs(u[k].cpu().double().numpy().ravel() - ref).max() < tol
 
 
 def _poisson17(dev, dtype=torch.float64):
