@@ -341,6 +341,16 @@ int odil_mg_restrict(int dtype, int ndim, const int* nw, int nvar,
 int odil_mg_prolong_add(int dtype, int ndim, const int* nw, int nvar,
                         const void* const* coarse, void* const* fine,
                         void* stream);
+/* y_k = omega f_k / d_k on N values of nvar fields: smoother_jacobi
+ * (multigrid.py:543-552) applied to the zero initial guess that each
+ * coarse level of MultigridOp.step starts from (multigrid.py:590-593), with
+ * d_k the diagonal of the level operator. fw != NULL also stores f_k
+ * converted to the y precision (an FP32 preconditioner's copy of its FP64
+ * input). dtype pairs (f, y): FP64/FP64, FP32/FP32, FP64/FP32. */
+int odil_jacobi_zero(int dtype_f, int dtype_y, int nvar, long long N,
+                     const void* const* f, const void* const* d,
+                     void* const* y, void* const* fw, double omega,
+                     void* stream);
 /* Dense FP64 row-major matrix (zeroed by the caller) of a square stride-1
  * operator (sym != 0: upper-half storage) for the coarsest-level direct
  * solve (smoother_direct, multigrid.py:554-565; SparseOperator.tocsr,

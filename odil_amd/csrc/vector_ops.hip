@@ -116,6 +116,26 @@ __global__ void __launch_bounds__(256)
     out[i] = x[i] + aa * y[i];
 }
 
+// One weighted-Jacobi sweep from a zero iterate, y_k = omega f_k / d_k
+// (x + omega (f - A x) / d at x = 0, same rounding), optionally writing
+// fw_k = f_k converted to the y precision: the first pre-smoothing sweep of
+// a V-cycle level without the fill of the iterate, the operator product or,
+// for an FP32 preconditioner, a separate conversion pass of its FP64 input.
+template <class Tf, class Ty>
+__global__ void __launch_bounds__(256)
+    k_jacobi_zero(CVP<Tf> f, CVP<Ty> d, VP<Ty> y, VP<Ty> fw, int nvar,
+                  size_t N, Ty omega) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const bool copy = fw.p[0] != nullptr;
+  for (int k = 0; k < nvar; ++k)
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += stride) {
+      const Ty fv = (Ty)f.p[k][i];
+      if (copy) fw.p[k][i] = fv;
+      y.p[k][i] = omega * fv / d.p[k][i];
+    }
+}
+
 }  // namespace odil
 
 using namespace odil;
@@ -244,6 +264,42 @@ int odil_adam_step(int dtype, long long N, void* x, const void* g, void* m,
     return kErrArg;
   }
   return check_launch("adam_step");
+}
+
+// y_k = omega f_k / d_k (k < nvar, N values each); fw_k = f_k in the y
+// precision when fw is not NULL. dtype pairs (f, y): (FP64, FP64),
+// (FP32, FP32), (FP64, FP32).
+int odil_jacobi_zero(int dtype_f, int dtype_y, int nvar, long long N,
+                     const void* const* f, const void* const* d,
+                     void* const* y, void* const* fw, double omega,
+                     void* stream) {
+  if (nvar < 1 || nvar > kMaxVec || N < 0 || !f || !d || !y) {
+    set_error("jacobi_zero: bad arguments");
+    return kErrArg;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const VP<double> fw64 = fw ? mkv<double>(fw, nvar) : VP<double>{};
+  const VP<float> fw32 = fw ? mkv<float>(fw, nvar) : VP<float>{};
+  if (dtype_f == kF64 && dtype_y == kF64)
+    hipLaunchKernelGGL((k_jacobi_zero<double, double>), dim3(grid_for(N)),
+                       dim3(256), 0, s, mkc<double>(f, nvar),
+                       mkc<double>(d, nvar), mkv<double>(y, nvar), fw64, nvar,
+                       (size_t)N, omega);
+  else if (dtype_f == kF32 && dtype_y == kF32)
+    hipLaunchKernelGGL((k_jacobi_zero<float, float>), dim3(grid_for(N)),
+                       dim3(256), 0, s, mkc<float>(f, nvar),
+                       mkc<float>(d, nvar), mkv<float>(y, nvar), fw32, nvar,
+                       (size_t)N, (float)omega);
+  else if (dtype_f == kF64 && dtype_y == kF32)
+    hipLaunchKernelGGL((k_jacobi_zero<double, float>), dim3(grid_for(N)),
+                       dim3(256), 0, s, mkc<double>(f, nvar),
+                       mkc<float>(d, nvar), mkv<float>(y, nvar), fw32, nvar,
+                       (size_t)N, (float)omega);
+  else {
+    set_error("jacobi_zero: unsupported dtypes %d -> %d", dtype_f, dtype_y);
+    return kErrArg;
+  }
+  return check_launch("jacobi_zero");
 }
 
 // out = x + a y on N values.

@@ -218,6 +218,17 @@ class BlockStencil:
         self._call_apply(x, y, f, 2, omega)
         return y
 
+    def jacobi_zero(self, f, y, omega, f_src=None):
+        """The Jacobi sweep from x = 0: y = omega f / diag(A), one pass
+        without the operator product. With `f_src` (same shape, e.g. the
+        FP64 vector a working-precision V-cycle starts from), reads f_src
+        and stores its conversion into f."""
+        if np.any(self._dent_host < 0):
+            raise ValueError('Jacobi needs a diagonal entry in every block')
+        d = [self.coef[int(q)] for q in self._dent_host]
+        return jacobi_zero(f if f_src is None else f_src, d, y, omega,
+                           fw=None if f_src is None else f)
+
     def tune(self, x, f, y, omega=0.5):
         """Pick the threads per cell of the symmetric sweep by timing the
         Jacobi sweep on these buffers (cached per operator shape)."""
@@ -521,6 +532,18 @@ def galerkin(A, restriction='full', rw_dev=None, mode='reference',
                         nvar_out=A.nvar_out, sym=A.sym)
 
 
+def jacobi_zero(f, d, y, omega, fw=None):
+    """y_k = omega f_k / d_k for each field (a weighted-Jacobi sweep from a
+    zero iterate); fw_k = f_k converted to y's dtype when fw is given."""
+    n = f[0].numel()
+    st = _lib.lib().odil_jacobi_zero(
+        c_int(_lib.dtype_tag(f[0].dtype)), c_int(_lib.dtype_tag(y[0].dtype)),
+        c_int(len(f)), ctypes.c_longlong(int(n)), ptrs(f), ptrs(d), ptrs(y),
+        ptrs(fw) if fw is not None else None, c_double(omega), stream())
+    check(st, 'jacobi_zero')
+    return y
+
+
 def restrict(fine, nw, rw_dev, coarse=None, restriction='full'):
     """Restriction R of a list of fields from node grid `nw` to nw//2+1."""
     nwh = tuple(n // 2 + 1 for n in nw)
@@ -737,32 +760,54 @@ class MultigridOp:
             self._buf = b
         return self._buf
 
-    def vcycle(self, u, f, omega=0.6, ndirect=5, pre=2, post=2):
+    def vcycle(self, u, f, omega=0.6, ndirect=5, pre=2, post=2, zero=False,
+               f_src=None):
         """V-cycle with the same arithmetic as step() but on preallocated
         level buffers (no allocation, capturable in a HIP graph). `u` is
-        updated in place; `f` is read only."""
+        updated in place; `f` is read only. zero=True: the cycle starts from
+        u = 0 whatever u holds (the first pre-smoothing sweep is then one
+        pass, y = omega f / d). f_src (with zero=True): the right-hand side
+        in another precision, converted into `f` by that first pass."""
         buf = self._buffers()
-        self._vc(0, u, f, buf[0]['v'], omega, ndirect, pre, post)
+        if f_src is not None and not (zero and pre > 0):
+            for a, b in zip(f, f_src):
+                a.copy_(b)
+            f_src = None
+        self._vc(0, u, f, buf[0]['v'], omega, ndirect, pre, post, zero, f_src)
         return u
 
-    def _vc(self, level, u, f, spare, omega, ndirect, pre, post):
-        # `u` holds the iterate; `spare` is a same-shape scratch. Returns the
-        # list holding the result (u or spare).
+    def _vc(self, level, u, f, spare, omega, ndirect, pre, post, zero=False,
+            f_src=None):
+        # `u` holds the iterate (ignored with zero=True); `spare` is a
+        # same-shape scratch. Returns u.
         if level + 1 >= self.nlevels or min(self.nnw[level]) <= ndirect:
             Minv = self._coarse(level)
             N = int(np.prod(self.nnw[level]))
             dt = _lib.dtype_tag(f[0].dtype)
+            if zero:  # u = Minv f directly
+                out = u
+            else:
+                out = spare
             check(_lib.lib().odil_dense_gemv(c_int(dt), c_int(dt),
                                              c_int(self.nvar), c_int(N),
-                                             ptr(Minv), ptrs(f), ptrs(spare),
+                                             ptr(Minv), ptrs(f), ptrs(out),
                                              stream()), 'dense_gemv')
-            for a, b in zip(u, spare):
-                a.copy_(b)
+            if out is not u:
+                for a, b in zip(u, out):
+                    a.copy_(b)
             return u
         A = self._op(level)
         buf = self._buffers()
         cur, oth = u, spare
-        for _ in range(pre):
+        k0 = 0
+        if zero and pre > 0:
+            A.jacobi_zero(f, oth, omega, f_src=f_src)
+            cur, oth = oth, cur
+            k0 = 1
+        elif zero:
+            for x in u:
+                x.zero_()
+        for _ in range(k0, pre):
             A.jacobi(cur, f, oth, omega)
             cur, oth = oth, cur
         r = buf[level]['r']
@@ -770,10 +815,8 @@ class MultigridOp:
         nxt = buf[level + 1]
         restrict(r, self.nnw[level], self.rw, nxt['f'],
                  restriction=self.restriction)
-        for x in nxt['u']:
-            x.zero_()
         res = self._vc(level + 1, nxt['u'], nxt['f'], nxt['v'], omega, 5, 2,
-                       2)
+                       2, zero=True)
         prolong_add(res, cur, self.nnw[level])
         for _ in range(post):
             A.jacobi(cur, f, oth, omega)

@@ -51,7 +51,8 @@ import torch
 
 from odil_amd import _lib
 from odil_amd._lib import c_double, c_int, c_vp, check, ptrs, stream
-from odil_amd.multigrid import BlockStencil, MultigridOp, normal_operator
+from odil_amd.multigrid import (BlockStencil, MultigridOp, jacobi_zero,
+                               normal_operator)
 
 
 class DeviceScalars:
@@ -119,6 +120,12 @@ class FusedNormal:
     def jacobi(self, x, f, y, omega):
         return self.op.normal_apply(self.state, x, y, b=f, d=self.diag,
                                     omega=omega, mode=2)
+
+    def jacobi_zero(self, f, y, omega, f_src=None):
+        """Jacobi sweep from x = 0 (y = omega f / diag N); with f_src, reads
+        it and stores its conversion into f."""
+        return jacobi_zero(f if f_src is None else f_src, self.diag, y, omega,
+                           fw=None if f_src is None else f)
 
 
 class NewtonMultigrid:
@@ -204,20 +211,17 @@ class NewtonMultigrid:
 
     # -- linear solve on fixed buffers (capturable) ----------------------
     def _precond(self, r, z):
-        """z = M r: one V-cycle from zero in the working precision."""
-        same = self.wdtype == torch.float64
-        rw = r if same else self.b_w
-        zw = z if same else self.x_w
-        if not same:
-            for a, b in zip(rw, r):
-                a.copy_(b)
-        for a in zw:
-            a.zero_()
-        self.mg.vcycle(zw, rw, omega=self.omega, ndirect=self.ndirect,
-                       pre=self.pre, post=self.post)
-        if not same:
-            for a, b in zip(z, zw):
-                a.copy_(b)
+        """z = M r: one V-cycle from zero in the working precision (an FP32
+        cycle converts r in its first sweep)."""
+        if self.wdtype == torch.float64:
+            self.mg.vcycle(z, r, omega=self.omega, ndirect=self.ndirect,
+                           pre=self.pre, post=self.post, zero=True)
+            return
+        self.mg.vcycle(self.x_w, self.b_w, omega=self.omega,
+                       ndirect=self.ndirect, pre=self.pre, post=self.post,
+                       zero=True, f_src=r)
+        for a, b in zip(z, self.x_w):
+            a.copy_(b)
 
     def _apply_N(self, p, q):
         """q = N p in FP64 on the finest level."""

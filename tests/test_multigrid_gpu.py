@@ -424,3 +424,31 @@ def test_sym_vcycle_matches_full(dev):
         u[2] = u[2] - u[2].mean()
         out.append(_flat(u))
     assert np.abs(out[0] - out[1]).max() <= 1e-9 * np.abs(out[0]).max()
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_vcycle_zero_start(dev, dtype):
+    """A V-cycle started with zero=True (first sweep y = omega f / d, no
+    fill of u, FP64 right-hand side converted in that sweep) equals the
+    cycle from an explicitly zeroed iterate, bit for bit."""
+    from odil_amd.multigrid import MultigridOp, normal_operator
+    _, J, res = _cavity_jac(dev, 33, 33, 4)
+    f64 = J.apply_t([-r for r in res])
+    N = normal_operator(J, dtype, sym=True)
+    M = MultigridOp((33, 33), nvar=3, nlevels=4, restriction='transpose',
+                    coarse_reg=1e-12, galerkin_mode='exact', dtype=dtype)
+    M.update_A(N)
+    M.prepare(3)
+    f = [a.to(dtype) for a in f64]
+    ref = [torch.zeros_like(a) for a in f]
+    M.vcycle(ref, f, omega=0.5, ndirect=3)
+    got = [torch.full_like(a, 7.) for a in f]  # contents must be ignored
+    M.vcycle(got, f, omega=0.5, ndirect=3, zero=True)
+    for a, b in zip(ref, got):
+        assert torch.equal(a, b)
+    fw = [torch.full_like(a, 3.) for a in f]
+    got2 = [torch.full_like(a, -1.) for a in f]
+    M.vcycle(got2, fw, omega=0.5, ndirect=3, zero=True, f_src=f64)
+    for a, b, c, d in zip(ref, got2, fw, f):
+        assert torch.equal(a, b)
+        assert torch.equal(c, d)
