@@ -388,6 +388,19 @@ int odil_dot(int dtype, int nvar, long long N, const void* const* x,
 int odil_axpy_ratio(int dtype, int nvar, long long N, void* const* y,
                     const void* const* x, const double* num,
                     const double* den, double sign, void* stream);
+/* x_k += (num / den) p_k and r_k -= (num / den) q_k in one pass (the two
+ * conjugate-gradient updates sharing a step length; den == 0: none). */
+int odil_axpy2_ratio(int dtype, int nvar, long long N, void* const* x,
+                     const void* const* p, void* const* r,
+                     const void* const* q, const double* num,
+                     const double* den, void* stream);
+/* Flexible-CG pass after the preconditioner, one read of each vector:
+ * z = zw (FP64 or FP32, converted to FP64); *rz_prev = *rz; *rz = r.z;
+ * *rzo = r.zo with zo the previous z; then zo = z. `partial` holds
+ * 2 * odil_dot_partials() FP64 values; r and zo are FP64. */
+int odil_fcg_z(int dtype_w, int nvar, long long N, const void* const* zw,
+               const void* const* r, void* const* zo, double* partial,
+               double* rz, double* rz_prev, double* rzo, void* stream);
 /* p_k = z_k + ((num - sub) / den) p_k (sub may be NULL; den == 0: p = z). */
 int odil_xpay_ratio(int dtype, int nvar, long long N, void* const* p,
                     const void* const* z, const double* num,

@@ -226,35 +226,32 @@ class PoissonBatchSolver:
         self._graphs = None
 
     def _precond(self, k):
-        same = self.wdtype == torch.float64
-        rw = k.r if same else k.b_w
-        zw = k.z if same else k.x_w
-        if not same:
-            rw[0].copy_(k.r[0])
-        zw[0].zero_()
-        k.mg.vcycle(zw, rw, omega=self.omega, ndirect=self.ndirect)
-        if not same:
-            k.z[0].copy_(zw[0])
+        """One V-cycle from zero on k.r; returns the working-precision
+        result (an FP32 cycle converts k.r in its first sweep)."""
+        if self.wdtype == torch.float64:
+            k.mg.vcycle(k.z, k.r, omega=self.omega, ndirect=self.ndirect,
+                        zero=True)
+            return k.z
+        k.mg.vcycle(k.x_w, k.b_w, omega=self.omega, ndirect=self.ndirect,
+                    zero=True, f_src=k.r)
+        return k.x_w
 
     def _solve_instance(self, i):
+        # Flexible CG as NewtonMultigrid._pcg_iters: zo holds the latest
+        # preconditioned residual in FP64.
         k = self.classes[self.batch.sizes[i]]
         op, u = self.ops[i], self.u64[i]
         v, s = self.vec, k.sc
         op.jtf([u], k.r, -1.0)             # r = -J^T f(u)
         k.x[0].zero_()
-        self._precond(k)
-        k.p[0].copy_(k.z[0])
-        v.dot(k.r, k.z, s.p(0))
+        v.fcg_z(self._precond(k), k.r, k.zo, s.p(0), s.p(1), s.p(3))
+        k.p[0].copy_(k.zo[0])
         for _ in range(self.maxiter):
             k.op.normal_apply(k.state64, k.p, k.q, mode=0)
             v.dot(k.p, k.q, s.p(2))
-            v.axpy(k.x, k.p, s.p(0), s.p(2), 1.)
-            v.axpy(k.r, k.q, s.p(0), s.p(2), -1.)
-            k.zo[0].copy_(k.z[0])
-            self._precond(k)
-            v.dot(k.r, k.z, s.p(0), keep_prev=True)
-            v.dot(k.r, k.zo, s.p(3))
-            v.xpay(k.p, k.z, s.p(0), s.p(1), sub=s.p(3))
+            v.axpy2(k.x, k.p, k.r, k.q, s.p(0), s.p(2))
+            v.fcg_z(self._precond(k), k.r, k.zo, s.p(0), s.p(1), s.p(3))
+            v.xpay(k.p, k.zo, s.p(0), s.p(1), sub=s.p(3))
         u.add_(k.x[0])
 
     def _chunks(self):

@@ -149,13 +149,19 @@ __device__ inline bool shifted_inside(int i0, int i1, int i2, int p,
          ((unsigned)(i2 + ((p >> 20) & 1023) - 512) < (unsigned)nw.n2);
 }
 
-// Stage items [i0e, i0e + ne) of the global table into LDS (see above).
+// Stage items [i0e, i0e + ne) of the global table into LDS (see above),
+// flat offsets multiplied by `scale` (1: elements; sizeof(T): bytes, taken
+// modulo 2^32 so that base + (unsigned)(cell bytes + offset) addresses the
+// element).
 template <bool CONTIG>
 __device__ inline void stage_items(const int* __restrict__ items, int i0e,
-                                   int ne, int N, int2* fast, int2* shf) {
+                                   int ne, int N, int2* fast, int2* shf,
+                                   unsigned scale = 1) {
   for (int t = threadIdx.x; t < ne; t += blockDim.x) {
     const int* it = items + 10 * (i0e + t);
-    fast[t] = make_int2(it[0] * N + it[2], (CONTIG ? it[1] * N : 0) + it[3]);
+    fast[t] = make_int2(
+        (int)((unsigned)(it[0] * N + it[2]) * scale),
+        (int)((unsigned)((CONTIG ? it[1] * N : 0) + it[3]) * scale));
     shf[t] = make_int2(pack_shift(it[4], it[5], it[6]),
                        pack_shift(it[7], it[8], it[9]) |
                            (int)((unsigned)it[1] << 30));
@@ -171,7 +177,18 @@ __device__ inline const T* pick(const CVecPtrs<T>& x, int k) {
   return p;
 }
 
-template <class T, int LANES, bool CONTIG>
+// Element at `base` + `bytes` (a 32-bit unsigned byte offset): the load
+// takes the base from scalar registers and the offset from one vector
+// register, one add per address instead of a 64-bit multiply-add.
+template <class T>
+__device__ inline T ld_b32(const T* base, unsigned bytes) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                     bytes);
+}
+
+// B32: every coefficient and x offset fits in 32 bits as bytes (checked on
+// the host); the staged offsets are then byte offsets.
+template <class T, int LANES, bool CONTIG, bool B32>
 __global__ void __launch_bounds__(256)
     k_apply_sym(const int* __restrict__ items, const int* __restrict__ ostart,
                 const int* __restrict__ dent, const T* __restrict__ coef,
@@ -188,13 +205,19 @@ __global__ void __launch_bounds__(256)
   const int i0e = ostart[o];
   const int ne = ostart[o + 1] - i0e;
   const int N = (int)nw.size();
-  stage_items<CONTIG>(items, i0e, ne, N, fast, shf);
+  constexpr unsigned S = B32 ? sizeof(T) : 1;
+  stage_items<CONTIG>(items, i0e, ne, N, fast, shf, S);
   __syncthreads();
   const int lane = threadIdx.x % LANES;
   for (int r = 0; r < per; ++r) {
     const int c = (int)((((size_t)bx * per + r) * blockDim.x + threadIdx.x) /
                         LANES);
     if (c >= N) return;  // whole groups exit together; later tiles too
+    // Staged offset `off` of array `a` relative to this cell.
+    const unsigned cs = (unsigned)c * S;
+    auto at = [&](const T* a, int off) -> T {
+      return B32 ? ld_b32(a, cs + (unsigned)off) : a[(unsigned)(c + off)];
+    };
     int i0, i1, i2;
     unflat(c, nw, i0, i1, i2);
     const bool inner = i0 >= reach.n0 && i0 < nw.n0 - reach.n0 &&
@@ -209,14 +232,13 @@ __global__ void __launch_bounds__(256)
 #pragma unroll 16
         for (int e = lane; e < ne; e += LANES) {
           const int2 it = fast[e];
-          acc += coef[(unsigned)(c + it.x)] * xb[(unsigned)(c + it.y)];
+          acc += at(coef, it.x) * at(xb, it.y);
         }
       } else {
 #pragma unroll 4
         for (int e = lane; e < ne; e += LANES) {
           const int2 it = fast[e];
-          acc += coef[(unsigned)(c + it.x)] *
-                 pick(x, (unsigned)shf[e].y >> 30)[(unsigned)(c + it.y)];
+          acc += at(coef, it.x) * at(pick(x, (unsigned)shf[e].y >> 30), it.y);
         }
       }
     } else {
@@ -228,8 +250,10 @@ __global__ void __launch_bounds__(256)
         const bool ok = shifted_inside(i0, i1, i2, sh.x, nw) &
                         shifted_inside(i0, i1, i2, sh.y, nw);
         const T* xp = CONTIG ? x.p[0] : pick(x, (unsigned)sh.y >> 30);
-        const T a = coef[ok ? (unsigned)(c + it.x) : 0u];
-        const T b = xp[ok ? (unsigned)(c + it.y) : 0u];
+        const T a = B32 ? ld_b32(coef, ok ? cs + (unsigned)it.x : 0u)
+                        : coef[ok ? (unsigned)(c + it.x) : 0u];
+        const T b = B32 ? ld_b32(xp, ok ? cs + (unsigned)it.y : 0u)
+                        : xp[ok ? (unsigned)(c + it.y) : 0u];
         acc += ok ? a * b : T(0);
       }
     }
@@ -903,11 +927,19 @@ int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
     return check_launch("stencil_apply_sym");
   }
   dim3 grid(ceil_div((int)nblk(g.size() * lanes), per), nvar);
-#define ODIL_APPLY2(T, L, C)                                                 \
-  hipLaunchKernelGGL((k_apply_sym<T, L, C>), grid, dim3(256), 0, s, items,   \
-                     ostart, dent, static_cast<const T*>(coef),              \
-                     cvp<T>(x, nvar), vp<T>(y, nvar), cvp<T>(f, nvar), g,    \
+  // 32-bit byte offsets when the coefficient and x arrays are below 4 GiB.
+  const bool b32 = (size_t)nent * g.size() * esz < (1ull << 32) &&
+                   (size_t)nvar * g.size() * esz < (1ull << 32);
+#define ODIL_APPLY3(T, L, C, B)                                               \
+  hipLaunchKernelGGL((k_apply_sym<T, L, C, B>), grid, dim3(256), 0, s, items, \
+                     ostart, dent, static_cast<const T*>(coef),               \
+                     cvp<T>(x, nvar), vp<T>(y, nvar), cvp<T>(f, nvar), g,     \
                      rc, mode, (T)omega, per)
+#define ODIL_APPLY2(T, L, C)      \
+  if (b32)                        \
+    ODIL_APPLY3(T, L, C, true);   \
+  else                            \
+    ODIL_APPLY3(T, L, C, false)
 #define ODIL_APPLY(T, L)      \
   if (contig)                 \
     ODIL_APPLY2(T, L, true);  \
@@ -929,6 +961,7 @@ int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
 #undef ODIL_APPLY_L
 #undef ODIL_APPLY
 #undef ODIL_APPLY2
+#undef ODIL_APPLY3
   return check_launch("stencil_apply_sym");
 }
 

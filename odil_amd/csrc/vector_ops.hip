@@ -68,6 +68,77 @@ __global__ void __launch_bounds__(256)
       y.p[k][i] += a * x.p[k][i];
 }
 
+// x_k += (num / den) p_k and r_k -= (num / den) q_k in one pass: the two
+// conjugate-gradient updates sharing one step length (den == 0 -> none).
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_axpy2_ratio(VP<T> x, CVP<T> p, VP<T> r, CVP<T> q, int nvar, size_t N,
+                  const double* __restrict__ num,
+                  const double* __restrict__ den) {
+  const double d = *den;
+  const T a = d != 0.0 ? T(*num / d) : T(0);
+  const T an = d != 0.0 ? T(-*num / d) : T(0);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (int k = 0; k < nvar; ++k)
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += stride) {
+      x.p[k][i] += a * p.p[k][i];
+      r.p[k][i] += an * q.p[k][i];
+    }
+}
+
+// Flexible-CG pass after the preconditioner: z = zw converted to FP64,
+// per-block partial sums of r.z (partial[b]) and r.zo (partial[nb + b])
+// with zo the previous z, then zo = z. Same grid and summation order as
+// k_dot_partial.
+template <class Tw>
+__global__ void __launch_bounds__(256)
+    k_fcg_z(CVP<Tw> zw, CVP<double> r, VP<double> zo, int nvar, size_t N,
+            double* __restrict__ partial) {
+  __shared__ double red[4];
+  double s = 0, t = 0;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (int k = 0; k < nvar; ++k) {
+    const Tw* a = zw.p[k];
+    const double* b = r.p[k];
+    double* c = zo.p[k];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += stride) {
+      const double z = (double)a[i];
+      const double rv = b[i];
+      s += rv * z;
+      t += rv * c[i];
+      c[i] = z;
+    }
+  }
+  s = block_sum(s, red);
+  t = block_sum(t, red);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = s;
+    partial[gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// Stage 2 of k_fcg_z: *rz_prev = *rz, *rz = sum, *rzo = sum (fixed order).
+__global__ void __launch_bounds__(256)
+    k_fcg_final(const double* __restrict__ partial, int nb,
+                double* __restrict__ rz, double* __restrict__ rz_prev,
+                double* __restrict__ rzo) {
+  __shared__ double red[4];
+  double s = 0, t = 0;
+  for (int b = threadIdx.x; b < nb; b += 256) {
+    s += partial[b];
+    t += partial[nb + b];
+  }
+  s = block_sum(s, red);
+  t = block_sum(t, red);
+  if (threadIdx.x == 0) {
+    *rz_prev = *rz;
+    *rz = s;
+    *rzo = t;
+  }
+}
+
 // p_k = z_k + ((num - sub) / den) * p_k  (sub optional; den == 0 -> p = z).
 template <class T>
 __global__ void __launch_bounds__(256)
@@ -212,6 +283,58 @@ int odil_axpy_ratio(int dtype, int nvar, long long N, void* const* y,
     return kErrArg;
   }
   return check_launch("axpy_ratio");
+}
+
+int odil_axpy2_ratio(int dtype, int nvar, long long N, void* const* x,
+                     const void* const* p, void* const* r,
+                     const void* const* q, const double* num,
+                     const double* den, void* stream) {
+  if (nvar < 1 || nvar > kMaxVec || N < 0 || !num || !den) {
+    set_error("axpy2_ratio: bad arguments");
+    return kErrArg;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_axpy2_ratio<double>, dim3(grid_for(N)), dim3(256), 0,
+                       s, mkv<double>(x, nvar), mkc<double>(p, nvar),
+                       mkv<double>(r, nvar), mkc<double>(q, nvar), nvar,
+                       (size_t)N, num, den);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_axpy2_ratio<float>, dim3(grid_for(N)), dim3(256), 0,
+                       s, mkv<float>(x, nvar), mkc<float>(p, nvar),
+                       mkv<float>(r, nvar), mkc<float>(q, nvar), nvar,
+                       (size_t)N, num, den);
+  else {
+    set_error("axpy2_ratio: bad dtype");
+    return kErrArg;
+  }
+  return check_launch("axpy2_ratio");
+}
+
+int odil_fcg_z(int dtype_w, int nvar, long long N, const void* const* zw,
+               const void* const* r, void* const* zo, double* partial,
+               double* rz, double* rz_prev, double* rzo, void* stream) {
+  if (nvar < 1 || nvar > kMaxVec || N < 0 || !partial || !rz || !rz_prev ||
+      !rzo) {
+    set_error("fcg_z: bad arguments");
+    return kErrArg;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype_w == kF64)
+    hipLaunchKernelGGL(k_fcg_z<double>, dim3(kDotBlocks), dim3(256), 0, s,
+                       mkc<double>(zw, nvar), mkc<double>(r, nvar),
+                       mkv<double>(zo, nvar), nvar, (size_t)N, partial);
+  else if (dtype_w == kF32)
+    hipLaunchKernelGGL(k_fcg_z<float>, dim3(kDotBlocks), dim3(256), 0, s,
+                       mkc<float>(zw, nvar), mkc<double>(r, nvar),
+                       mkv<double>(zo, nvar), nvar, (size_t)N, partial);
+  else {
+    set_error("fcg_z: bad dtype");
+    return kErrArg;
+  }
+  hipLaunchKernelGGL(k_fcg_final, dim3(1), dim3(256), 0, s, partial,
+                     kDotBlocks, rz, rz_prev, rzo);
+  return check_launch("fcg_z");
 }
 
 // p = z + ((num - sub) / den) p; `sub` may be NULL (conjugate gradients:

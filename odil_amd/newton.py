@@ -75,6 +75,8 @@ class VectorOps:
     def __init__(self, device):
         self.partial = torch.zeros(_lib.lib().odil_dot_partials(),
                                    dtype=torch.float64, device=device)
+        self.partial2 = torch.zeros(2 * _lib.lib().odil_dot_partials(),
+                                    dtype=torch.float64, device=device)
 
     def dot(self, x, y, out_ptr, keep_prev=False):
         check(_lib.lib().odil_dot(c_int(_lib.dtype_tag(x[0].dtype)),
@@ -89,6 +91,21 @@ class VectorOps:
                                          ptrs(y), ptrs(x), num, den,
                                          c_double(sign), stream()),
               'axpy_ratio')
+
+    def axpy2(self, x, p, r, q, num, den):
+        """x += (num / den) p and r -= (num / den) q in one pass."""
+        check(_lib.lib().odil_axpy2_ratio(c_int(_lib.dtype_tag(x[0].dtype)),
+                                          c_int(len(x)), _ll(x[0].numel()),
+                                          ptrs(x), ptrs(p), ptrs(r), ptrs(q),
+                                          num, den, stream()), 'axpy2_ratio')
+
+    def fcg_z(self, zw, r, zo, rz, rz_prev, rzo):
+        """z = zw in FP64; rz_prev = rz, rz = r.z, rzo = r.zo; zo = z."""
+        check(_lib.lib().odil_fcg_z(c_int(_lib.dtype_tag(zw[0].dtype)),
+                                    c_int(len(zw)), _ll(zw[0].numel()),
+                                    ptrs(zw), ptrs(r), ptrs(zo),
+                                    c_vp(self.partial2.data_ptr()), rz,
+                                    rz_prev, rzo, stream()), 'fcg_z')
 
     def xpay(self, p, z, num, den, sub=None):
         """p = z + ((num - sub) / den) p with device scalars."""
@@ -210,18 +227,18 @@ class NewtonMultigrid:
         self._ready = True
 
     # -- linear solve on fixed buffers (capturable) ----------------------
-    def _precond(self, r, z):
-        """z = M r: one V-cycle from zero in the working precision (an FP32
-        cycle converts r in its first sweep)."""
+    def _precond(self, r):
+        """M r: one V-cycle from zero in the working precision (an FP32
+        cycle converts r in its first sweep). Returns the result in the
+        working precision (self.z or self.x_w)."""
         if self.wdtype == torch.float64:
-            self.mg.vcycle(z, r, omega=self.omega, ndirect=self.ndirect,
+            self.mg.vcycle(self.z, r, omega=self.omega, ndirect=self.ndirect,
                            pre=self.pre, post=self.post, zero=True)
-            return
+            return self.z
         self.mg.vcycle(self.x_w, self.b_w, omega=self.omega,
                        ndirect=self.ndirect, pre=self.pre, post=self.post,
                        zero=True, f_src=r)
-        for a, b in zip(z, self.x_w):
-            a.copy_(b)
+        return self.x_w
 
     def _apply_N(self, p, q):
         """q = N p in FP64 on the finest level."""
@@ -232,19 +249,17 @@ class NewtonMultigrid:
 
     def _pcg_iters(self, iters):
         # Scalars: s0 = r.z, s1 = previous r.z, s2 = p.N p, s3 = r.z_old,
-        # s4 = |b|^2, s6 = |r|^2.
+        # s4 = |b|^2, s6 = |r|^2. zo holds the latest preconditioned
+        # residual z in FP64 (fcg_z moves the new z there after using the
+        # old one for s3).
         v, s = self.vec, self.sc
         for _ in range(iters):
             self._apply_N(self.p, self.q)
             v.dot(self.p, self.q, s.p(2))
-            v.axpy(self.x, self.p, s.p(0), s.p(2), 1.)
-            v.axpy(self.r, self.q, s.p(0), s.p(2), -1.)
-            for a, b in zip(self.zo, self.z):
-                a.copy_(b)
-            self._precond(self.r, self.z)
-            v.dot(self.r, self.z, s.p(0), keep_prev=True)
-            v.dot(self.r, self.zo, s.p(3))
-            v.xpay(self.p, self.z, s.p(0), s.p(1), sub=s.p(3))
+            v.axpy2(self.x, self.p, self.r, self.q, s.p(0), s.p(2))
+            zw = self._precond(self.r)
+            v.fcg_z(zw, self.r, self.zo, s.p(0), s.p(1), s.p(3))
+            v.xpay(self.p, self.zo, s.p(0), s.p(1), sub=s.p(3))
         v.dot(self.r, self.r, s.p(6))
 
     def _solve_body(self, iters):
@@ -265,10 +280,11 @@ class NewtonMultigrid:
             a.copy_(b)
         for a in self.x:
             a.zero_()
-        self._precond(self.r, self.z)
-        for a, b in zip(self.p, self.z):
+        zw = self._precond(self.r)
+        # s0 = r.z and zo = z (s1, s3 are not used before being set again).
+        v.fcg_z(zw, self.r, self.zo, s.p(0), s.p(1), s.p(3))
+        for a, b in zip(self.p, self.zo):
             a.copy_(b)
-        v.dot(self.r, self.z, s.p(0))
         v.dot(self.rhs, self.rhs, s.p(4))
         self._pcg_iters(iters)
 
