@@ -291,7 +291,8 @@ int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
  * host int[3] bounds the offsets per axis (at most 511), max_items <= 512,
  * nent the coefficient array count, lanes the threads per cell (1, 2, 4, 8,
  * 16; 0 picks by grid size; -1: one thread per 16 bytes of consecutive cells
- * with vector loads, x in one allocation only), per the 256-thread tiles one
+ * with vector loads; -2: two interleaved cells per thread; x in one
+ * allocation only for -1 and -2), per the 256-thread tiles one
  * workgroup sweeps (1..64; the autotuner chooses lanes and per for each
  * operator shape). Fields of x in one allocation (x_k = x_0 + k N) take
  * faster paths. No reference counterpart for the storage: it applies the
@@ -336,6 +337,15 @@ int odil_stencil_galerkin(int dtype, int ndim, const int* nw, const int* oent,
 int odil_mg_restrict(int dtype, int ndim, const int* nw, int nvar,
                      const double* rw, int rmode, const void* const* fine,
                      void* const* coarse, void* stream);
+/* odil_mg_restrict followed by the coarse level's first weighted-Jacobi
+ * sweep from the zero initial guess (smoother_jacobi, multigrid.py:543-552,
+ * on the zeros of MultigridOp.step, multigrid.py:590-593):
+ * u_k = omega coarse_k / d_k with d_k the coarse operator's diagonal. */
+int odil_mg_restrict_smooth(int dtype, int ndim, const int* nw, int nvar,
+                            const double* rw, int rmode,
+                            const void* const* fine, void* const* coarse,
+                            const void* const* d, void* const* u,
+                            double omega, void* stream);
 /* fine += T coarse, multilinear interpolation (get_TT, multigrid.py:
  * 472-494, applied as in MultigridOp.step, multigrid.py:594-596). */
 int odil_mg_prolong_add(int dtype, int ndim, const int* nw, int nvar,

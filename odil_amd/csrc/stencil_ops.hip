@@ -188,14 +188,20 @@ __device__ inline T ld_b32(const T* base, unsigned bytes) {
 
 // B32: every coefficient and x offset fits in 32 bits as bytes (checked on
 // the host); the staged offsets are then byte offsets.
+// LANES == 1: the item index is uniform across the wavefront, so items are
+// read straight from the global table with scalar loads (scalar cache,
+// offsets formed in scalar ALU): no LDS staging and no barrier ahead of the
+// products. The epilogue operands (f, x, diagonal) are loaded before the
+// products so that their latency overlaps them.
 template <class T, int LANES, bool CONTIG, bool B32>
 __global__ void __launch_bounds__(256)
     k_apply_sym(const int* __restrict__ items, const int* __restrict__ ostart,
                 const int* __restrict__ dent, const T* __restrict__ coef,
                 CVecPtrs<T> x, VecPtrs<T> y, CVecPtrs<T> f, Grid3 nw,
                 Grid3 reach, int mode, T omega, int per) {
-  __shared__ int2 fast[kMaxItems];
-  __shared__ int2 shf[kMaxItems];
+  constexpr bool SCALAR = false;
+  __shared__ int2 fast[SCALAR ? 1 : kMaxItems];
+  __shared__ int2 shf[SCALAR ? 1 : kMaxItems];
   // Logical block: a chunk of `per` consecutive tiles, output field o
   // fastest (one XCD per chunk); the item list is staged once per chunk.
   const unsigned g = xcd_block(blockIdx.x + blockIdx.y * gridDim.x,
@@ -206,13 +212,52 @@ __global__ void __launch_bounds__(256)
   const int ne = ostart[o + 1] - i0e;
   const int N = (int)nw.size();
   constexpr unsigned S = B32 ? sizeof(T) : 1;
-  stage_items<CONTIG>(items, i0e, ne, N, fast, shf, S);
-  __syncthreads();
+  const int* __restrict__ itab = items + 10 * i0e;
+  if constexpr (!SCALAR) {
+    stage_items<CONTIG>(items, i0e, ne, N, fast, shf, S);
+    __syncthreads();
+  }
+  // Flat (scaled) offsets and packed shifts of item e.
+  auto item = [&](int e) -> int2 {
+    if constexpr (SCALAR) {
+      const int* it = itab + 10 * e;
+      return make_int2(
+          (int)((unsigned)(it[0] * N + it[2]) * S),
+          (int)((unsigned)((CONTIG ? it[1] * N : 0) + it[3]) * S));
+    } else {
+      return fast[e];
+    }
+  };
+  auto shifts = [&](int e) -> int2 {
+    if constexpr (SCALAR) {
+      const int* it = itab + 10 * e;
+      return make_int2(pack_shift(it[4], it[5], it[6]),
+                       pack_shift(it[7], it[8], it[9]) |
+                           (int)((unsigned)it[1] << 30));
+    } else {
+      return shf[e];
+    }
+  };
+  auto var_of = [&](int e) -> int {
+    if constexpr (SCALAR)
+      return itab[10 * e + 1];
+    else
+      return (int)((unsigned)shf[e].y >> 30);
+  };
   const int lane = threadIdx.x % LANES;
   for (int r = 0; r < per; ++r) {
     const int c = (int)((((size_t)bx * per + r) * blockDim.x + threadIdx.x) /
                         LANES);
     if (c >= N) return;  // whole groups exit together; later tiles too
+    // Epilogue operands first (their loads overlap the products).
+    T fo = 0, xo = 0, dd = 1;
+    if (lane == 0) {
+      if (mode >= 1) fo = f.p[o][c];
+      if (mode == 2) {
+        xo = x.p[o][c];
+        dd = coef[(size_t)dent[o] * N + c];
+      }
+    }
     // Staged offset `off` of array `a` relative to this cell.
     const unsigned cs = (unsigned)c * S;
     auto at = [&](const T* a, int off) -> T {
@@ -231,22 +276,22 @@ __global__ void __launch_bounds__(256)
         const T* __restrict__ xb = x.p[0];
 #pragma unroll 16
         for (int e = lane; e < ne; e += LANES) {
-          const int2 it = fast[e];
+          const int2 it = item(e);
           acc += at(coef, it.x) * at(xb, it.y);
         }
       } else {
 #pragma unroll 4
         for (int e = lane; e < ne; e += LANES) {
-          const int2 it = fast[e];
-          acc += at(coef, it.x) * at(pick(x, (unsigned)shf[e].y >> 30), it.y);
+          const int2 it = item(e);
+          acc += at(coef, it.x) * at(pick(x, var_of(e)), it.y);
         }
       }
     } else {
       // Cells near a face: products outside the grid masked to zero.
 #pragma unroll 4
       for (int e = lane; e < ne; e += LANES) {
-        const int2 it = fast[e];
-        const int2 sh = shf[e];
+        const int2 it = item(e);
+        const int2 sh = shifts(e);
         const bool ok = shifted_inside(i0, i1, i2, sh.x, nw) &
                         shifted_inside(i0, i1, i2, sh.y, nw);
         const T* xp = CONTIG ? x.p[0] : pick(x, (unsigned)sh.y >> 30);
@@ -264,12 +309,96 @@ __global__ void __launch_bounds__(256)
     if (mode == 0) {
       y.p[o][c] = acc;
     } else if (mode == 1) {
+      y.p[o][c] = fo - acc;
+    } else {
+      y.p[o][c] = xo + omega * (fo - acc) / dd;
+    }
+  }
+}
+
+// Two cells per thread for contiguous x (x_k = x_0 + k N), one thread per
+// cell otherwise as k_apply_sym: cells c and c + 256 (the same lane of two
+// consecutive tiles) with their products interleaved, so a wavefront keeps
+// twice the loads in flight and the grid has half the wavefronts (one
+// resident round where single cells would need two). A second cell past
+// the end of the grid mirrors the first and is not written.
+template <class T, bool B32>
+__global__ void __launch_bounds__(256)
+    k_apply_sym2(const int* __restrict__ items, const int* __restrict__ ostart,
+                 const int* __restrict__ dent, const T* __restrict__ coef,
+                 CVecPtrs<T> x, VecPtrs<T> y, CVecPtrs<T> f, Grid3 nw,
+                 Grid3 reach, int mode, T omega) {
+  __shared__ int2 fast[kMaxItems];
+  __shared__ int2 shf[kMaxItems];
+  const unsigned g = xcd_block(blockIdx.x + blockIdx.y * gridDim.x,
+                               gridDim.x * gridDim.y);
+  const int o = g % gridDim.y;
+  const unsigned bx = g / gridDim.y;
+  const int i0e = ostart[o];
+  const int ne = ostart[o + 1] - i0e;
+  const int N = (int)nw.size();
+  constexpr unsigned S = B32 ? sizeof(T) : 1;
+  stage_items<true>(items, i0e, ne, N, fast, shf, S);
+  __syncthreads();
+  const int c0 = (int)(bx * 2 * blockDim.x + threadIdx.x);
+  if (c0 >= N) return;
+  const bool has1 = c0 + (int)blockDim.x < N;
+  const int c1 = has1 ? c0 + (int)blockDim.x : c0;
+  const unsigned s0 = (unsigned)c0 * S, s1 = (unsigned)c1 * S;
+  auto at = [&](const T* a, unsigned cs, int c, int off) -> T {
+    return B32 ? ld_b32(a, cs + (unsigned)off) : a[(unsigned)(c + off)];
+  };
+  int a0, a1, a2, b0, b1, b2;
+  unflat(c0, nw, a0, a1, a2);
+  unflat(c1, nw, b0, b1, b2);
+  auto interior = [&](int i0, int i1, int i2) {
+    return i0 >= reach.n0 && i0 < nw.n0 - reach.n0 && i1 >= reach.n1 &&
+           i1 < nw.n1 - reach.n1 && i2 >= reach.n2 && i2 < nw.n2 - reach.n2;
+  };
+  const bool inner = interior(a0, a1, a2) && interior(b0, b1, b2);
+  const T* __restrict__ xb = x.p[0];
+  T acc0 = 0, acc1 = 0;
+  if (__all(inner)) {
+#pragma unroll 8
+    for (int e = 0; e < ne; ++e) {
+      const int2 it = fast[e];
+      acc0 += at(coef, s0, c0, it.x) * at(xb, s0, c0, it.y);
+      acc1 += at(coef, s1, c1, it.x) * at(xb, s1, c1, it.y);
+    }
+  } else {
+    // Near a face: products outside the grid masked to zero.
+#pragma unroll 2
+    for (int e = 0; e < ne; ++e) {
+      const int2 it = fast[e];
+      const int2 sh = shf[e];
+      const bool ok0 = shifted_inside(a0, a1, a2, sh.x, nw) &
+                       shifted_inside(a0, a1, a2, sh.y, nw);
+      const bool ok1 = shifted_inside(b0, b1, b2, sh.x, nw) &
+                       shifted_inside(b0, b1, b2, sh.y, nw);
+      const T p0 = B32 ? ld_b32(coef, ok0 ? s0 + (unsigned)it.x : 0u)
+                       : coef[ok0 ? (unsigned)(c0 + it.x) : 0u];
+      const T q0 = B32 ? ld_b32(xb, ok0 ? s0 + (unsigned)it.y : 0u)
+                       : xb[ok0 ? (unsigned)(c0 + it.y) : 0u];
+      const T p1 = B32 ? ld_b32(coef, ok1 ? s1 + (unsigned)it.x : 0u)
+                       : coef[ok1 ? (unsigned)(c1 + it.x) : 0u];
+      const T q1 = B32 ? ld_b32(xb, ok1 ? s1 + (unsigned)it.y : 0u)
+                       : xb[ok1 ? (unsigned)(c1 + it.y) : 0u];
+      acc0 += ok0 ? p0 * q0 : T(0);
+      acc1 += ok1 ? p1 * q1 : T(0);
+    }
+  }
+  auto out = [&](int c, T acc) {
+    if (mode == 0) {
+      y.p[o][c] = acc;
+    } else if (mode == 1) {
       y.p[o][c] = f.p[o][c] - acc;
     } else {
       const T d = coef[(size_t)dent[o] * N + c];
       y.p[o][c] = x.p[o][c] + omega * (f.p[o][c] - acc) / d;
     }
-  }
+  };
+  out(c0, acc0);
+  if (has1) out(c1, acc1);
 }
 
 // Vector form of k_apply_sym for contiguous x (x_k = x_0 + k N): each
@@ -526,11 +655,14 @@ __global__ void __launch_bounds__(256)
 // taps are fully unrolled with every field's load issued before the sums: a
 // tap with zero weight or outside the fine grid reads element 0 and adds an
 // exact zero (same sum order as skipping it), so no load waits on a branch.
+// With sm.p[0] != NULL, also the coarse level's first Jacobi sweep from a
+// zero iterate, u_k = omega coarse_k / d_k (d: the coarse diagonal), the
+// same rounding as odil_jacobi_zero on the stored coarse values.
 template <class T, int NV, int DIM>
 __global__ void __launch_bounds__(256)
     k_restrict(CVecPtrs<T> fine, VecPtrs<T> coarse,
                const double* __restrict__ rw, Grid3 nw, Grid3 nwh,
-               int rmode) {
+               int rmode, CVecPtrs<T> d, VecPtrs<T> sm, T omega) {
   const unsigned I = xcd_block(blockIdx.x, gridDim.x) * blockDim.x +
                      threadIdx.x;
   if (I >= nwh.size()) return;
@@ -570,6 +702,10 @@ __global__ void __launch_bounds__(256)
   }
 #pragma unroll
   for (int k = 0; k < NV; ++k) coarse.p[k][I] = acc[k];
+  if (sm.p[0] != nullptr) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sm.p[k][I] = omega * acc[k] / d.p[k][I];
+  }
 }
 
 // Prolongation with add: u_fine += T e_coarse, multilinear interpolation on
@@ -688,9 +824,13 @@ __global__ void __launch_bounds__(256)
   if (row >= n) return;
   double acc = 0;
   const double* m = M + (size_t)row * n;
-  for (int c = lane; c < n; c += 64) {
-    const int k = c / N;
-    acc += m[c] * (double)x.p[k][c - k * N];
+  // Field by field, unrolled: the row's loads are in flight together (a
+  // loop with a division per element waits on one load at a time).
+  for (int k = 0; k < nvar; ++k) {
+    const double* __restrict__ mk = m + (size_t)k * N;
+    const Tx* __restrict__ xk = x.p[k];
+#pragma unroll 8
+    for (int j = lane; j < N; j += 64) acc += mk[j] * (double)xk[j];
   }
   acc = wave_sum(acc);
   if (lane == 0) {
@@ -838,8 +978,9 @@ int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
 // k_apply_sym); `reach` (host, 3 ints) bounds |cs|, |xs| per axis;
 // `max_items` is the longest per-field list (at most 512); `nent` the number
 // of coefficient arrays; `lanes` threads per cell (1, 2, 4, 8, 16, or 0 for
-// a size-based default) or -1: one thread per 16 bytes of consecutive cells
-// with vector loads (x in one allocation only); `per` tiles of 256 threads
+// a size-based default), -1: one thread per 16 bytes of consecutive cells
+// with vector loads, or -2: two interleaved cells per thread (x in one
+// allocation only for -1 and -2); `per` tiles of 256 threads
 // per workgroup (1..64, scalar paths; the item list is staged once per
 // workgroup). x fields laid out as one allocation (x_k = x_0 + k N) take the
 // fast paths.
@@ -897,14 +1038,37 @@ int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
     lanes = 1;
     while (lanes < 16 && groups * lanes < (1u << 20)) lanes *= 2;
   }
-  if (lanes != -1 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 &&
-      lanes != 16) {
-    set_error("stencil_apply_sym: lanes must be -1, 0, 1, 2, 4, 8 or 16");
+  if (lanes != -2 && lanes != -1 && lanes != 1 && lanes != 2 && lanes != 4 &&
+      lanes != 8 && lanes != 16) {
+    set_error("stencil_apply_sym: lanes must be -2, -1, 0, 1, 2, 4, 8 or 16");
     return kErrArg;
   }
   if (dtype != kF64 && dtype != kF32) {
     set_error("stencil_apply_sym: bad dtype %d", dtype);
     return kErrArg;
+  }
+  // 32-bit byte offsets when the coefficient and x arrays are below 4 GiB.
+  const bool b32 = (size_t)nent * g.size() * esz < (1ull << 32) &&
+                   (size_t)nvar * g.size() * esz < (1ull << 32);
+  if (lanes == -2) {
+    if (!contig) {
+      set_error("stencil_apply_sym: two cells per thread needs x in one "
+                "allocation");
+      return kErrArg;
+    }
+    dim3 grid2((unsigned)ceil_div((int)nblk(g.size()), 2), nvar);
+#define ODIL_APPLY_2C(T, B)                                                  \
+  hipLaunchKernelGGL((k_apply_sym2<T, B>), grid2, dim3(256), 0, s, items,    \
+                     ostart, dent, static_cast<const T*>(coef),              \
+                     cvp<T>(x, nvar), vp<T>(y, nvar), cvp<T>(f, nvar), g,    \
+                     rc, mode, (T)omega)
+    if (dtype == kF64) {
+      if (b32) ODIL_APPLY_2C(double, true); else ODIL_APPLY_2C(double, false);
+    } else {
+      if (b32) ODIL_APPLY_2C(float, true); else ODIL_APPLY_2C(float, false);
+    }
+#undef ODIL_APPLY_2C
+    return check_launch("stencil_apply_sym");
   }
   if (lanes == -1) {
     if (!contig) {
@@ -927,9 +1091,6 @@ int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
     return check_launch("stencil_apply_sym");
   }
   dim3 grid(ceil_div((int)nblk(g.size() * lanes), per), nvar);
-  // 32-bit byte offsets when the coefficient and x arrays are below 4 GiB.
-  const bool b32 = (size_t)nent * g.size() * esz < (1ull << 32) &&
-                   (size_t)nvar * g.size() * esz < (1ull << 32);
 #define ODIL_APPLY3(T, L, C, B)                                               \
   hipLaunchKernelGGL((k_apply_sym<T, L, C, B>), grid, dim3(256), 0, s, items, \
                      ostart, dent, static_cast<const T*>(coef),               \
@@ -1076,22 +1237,24 @@ int odil_stencil_galerkin(int dtype, int ndim, const int* nw, const int* oent,
   return check_launch("stencil_galerkin");
 }
 
-int odil_mg_restrict(int dtype, int ndim, const int* nw, int nvar,
-                     const double* rw, int rmode, const void* const* fine,
-                     void* const* coarse, void* stream) {
-  if (bad_dim(ndim, nw, "mg_restrict") || bad_var(nvar, "mg_restrict"))
-    return kErrArg;
+static int mg_restrict(int dtype, int ndim, const int* nw, int nvar,
+                       const double* rw, int rmode, const void* const* fine,
+                       void* const* coarse, const void* const* d,
+                       void* const* u, double omega, hipStream_t s,
+                       const char* what) {
+  if (bad_dim(ndim, nw, what) || bad_var(nvar, what)) return kErrArg;
   Grid3 g = mk_grid(ndim, nw);
   Grid3 gh{g.n0 / 2 + 1, g.n1 / 2 + 1, ndim == 3 ? g.n2 / 2 + 1 : 1};
-  hipStream_t s = static_cast<hipStream_t>(stream);
   if (dtype != kF64 && dtype != kF32) {
-    set_error("mg_restrict: bad dtype %d", dtype);
+    set_error("%s: bad dtype %d", what, dtype);
     return kErrArg;
   }
-#define ODIL_RESTRICT(T, NV, D)                                              \
-  hipLaunchKernelGGL((k_restrict<T, NV, D>), dim3(nblk(gh.size())),          \
-                     dim3(256), 0, s, cvp<T>(fine, nvar), vp<T>(coarse, nvar), \
-                     rw, g, gh, rmode)
+#define ODIL_RESTRICT(T, NV, D)                                             \
+  hipLaunchKernelGGL((k_restrict<T, NV, D>), dim3(nblk(gh.size())),         \
+                     dim3(256), 0, s, cvp<T>(fine, nvar),                   \
+                     vp<T>(coarse, nvar), rw, g, gh, rmode,                 \
+                     d ? cvp<T>(d, nvar) : CVecPtrs<T>{},                   \
+                     u ? vp<T>(u, nvar) : VecPtrs<T>{}, (T)omega)
 #define ODIL_RESTRICT_NV(T, D)        \
   switch (nvar) {                     \
     case 1: ODIL_RESTRICT(T, 1, D); break; \
@@ -1106,7 +1269,31 @@ int odil_mg_restrict(int dtype, int ndim, const int* nw, int nvar,
   }
 #undef ODIL_RESTRICT_NV
 #undef ODIL_RESTRICT
-  return check_launch("mg_restrict");
+  return check_launch(what);
+}
+
+int odil_mg_restrict(int dtype, int ndim, const int* nw, int nvar,
+                     const double* rw, int rmode, const void* const* fine,
+                     void* const* coarse, void* stream) {
+  return mg_restrict(dtype, ndim, nw, nvar, rw, rmode, fine, coarse, nullptr,
+                     nullptr, 0.0, static_cast<hipStream_t>(stream),
+                     "mg_restrict");
+}
+
+// Restriction followed by the coarse level's first weighted-Jacobi sweep
+// from a zero iterate: coarse = R fine, u_k = omega coarse_k / d_k.
+int odil_mg_restrict_smooth(int dtype, int ndim, const int* nw, int nvar,
+                            const double* rw, int rmode,
+                            const void* const* fine, void* const* coarse,
+                            const void* const* d, void* const* u,
+                            double omega, void* stream) {
+  if (!d || !u) {
+    set_error("mg_restrict_smooth: missing diagonal or output");
+    return kErrArg;
+  }
+  return mg_restrict(dtype, ndim, nw, nvar, rw, rmode, fine, coarse, d, u,
+                     omega, static_cast<hipStream_t>(stream),
+                     "mg_restrict_smooth");
 }
 
 int odil_mg_prolong_add(int dtype, int ndim, const int* nw, int nvar,

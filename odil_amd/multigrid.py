@@ -179,7 +179,7 @@ class BlockStencil:
                 _grid_arr(self.nw), c_int(self.nvar_out), ptr(self.t_items),
                 ptr(self.t_iostart), ptr(self.t_dent), self._reach,
                 c_int(self._max_items), c_int(len(self.entries)),
-                c_int(self.lanes), c_int(self.per if self.lanes != -1 else 1),
+                c_int(self.lanes), c_int(self.per if self.lanes >= 0 else 1),
                 ptr(self.coef), ptrs(x), ptrs(y),
                 ptrs(f) if f is not None else None, c_int(mode),
                 c_double(omega), stream())
@@ -218,16 +218,19 @@ class BlockStencil:
         self._call_apply(x, y, f, 2, omega)
         return y
 
+    def diagonal(self):
+        """Views of the diagonal coefficient array of each field."""
+        if np.any(self._dent_host < 0):
+            raise ValueError('Jacobi needs a diagonal entry in every block')
+        return [self.coef[int(q)] for q in self._dent_host]
+
     def jacobi_zero(self, f, y, omega, f_src=None):
         """The Jacobi sweep from x = 0: y = omega f / diag(A), one pass
         without the operator product. With `f_src` (same shape, e.g. the
         FP64 vector a working-precision V-cycle starts from), reads f_src
         and stores its conversion into f."""
-        if np.any(self._dent_host < 0):
-            raise ValueError('Jacobi needs a diagonal entry in every block')
-        d = [self.coef[int(q)] for q in self._dent_host]
-        return jacobi_zero(f if f_src is None else f_src, d, y, omega,
-                           fw=None if f_src is None else f)
+        return jacobi_zero(f if f_src is None else f_src, self.diagonal(), y,
+                           omega, fw=None if f_src is None else f)
 
     def tune(self, x, f, y, omega=0.5):
         """Pick the threads per cell of the symmetric sweep by timing the
@@ -246,7 +249,7 @@ class BlockStencil:
             self._call_apply(x, y, f, 2, omega)
         cand = [(L, p) for L in (0, 1, 2, 4, 8, 16) for p in (1, 2, 4, 8)]
         if contig:
-            cand.append((-1, 1))
+            cand += [(-1, 1), (-2, 1)]
         self.lanes, self.per = autotune.pick(key, cand, run)
         return self
 
@@ -558,6 +561,20 @@ def restrict(fine, nw, rw_dev, coarse=None, restriction='full'):
     return coarse
 
 
+def restrict_smooth(fine, nw, rw_dev, coarse, d, u, omega,
+                    restriction='full'):
+    """coarse = R fine, then the coarse level's first Jacobi sweep from a
+    zero iterate, u = omega coarse / d (d: the coarse diagonal), in one
+    launch."""
+    st = _lib.lib().odil_mg_restrict_smooth(
+        c_int(_lib.dtype_tag(fine[0].dtype)), c_int(len(nw)), _grid_arr(nw),
+        c_int(len(fine)), ptr(rw_dev), c_int(_rmode(restriction)),
+        ptrs(fine), ptrs(coarse), ptrs(d), ptrs(u), c_double(omega),
+        stream())
+    check(st, 'mg_restrict_smooth')
+    return coarse
+
+
 def prolong_add(coarse, fine, nw):
     """fine += T coarse (multilinear interpolation, node grids)."""
     st = _lib.lib().odil_mg_prolong_add(
@@ -776,11 +793,15 @@ class MultigridOp:
         self._vc(0, u, f, buf[0]['v'], omega, ndirect, pre, post, zero, f_src)
         return u
 
+    def _is_direct(self, level, ndirect):
+        return level + 1 >= self.nlevels or min(self.nnw[level]) <= ndirect
+
     def _vc(self, level, u, f, spare, omega, ndirect, pre, post, zero=False,
-            f_src=None):
+            f_src=None, first_done=False):
         # `u` holds the iterate (ignored with zero=True); `spare` is a
-        # same-shape scratch. Returns u.
-        if level + 1 >= self.nlevels or min(self.nnw[level]) <= ndirect:
+        # same-shape scratch. first_done (with zero=True): the first sweep
+        # from zero is already in `spare`. Returns u.
+        if self._is_direct(level, ndirect):
             Minv = self._coarse(level)
             N = int(np.prod(self.nnw[level]))
             dt = _lib.dtype_tag(f[0].dtype)
@@ -801,7 +822,8 @@ class MultigridOp:
         cur, oth = u, spare
         k0 = 0
         if zero and pre > 0:
-            A.jacobi_zero(f, oth, omega, f_src=f_src)
+            if not first_done:
+                A.jacobi_zero(f, oth, omega, f_src=f_src)
             cur, oth = oth, cur
             k0 = 1
         elif zero:
@@ -813,10 +835,18 @@ class MultigridOp:
         r = buf[level]['r']
         A.residual(cur, f, r)
         nxt = buf[level + 1]
-        restrict(r, self.nnw[level], self.rw, nxt['f'],
-                 restriction=self.restriction)
+        # The coarse cycle starts from zero (pre = 2 there): unless it is
+        # the direct solve, the restriction also does its first sweep.
+        fused = not self._is_direct(level + 1, 5)
+        if fused:
+            restrict_smooth(r, self.nnw[level], self.rw, nxt['f'],
+                            self.AA[level + 1].diagonal(), nxt['v'], omega,
+                            restriction=self.restriction)
+        else:
+            restrict(r, self.nnw[level], self.rw, nxt['f'],
+                     restriction=self.restriction)
         res = self._vc(level + 1, nxt['u'], nxt['f'], nxt['v'], omega, 5, 2,
-                       2, zero=True)
+                       2, zero=True, first_done=fused)
         prolong_add(res, cur, self.nnw[level])
         for _ in range(post):
             A.jacobi(cur, f, oth, omega)

@@ -348,11 +348,12 @@ def test_sym_apply(dev, nw, nvar, dtype):
         want = xn[k] + 0.6 * (fn[k] - ref[k]) / d
         assert np.abs(y[k].double().cpu().numpy() - want).max() <= tol * sc
     # every launch shape on contiguous x (one allocation), incl. the vector
-    # path (-1: 16 bytes of consecutive cells per thread)
+    # path (-1: 16 bytes of consecutive cells per thread) and two
+    # interleaved cells per thread (-2)
     xc = list(torch.tensor(np.stack(xn), dtype=dtype, device=dev).unbind(0))
     fc = list(torch.tensor(np.stack(fn), dtype=dtype, device=dev).unbind(0))
-    for lanes, per in [(-1, 1)] + [(L, p) for L in (1, 2, 4, 8, 16)
-                                    for p in (1, 3)]:
+    for lanes, per in [(-1, 1), (-2, 1)] + [(L, p) for L in (1, 2, 4, 8, 16)
+                                             for p in (1, 3)]:
         A.lanes, A.per = lanes, per
         for a, r in zip(A.apply(xc), ref):
             assert np.abs(a.double().cpu().numpy() - r).max() <= tol * sc
@@ -452,3 +453,23 @@ def test_vcycle_zero_start(dev, dtype):
     for a, b, c, d in zip(ref, got2, fw, f):
         assert torch.equal(a, b)
         assert torch.equal(c, d)
+
+
+@pytest.mark.parametrize('nw', [(17, 33), (9, 9, 17)])
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_restrict_smooth_matches_two_passes(dev, nw, dtype):
+    """The fused restriction + first coarse sweep equals restriction
+    followed by the zero-start sweep, bit for bit."""
+    from odil_amd.multigrid import (jacobi_zero, restrict, restrict_smooth,
+                                    restriction_weights)
+    rw = torch.tensor(restriction_weights('transpose', len(nw)), device=dev)
+    x = _vec(dev, dtype, 3, nw, 11)
+    nwh = tuple(n // 2 + 1 for n in nw)
+    d = [a.abs() + 0.5 for a in _vec(dev, dtype, 3, nwh, 12)]
+    c1 = restrict(x, nw, rw, restriction='transpose')
+    u1 = jacobi_zero(c1, d, [torch.empty_like(a) for a in c1], 0.6)
+    c2 = [torch.empty_like(a) for a in c1]
+    u2 = [torch.empty_like(a) for a in c1]
+    restrict_smooth(x, nw, rw, c2, d, u2, 0.6, restriction='transpose')
+    for a, b in zip(c1 + u1, c2 + u2):
+        assert torch.equal(a, b)
