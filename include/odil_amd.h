@@ -405,12 +405,14 @@ int odil_axpy2_ratio(int dtype, int nvar, long long N, void* const* x,
                      const void* const* q, const double* num,
                      const double* den, void* stream);
 /* Flexible-CG pass after the preconditioner, one read of each vector:
- * z = zw (FP64 or FP32, converted to FP64); *rz_prev = *rz; *rz = r.z;
- * *rzo = r.zo with zo the previous z; then zo = z. `partial` holds
- * 2 * odil_dot_partials() FP64 values; r and zo are FP64. */
-int odil_fcg_z(int dtype_w, int nvar, long long N, const void* const* zw,
-               const void* const* r, void* const* zo, double* partial,
-               double* rz, double* rz_prev, double* rzo, void* stream);
+ * z = zw converted to dtype (the Krylov precision of r and zo);
+ * *rz_prev = *rz; *rz = r.z; *rzo = r.zo with zo the previous z (sums in
+ * FP64); then zo = z. `partial` holds 2 * odil_dot_partials() FP64 values.
+ * (dtype_w, dtype): FP64/FP64, FP32/FP64, FP32/FP32. */
+int odil_fcg_z(int dtype_w, int dtype, int nvar, long long N,
+               const void* const* zw, const void* const* r, void* const* zo,
+               double* partial, double* rz, double* rz_prev, double* rzo,
+               void* stream);
 /* p_k = z_k + ((num - sub) / den) p_k (sub may be NULL; den == 0: p = z). */
 int odil_xpay_ratio(int dtype, int nvar, long long N, void* const* p,
                     const void* const* z, const double* num,

@@ -87,27 +87,27 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// Flexible-CG pass after the preconditioner: z = zw converted to FP64,
-// per-block partial sums of r.z (partial[b]) and r.zo (partial[nb + b])
-// with zo the previous z, then zo = z. Same grid and summation order as
-// k_dot_partial.
-template <class Tw>
+// Flexible-CG pass after the preconditioner: z = zw converted to the
+// Krylov precision T, per-block FP64 partial sums of r.z (partial[b]) and
+// r.zo (partial[nb + b]) with zo the previous z, then zo = z. Same grid and
+// summation order as k_dot_partial.
+template <class Tw, class T>
 __global__ void __launch_bounds__(256)
-    k_fcg_z(CVP<Tw> zw, CVP<double> r, VP<double> zo, int nvar, size_t N,
+    k_fcg_z(CVP<Tw> zw, CVP<T> r, VP<T> zo, int nvar, size_t N,
             double* __restrict__ partial) {
   __shared__ double red[4];
   double s = 0, t = 0;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (int k = 0; k < nvar; ++k) {
     const Tw* a = zw.p[k];
-    const double* b = r.p[k];
-    double* c = zo.p[k];
+    const T* b = r.p[k];
+    T* c = zo.p[k];
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += stride) {
-      const double z = (double)a[i];
-      const double rv = b[i];
-      s += rv * z;
-      t += rv * c[i];
+      const T z = (T)a[i];
+      const double rv = (double)b[i];
+      s += rv * (double)z;
+      t += rv * (double)c[i];
       c[i] = z;
     }
   }
@@ -311,27 +311,31 @@ int odil_axpy2_ratio(int dtype, int nvar, long long N, void* const* x,
   return check_launch("axpy2_ratio");
 }
 
-int odil_fcg_z(int dtype_w, int nvar, long long N, const void* const* zw,
-               const void* const* r, void* const* zo, double* partial,
-               double* rz, double* rz_prev, double* rzo, void* stream) {
+int odil_fcg_z(int dtype_w, int dtype, int nvar, long long N,
+               const void* const* zw, const void* const* r, void* const* zo,
+               double* partial, double* rz, double* rz_prev, double* rzo,
+               void* stream) {
   if (nvar < 1 || nvar > kMaxVec || N < 0 || !partial || !rz || !rz_prev ||
       !rzo) {
     set_error("fcg_z: bad arguments");
     return kErrArg;
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (dtype_w == kF64)
-    hipLaunchKernelGGL(k_fcg_z<double>, dim3(kDotBlocks), dim3(256), 0, s,
-                       mkc<double>(zw, nvar), mkc<double>(r, nvar),
-                       mkv<double>(zo, nvar), nvar, (size_t)N, partial);
-  else if (dtype_w == kF32)
-    hipLaunchKernelGGL(k_fcg_z<float>, dim3(kDotBlocks), dim3(256), 0, s,
-                       mkc<float>(zw, nvar), mkc<double>(r, nvar),
-                       mkv<double>(zo, nvar), nvar, (size_t)N, partial);
+#define ODIL_FCG_Z(TW, T)                                                    \
+  hipLaunchKernelGGL((k_fcg_z<TW, T>), dim3(kDotBlocks), dim3(256), 0, s,    \
+                     mkc<TW>(zw, nvar), mkc<T>(r, nvar), mkv<T>(zo, nvar),   \
+                     nvar, (size_t)N, partial)
+  if (dtype_w == kF64 && dtype == kF64)
+    ODIL_FCG_Z(double, double);
+  else if (dtype_w == kF32 && dtype == kF64)
+    ODIL_FCG_Z(float, double);
+  else if (dtype_w == kF32 && dtype == kF32)
+    ODIL_FCG_Z(float, float);
   else {
-    set_error("fcg_z: bad dtype");
+    set_error("fcg_z: unsupported dtypes %d, %d", dtype_w, dtype);
     return kErrArg;
   }
+#undef ODIL_FCG_Z
   hipLaunchKernelGGL(k_fcg_final, dim3(1), dim3(256), 0, s, partial,
                      kDotBlocks, rz, rz_prev, rzo);
   return check_launch("fcg_z");

@@ -102,6 +102,7 @@ class VectorOps:
     def fcg_z(self, zw, r, zo, rz, rz_prev, rzo):
         """z = zw in FP64; rz_prev = rz, rz = r.z, rzo = r.zo; zo = z."""
         check(_lib.lib().odil_fcg_z(c_int(_lib.dtype_tag(zw[0].dtype)),
+                                    c_int(_lib.dtype_tag(r[0].dtype)),
                                     c_int(len(zw)), _ll(zw[0].numel()),
                                     ptrs(zw), ptrs(r), ptrs(zo),
                                     c_vp(self.partial2.data_ptr()), rz,
@@ -155,15 +156,30 @@ class NewtonMultigrid:
                  galerkin_mode='exact', precision='fp64', linsolver='pcg',
                  rtol=0., check_every=10, coarse_reg='auto',
                  matrix_free=True, use_graph=True, setup_every=1,
-                 device=None):
+                 krylov='auto', refine=0, device=None):
         if precision not in ('fp64', 'mixed'):
             raise ValueError('precision must be fp64 or mixed')
         if linsolver not in ('vcycle', 'pcg'):
             raise ValueError('linsolver must be vcycle or pcg')
+        if krylov not in ('auto', 'fp64', 'fp32'):
+            raise ValueError('krylov must be auto, fp64 or fp32')
         self.op = op
         self.device = device or torch.device('cuda')
         self.wdtype = torch.float64 if precision == 'fp64' else torch.float32
         self.precision = precision
+        # Krylov precision: 'auto' is that of the preconditioner. An FP32
+        # flexible CG solves the correction equation of each FP64 Newton
+        # residual; `refine` extra passes each recompute the FP64 linear
+        # residual b - N x and add an FP32 solve for it (iterative
+        # refinement).
+        if krylov == 'auto':
+            krylov = 'fp64' if precision == 'fp64' else 'fp32'
+        if krylov == 'fp32' and precision == 'fp64':
+            raise ValueError('an FP32 Krylov solve needs precision=mixed')
+        self.kdtype = torch.float64 if krylov == 'fp64' else torch.float32
+        self.refine = max(0, int(refine))
+        if self.refine and self.kdtype == torch.float64:
+            raise ValueError('refine needs krylov=fp32')
         self.linsolver = linsolver
         self.maxiter = int(maxiter)
         self.rtol = float(rtol)
@@ -220,18 +236,23 @@ class NewtonMultigrid:
         self.b_w = self._vecs(self.wdtype)
         self.x_w = self._vecs(self.wdtype)
         if self.linsolver == 'pcg':
+            kd = self.kdtype
             self.r, self.z, self.p, self.q, self.zo = (
-                self._vecs(f64), self._vecs(f64), self._vecs(f64),
-                self._vecs(f64), self._vecs(f64))
+                self._vecs(kd), self._vecs(kd), self._vecs(kd),
+                self._vecs(kd), self._vecs(kd))
+            # Krylov solution (the FP64 solution itself for an FP64 solve)
+            # and the FP64 linear residual of the refinement passes.
+            self.xk = self.x if kd == f64 else self._vecs(kd)
+            self.r64 = self._vecs(f64) if self.refine else None
         self._fields = fields
         self._ready = True
 
     # -- linear solve on fixed buffers (capturable) ----------------------
     def _precond(self, r):
         """M r: one V-cycle from zero in the working precision (an FP32
-        cycle converts r in its first sweep). Returns the result in the
-        working precision (self.z or self.x_w)."""
-        if self.wdtype == torch.float64:
+        cycle converts an FP64 r in its first sweep). Returns the result in
+        the working precision (self.z or self.x_w)."""
+        if self.wdtype == r[0].dtype:
             self.mg.vcycle(self.z, r, omega=self.omega, ndirect=self.ndirect,
                            pre=self.pre, post=self.post, zero=True)
             return self.z
@@ -241,8 +262,14 @@ class NewtonMultigrid:
         return self.x_w
 
     def _apply_N(self, p, q):
-        """q = N p in FP64 on the finest level."""
-        if self.matrix_free:
+        """q = N p on the finest level in the Krylov precision (an FP32
+        solve applies N matrix-free from the FP32 copy of the state)."""
+        if self.kdtype == torch.float32:
+            if self.matrix_free:
+                self.op.normal_apply(self.state_w, p, q, mode=0)
+            else:
+                self.N.apply(p, q)
+        elif self.matrix_free:
             self.op.normal_apply(self._fields, p, q, mode=0)
         else:
             self._N64.apply(p, q)
@@ -256,13 +283,13 @@ class NewtonMultigrid:
         for _ in range(iters):
             self._apply_N(self.p, self.q)
             v.dot(self.p, self.q, s.p(2))
-            v.axpy2(self.x, self.p, self.r, self.q, s.p(0), s.p(2))
+            v.axpy2(self.xk, self.p, self.r, self.q, s.p(0), s.p(2))
             zw = self._precond(self.r)
             v.fcg_z(zw, self.r, self.zo, s.p(0), s.p(1), s.p(3))
             v.xpay(self.p, self.zo, s.p(0), s.p(1), sub=s.p(3))
         v.dot(self.r, self.r, s.p(6))
 
-    def _solve_body(self, iters):
+    def _solve_body(self, iters, finish=True):
         if self.linsolver == 'vcycle':
             for a, b in zip(self.b_w, self.rhs):
                 a.copy_(b)
@@ -275,50 +302,76 @@ class NewtonMultigrid:
             for a, b in zip(self.x, self.x_w):
                 a.copy_(b)
             return
+        self._fcg_pass(self.rhs, iters)
+        if finish:
+            self._finish()
+
+    def _finish(self):
+        """x = xk in FP64, then the refinement passes."""
+        if self.xk is not self.x:
+            for a, b in zip(self.x, self.xk):
+                a.copy_(b)
+        iters = self.maxiter
+        for _ in range(self.refine):
+            # FP64 residual of the current solution, solved again in FP32.
+            self.op.normal_apply(self._fields, self.x, self.r64, b=self.rhs,
+                                 mode=1)
+            self._fcg_pass(self.r64, iters)
+            for a, b in zip(self.x, self.xk):
+                a.add_(b)
+
+    def _fcg_pass(self, b, iters):
+        """Flexible CG from x = 0 on N x = b (b FP64, converted to the
+        Krylov precision): xk holds the result."""
         v, s = self.vec, self.sc
-        for a, b in zip(self.r, self.rhs):
-            a.copy_(b)
-        for a in self.x:
+        for a, c in zip(self.r, b):
+            a.copy_(c)
+        for a in self.xk:
             a.zero_()
         zw = self._precond(self.r)
         # s0 = r.z and zo = z (s1, s3 are not used before being set again).
         v.fcg_z(zw, self.r, self.zo, s.p(0), s.p(1), s.p(3))
-        for a, b in zip(self.p, self.zo):
-            a.copy_(b)
-        v.dot(self.rhs, self.rhs, s.p(4))
+        for a, c in zip(self.p, self.zo):
+            a.copy_(c)
+        v.dot(b, b, s.p(4))
         self._pcg_iters(iters)
 
-    def _run_chunk(self, iters, fresh=True):
+    def _run_chunk(self, iters, fresh=True, finish=True):
         if not fresh:
             self._pcg_iters(iters)
             return
+        key = (iters, finish)
         if not self.use_graph:
-            self._solve_body(iters)
+            self._solve_body(iters, finish)
             return
-        if iters not in self._graph:
+        if key not in self._graph:
             # First use: run eagerly (the result stays in x), then record the
             # same work; capture does not execute it.
-            self._solve_body(iters)
+            self._solve_body(iters, finish)
             torch.cuda.synchronize()
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                self._solve_body(iters)
-            self._graph[iters] = g
+                self._solve_body(iters, finish)
+            self._graph[key] = g
             return
-        self._graph[iters].replay()
+        self._graph[key].replay()
 
     def _linear_solve(self):
+        """Krylov iterations run (refinement passes included)."""
         if self.linsolver == 'vcycle' or self.rtol <= 0:
             self._run_chunk(self.maxiter)
-            return self.maxiter
+            return self.maxiter * (1 + self.refine)
         done = 0
         while done < self.maxiter:
             k = min(self.check_every, self.maxiter - done)
-            self._run_chunk(k, fresh=(done == 0))
+            self._run_chunk(k, fresh=(done == 0), finish=False)
             done += k
             rr, bb = self.sc.t[6].item(), self.sc.t[4].item()
             if bb == 0 or (rr / bb)**0.5 <= self.rtol:
                 break
+        if self.xk is not self.x:
+            for a, b in zip(self.x, self.xk):
+                a.copy_(b)
         return done
 
     # -- Newton ------------------------------------------------------------
