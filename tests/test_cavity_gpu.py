@@ -128,10 +128,15 @@ def oracle_cavity65():
     return tape.unpack_fields(dom, packed)
 
 
-@pytest.mark.parametrize('precision,tol,rtol', [('fp64', 1e-11, 1e-8),
-                                                ('mixed', 1e-9, 1e-6)])
+@pytest.mark.parametrize('precision,tol,rtol,kw', [
+    ('fp64', 1e-11, 1e-8, {}),
+    ('mixed', 1e-9, 1e-6, {}),                      # FP32 Krylov
+    ('mixed', 1e-9, 1e-6, {'krylov': 'fp64'}),      # FP64 Krylov, FP32 MG
+    ('mixed', 1e-9, 1e-6, {'refine': 1}),           # + FP64 refinement
+    ('mixed', 1e-9, 1e-6, {'setup_every': 4}),      # lagged hierarchy
+])
 def test_newton_multigrid_matches_oracle_direct(dev, oracle_cavity65,
-                                                precision, tol, rtol):
+                                                precision, tol, rtol, kw):
     """The GPU Newton-multigrid solve (FCG on J^T J, V(2,2) preconditioner,
     5 levels) reaches the same converged cavity solution as the oracle's
     direct Newton; pressure is compared up to its constant (null mode)."""
@@ -139,7 +144,7 @@ def test_newton_multigrid_matches_oracle_direct(dev, oracle_cavity65,
     from odil_amd.operators import Cavity
     n = 65
     op = Cavity((n, n), 0.01)
-    nm = NewtonMultigrid(op, nlvl=5, maxiter=10, precision=precision)
+    nm = NewtonMultigrid(op, nlvl=5, maxiter=10, precision=precision, **kw)
     fields = [torch.zeros(n, n, dtype=torch.float64, device=dev)
               for _ in range(3)]
     hist = nm.run(fields, 80, tol=tol)
@@ -149,3 +154,16 @@ def test_newton_multigrid_matches_oracle_direct(dev, oracle_cavity65,
         _close(fields[k].cpu().numpy(), ref[name], rtol)
     p = fields[2].cpu().numpy()
     _close(p - p.mean(), ref['p'] - ref['p'].mean(), rtol)
+
+
+def test_newton_multigrid_options_rejected(dev):
+    from odil_amd.newton import NewtonMultigrid
+    from odil_amd.operators import Cavity
+    op = Cavity((17, 17), 0.01)
+    with pytest.raises(ValueError):
+        NewtonMultigrid(op, nlvl=3, precision='fp64', krylov='fp32')
+    with pytest.raises(ValueError):
+        NewtonMultigrid(op, nlvl=3, precision='mixed', krylov='fp64',
+                        refine=1)
+    with pytest.raises(ValueError):
+        NewtonMultigrid(op, nlvl=3, precision='mixed', krylov='bf16')
