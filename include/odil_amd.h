@@ -377,6 +377,18 @@ int odil_stencil_to_csr(int dtype, int ndim, const int* nw, int nout,
                         const int* ent, const int* ostart, int nent,
                         const void* coef, int* rowptr, int* col, void* val,
                         void* stream);
+/* One diagonal block of a right-looking blocked Cholesky factorisation of a
+ * dense SPD FP64 matrix A (row-major, leading dimension lda, already
+ * updated by the previous panels): A[k0:k0+nb, k0:k0+nb] = L L^T with L
+ * written back (strict upper part of the block zeroed) and W = L^-1
+ * (row-major, leading dimension odil_chol_block_size()); nb <= that size.
+ * The panel and trailing updates are GEMMs of the caller. It replaces the
+ * library factorisation of the coarsest level's Tikhonov solve
+ * (smoother_direct, multigrid.py:554-565, solves the coarsest system with
+ * a dense factorisation). */
+int odil_chol_block_size(void);
+int odil_chol_block(double* A, int lda, int k0, int nb, double* W,
+                    void* stream);
 /* y = M x with M dense FP64 [nvar N][nvar N] (the coarsest-level solve as
  * a precomputed solution operator). */
 int odil_dense_gemv(int dtype_x, int dtype_y, int nvar, int N,

@@ -577,13 +577,17 @@ __global__ void __launch_bounds__(256)
   int m0, m1, m2;
   unflat(m, g, m0, m1, m2);
   double acc = 0;
+  // A term whose cell falls outside reads element 0 and adds an exact zero
+  // (same order as skipping it): the loads are not behind branches.
+#pragma unroll 8
   for (int q = tstart[o]; q < tstart[o + 1]; ++q) {
     const int q0 = m0 - terms[5 * q + 2], q1 = m1 - terms[5 * q + 3],
               q2 = m2 - terms[5 * q + 4];
-    if (!inside(q0, q1, q2, g)) continue;
+    const bool ok = inside(q0, q1, q2, g);
     const size_t n = ((size_t)q0 * g.n1 + q1) * g.n2 + q2;
-    acc += (double)jc[(size_t)terms[5 * q] * N + n] *
-           (double)jc[(size_t)terms[5 * q + 1] * N + n];
+    const double x = (double)jc[ok ? (size_t)terms[5 * q] * N + n : 0];
+    const double y = (double)jc[ok ? (size_t)terms[5 * q + 1] * N + n : 0];
+    acc += ok ? x * y : 0.0;
   }
   out[(size_t)o * N + m] = (Tout)acc;
 }
@@ -629,23 +633,25 @@ __global__ void __launch_bounds__(256)
             J2 = I2 + sigma * oent[5 * o + 4];
   double acc = 0;
   if (inside(J0, J1, J2, nwh)) {
+    // Terms that drop out (zero weight, fine point outside the grid) read
+    // element 0 and add an exact zero, in the same order: no load waits on
+    // a branch.
+    const bool wfull = inner || rmode == 1;
+#pragma unroll 8
     for (int t = tstart[o]; t < tstart[o + 1]; ++t) {
       const int* tm = terms + 9 * t;
       const int r0 = tm[1], r1 = tm[2], r2 = tm[3];
-      double w;
-      if (inner || rmode == 1)
-        w = rw[(r0 + 1) * 9 + (r1 + 1) * 3 + (r2 + 1)];
-      else
-        w = (r0 == 0 && r1 == 0 && r2 == 0) ? 1.0 : 0.0;
-      if (w == 0.0) continue;
+      const double w = wfull ? rw[(r0 + 1) * 9 + (r1 + 1) * 3 + (r2 + 1)]
+                             : ((r0 == 0 && r1 == 0 && r2 == 0) ? 1.0 : 0.0);
       const int a0 = 2 * I0 + r0, a1 = 2 * I1 + r1, a2 = 2 * I2 + r2;
-      if (!inside(a0, a1, a2, nw)) continue;
       const int b0 = a0 + sigma * tm[5], b1 = a1 + sigma * tm[6],
                 b2 = a2 + sigma * tm[7];
-      if (!inside(b0, b1, b2, nw)) continue;
+      const bool ok = (w != 0.0) & inside(a0, a1, a2, nw) &
+                      inside(b0, b1, b2, nw);
       const size_t a = tm[8] ? ((size_t)b0 * nw.n1 + b1) * nw.n2 + b2
                              : ((size_t)a0 * nw.n1 + a1) * nw.n2 + a2;
-      acc += w * wtab[tm[4]] * (double)fc[(size_t)tm[0] * Nf + a];
+      const double v = (double)fc[ok ? (size_t)tm[0] * Nf + a : 0];
+      acc += ok ? w * wtab[tm[4]] * v : 0.0;
     }
   }
   cc[(size_t)o * Nc + I] = (T)acc;

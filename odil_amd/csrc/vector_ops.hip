@@ -207,6 +207,56 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// One diagonal block of a right-looking blocked Cholesky factorisation of a
+// dense SPD matrix (row-major, leading dimension lda): the block
+// A[k0:k0+nb, k0:k0+nb] (already updated by the previous panels) is
+// factored in LDS, A_kk = L L^T, L written back (the block's strict upper
+// part zeroed), and W = L^{-1} (lower) written to w (nb x nb, row-major,
+// leading dimension kCholNb). The panel and trailing updates around it are
+// GEMMs. Both steps are right-looking over the block's columns: column j is
+// scaled by 1/sqrt(pivot), then the trailing part takes the rank-1 update;
+// W starts as I and takes the same eliminations. nb <= kCholNb.
+constexpr int kCholNb = 64;
+
+__global__ void __launch_bounds__(256)
+    k_chol_block(double* __restrict__ A, int lda, int k0, int nb,
+                 double* __restrict__ w) {
+  __shared__ double L[kCholNb][kCholNb + 1];
+  __shared__ double W[kCholNb][kCholNb + 1];
+  // 16 x 16 threads; element loops stride by 16 in both directions (no
+  // index divisions on the per-column critical path).
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  for (int i = ty; i < nb; i += 16)
+    for (int j = tx; j < nb; j += 16) {
+      L[i][j] = j <= i ? A[(size_t)(k0 + i) * lda + k0 + j] : 0.0;
+      W[i][j] = i == j ? 1.0 : 0.0;
+    }
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    const double d = sqrt(L[j][j]);
+    const double inv = 1.0 / d;
+    // Column j of L below the pivot; row j of W (eliminations so far only
+    // touched columns <= j of W's rows).
+    for (int i = j + 1 + threadIdx.x; i < nb; i += blockDim.x) L[i][j] *= inv;
+    for (int c = threadIdx.x; c <= j; c += blockDim.x) W[j][c] *= inv;
+    __syncthreads();
+    if (threadIdx.x == 0) L[j][j] = d;
+    // Rank-1 updates: L[i][l] -= L[i][j] L[l][j] (j < l <= i) and
+    // W[i][c] -= L[i][j] W[j][c] (i > j, c <= j).
+    for (int i = j + 1 + ty; i < nb; i += 16) {
+      const double lij = L[i][j];
+      for (int l = j + 1 + tx; l <= i; l += 16) L[i][l] -= lij * L[l][j];
+      for (int c = tx; c <= j; c += 16) W[i][c] -= lij * W[j][c];
+    }
+    __syncthreads();
+  }
+  for (int i = ty; i < nb; i += 16)
+    for (int j = tx; j < nb; j += 16) {
+      A[(size_t)(k0 + i) * lda + k0 + j] = j <= i ? L[i][j] : 0.0;
+      w[i * kCholNb + j] = W[i][j];
+    }
+}
+
 }  // namespace odil
 
 using namespace odil;
@@ -427,6 +477,23 @@ int odil_jacobi_zero(int dtype_f, int dtype_y, int nvar, long long N,
     return kErrArg;
   }
   return check_launch("jacobi_zero");
+}
+
+// Block size of odil_chol_block.
+int odil_chol_block_size(void) { return kCholNb; }
+
+// Factor and invert the diagonal block [k0, k0 + nb) of the dense SPD
+// matrix A (FP64, row-major, n x n, leading dimension lda) in place; W gets
+// the inverse of its Cholesky factor (kCholNb x kCholNb, row-major).
+int odil_chol_block(double* A, int lda, int k0, int nb, double* W,
+                    void* stream) {
+  if (!A || !W || nb < 1 || nb > kCholNb || k0 < 0 || lda < k0 + nb) {
+    set_error("chol_block: bad arguments");
+    return kErrArg;
+  }
+  hipLaunchKernelGGL(k_chol_block, dim3(1), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), A, lda, k0, nb, W);
+  return check_launch("chol_block");
 }
 
 // out = x + a y on N values.

@@ -15,6 +15,7 @@ import torch
 
 from odil_amd import _lib
 from odil_amd._lib import c_double, c_int, check, ptr, ptrs, stream
+from odil_amd.dense import SpdSolver
 
 
 def _grid_arr(nw):
@@ -615,6 +616,7 @@ class MultigridOp:
                  galerkin_mode='reference'):
         self.nnw = level_sizes(nw, nlevels)
         self.nlevels = len(self.nnw)
+        self._spd = {}
         for n in self.nnw[:-1]:
             if any(x % 2 == 0 for x in n):
                 raise ValueError('grid sizes {} must be odd on every level '
@@ -694,8 +696,10 @@ class MultigridOp:
         G = M.T @ M
         d = self.coarse_reg * G.diagonal().max()
         G.diagonal().add_(d)
-        L = torch.linalg.cholesky(G)
-        return torch.cholesky_solve(M.T.contiguous(), L).contiguous()
+        n = G.shape[0]
+        if n not in self._spd:
+            self._spd[n] = SpdSolver(n, G.device)
+        return self._spd[n].solve(G, M.T.contiguous()).clone()
 
     def _coarse(self, level):
         if level not in self._coarse_inv:
