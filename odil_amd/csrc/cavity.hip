@@ -352,10 +352,10 @@ int cavity_dispatch_grad(const void* const* fields, void* const* grad,
                          const CavityParams& P, hipStream_t s, int* nb) {
   const T scale = T(2.0 / ((double)nx * ny));
   if constexpr (sizeof(T) == 4)
-    return launch_loss_grad_t<T, CavityOp<T>, 16, 64>(
+    return launch_loss_grad_t<T, CavityOp<T>, kGradTx32, kGradTy>(
         fields, grad, partial, nx, ny, scale, P, s, nb);
   else
-    return launch_loss_grad_t<T, CavityOp<T>, 8, 64>(
+    return launch_loss_grad_t<T, CavityOp<T>, kGradTx64, kGradTy>(
         fields, grad, partial, nx, ny, scale, P, s, nb);
 }
 }  // namespace
@@ -445,9 +445,8 @@ int odil_cavity_loss_grad(int dtype, int nx, int ny, double dx, double dy,
 // Number of per-block partials written by the residual (kind 0) and
 // loss-grad (kind 1) kernels of a 2D operator for the given dtype.
 int odil_cavity_nblocks(int kind, int dtype, int nx, int ny) {
-  if (kind == 0) return ceil_div(ny, 64) * ceil_div(nx, 4);
-  const int tx = dtype == kF32 ? 16 : 8;
-  return ceil_div(ny, 64) * ceil_div(nx, tx);
+  if (kind == 0) return residual_nblocks(nx, ny);
+  return grad_nblocks(dtype == kF32, nx, ny);
 }
 
 // Matrix-free normal operator J^T J with J recomputed from `state`

@@ -130,18 +130,18 @@ int odil_poisson_loss_grad(int dtype, int n0, int n1, double dx, double dy,
   hipStream_t s = static_cast<hipStream_t>(stream);
   const double scale = 2.0 / ((double)n0 * n1);
   if (dtype == kF64)
-    return launch_loss_grad_t<double, PoissonOp<double>, 8, 64>(
+    return launch_loss_grad_t<double, PoissonOp<double>, kGradTx64, kGradTy>(
         fields, grad, partial, n0, n1, scale, P, s, nblocks);
   if (dtype == kF32)
-    return launch_loss_grad_t<float, PoissonOp<float>, 16, 64>(
+    return launch_loss_grad_t<float, PoissonOp<float>, kGradTx32, kGradTy>(
         fields, grad, partial, n0, n1, (float)scale, P, s, nblocks);
   set_error("poisson_loss_grad: bad dtype %d", dtype);
   return kErrArg;
 }
 
 int odil_poisson_nblocks(int kind, int dtype, int n0, int n1) {
-  if (kind == 0) return ceil_div(n1, 64) * ceil_div(n0, 4);
-  return ceil_div(n1, 64) * ceil_div(n0, dtype == kF32 ? 16 : 8);
+  if (kind == 0) return residual_nblocks(n0, n1);
+  return grad_nblocks(dtype == kF32, n0, n1);
 }
 
 int odil_laplace_residual(int dtype, int n0, int n1, double dx, double dy,
@@ -197,18 +197,18 @@ int odil_laplace_loss_grad(int dtype, int n0, int n1, double dx, double dy,
   hipStream_t s = static_cast<hipStream_t>(stream);
   const double scale = 2.0 / ((double)n0 * n1);
   if (dtype == kF64)
-    return launch_loss_grad_t<double, LaplaceOp<double>, 8, 64>(
+    return launch_loss_grad_t<double, LaplaceOp<double>, kGradTx64, kGradTy>(
         fields, grad, partial, n0, n1, scale, P, s, nblocks);
   if (dtype == kF32)
-    return launch_loss_grad_t<float, LaplaceOp<float>, 16, 64>(
+    return launch_loss_grad_t<float, LaplaceOp<float>, kGradTx32, kGradTy>(
         fields, grad, partial, n0, n1, (float)scale, P, s, nblocks);
   set_error("laplace_loss_grad: bad dtype %d", dtype);
   return kErrArg;
 }
 
 int odil_laplace_nblocks(int kind, int dtype, int n0, int n1) {
-  if (kind == 0) return ceil_div(n1, 64) * ceil_div(n0, 4);
-  return ceil_div(n1, 64) * ceil_div(n0, dtype == kF32 ? 16 : 8);
+  if (kind == 0) return residual_nblocks(n0, n1);
+  return grad_nblocks(dtype == kF32, n0, n1);
 }
 
 int odil_poisson_normal_apply(int dtype, int n0, int n1, double dx, double dy,

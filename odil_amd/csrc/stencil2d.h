@@ -83,6 +83,21 @@ struct OutPtrs {
   T* f[NF];
 };
 
+// Tiles of the residual (64 x 4 threads, one cell each) and of the fused
+// loss-adjoint kernels (cells along axis 0 per precision x 64 along axis 1),
+// with the per-block partial counts every operator's *_nblocks reports.
+constexpr int kResTy = 64, kResTx = 4;
+// (On the 1025^2 cavity 16 x 32 (FP32) and 8 x 32 (FP64) were the fastest
+// of 4..32 x 32..128.)
+constexpr int kGradTx32 = 16, kGradTx64 = 8, kGradTy = 32;
+
+inline int residual_nblocks(int n0, int n1) {
+  return ceil_div(n1, kResTy) * ceil_div(n0, kResTx);
+}
+inline int grad_nblocks(bool fp32, int n0, int n1) {
+  return ceil_div(n1, kGradTy) * ceil_div(n0, fp32 ? kGradTx32 : kGradTx64);
+}
+
 // Residual f_e at every cell, and per-block partial sums of f_e^2.
 template <class T, class Op>
 __global__ void __launch_bounds__(256)
@@ -579,7 +594,7 @@ int dispatch_normal_apply(int dtype, const void* const* state,
     return launch_normal_apply_t<double, Op64, 8, 32>(state, x, b, d, y, n0,
                                                       n1, omega, mode, P, s);
   if (dtype == kF32)
-    return launch_normal_apply_t<float, Op32, 16, 32>(state, x, b, d, y, n0,
+    return launch_normal_apply_t<float, Op32, 8, 32>(state, x, b, d, y, n0,
                                                       n1, omega, mode, P, s);
   set_error("normal_apply: bad dtype %d", dtype);
   return kErrArg;
@@ -594,10 +609,10 @@ int dispatch_jtf(int dtype, const void* const* fields, void* const* out,
                  const typename Op64::Params& P, hipStream_t s,
                  int* nblocks) {
   if (dtype == kF64)
-    return launch_loss_grad_t<double, Op64, 8, 64>(fields, out, partial, n0,
-                                                   n1, scale, P, s, nblocks);
+    return launch_loss_grad_t<double, Op64, kGradTx64, kGradTy>(
+        fields, out, partial, n0, n1, scale, P, s, nblocks);
   if (dtype == kF32)
-    return launch_loss_grad_t<float, Op32, 16, 64>(
+    return launch_loss_grad_t<float, Op32, kGradTx32, kGradTy>(
         fields, out, partial, n0, n1, (float)scale, P, s, nblocks);
   set_error("jtf: bad dtype %d", dtype);
   return kErrArg;

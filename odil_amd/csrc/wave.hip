@@ -80,18 +80,18 @@ int odil_wave_loss_grad(int dtype, int nt, int nx, double dt, double dx,
   hipStream_t s = static_cast<hipStream_t>(stream);
   const double scale = 2.0 / ((double)nt * nx);
   if (dtype == kF64)
-    return launch_loss_grad_t<double, WaveOp<double>, 8, 64>(
+    return launch_loss_grad_t<double, WaveOp<double>, kGradTx64, kGradTy>(
         fields, grad, partial, nt, nx, scale, P, s, nblocks);
   if (dtype == kF32)
-    return launch_loss_grad_t<float, WaveOp<float>, 16, 64>(
+    return launch_loss_grad_t<float, WaveOp<float>, kGradTx32, kGradTy>(
         fields, grad, partial, nt, nx, (float)scale, P, s, nblocks);
   set_error("wave_loss_grad: bad dtype %d", dtype);
   return kErrArg;
 }
 
 int odil_wave_nblocks(int kind, int dtype, int nt, int nx) {
-  if (kind == 0) return ceil_div(nx, 64) * ceil_div(nt, 4);
-  return ceil_div(nx, 64) * ceil_div(nt, dtype == kF32 ? 16 : 8);
+  if (kind == 0) return residual_nblocks(nt, nx);
+  return grad_nblocks(dtype == kF32, nt, nx);
 }
 
 int odil_wave_normal_apply(int dtype, int nt, int nx, double dt, double dx,

@@ -162,11 +162,6 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
-template <int TX, int TY>
-int winv_blocks(int nt, int nx) {
-  return ceil_div(nx, TY) * ceil_div(nt, TX);
-}
-
 }  // namespace odil
 
 using namespace odil;
@@ -182,8 +177,7 @@ bool winv_bad(int nt, int nx, const double* a, const double* b,
 }
 
 int loss_blocks(int dtype, int nt, int nx) {
-  return dtype == kF32 ? winv_blocks<16, 64>(nt, nx)
-                       : winv_blocks<8, 64>(nt, nx);
+  return grad_nblocks(dtype == kF32, nt, nx);
 }
 
 int data_blocks(int nt, int nx) {
@@ -207,11 +201,11 @@ int winv_run(int nt, int nx, const WaveParams& P, const int* sensor_of,
   const T scale = T(2.0 / ncell);
   int st;
   if (sizeof(T) == 4)
-    st = launch_loss_grad_t<T, WaveOp<T>, 16, 64>(fields, grads, pw, nt, nx,
-                                                  scale, P, s, nullptr);
+    st = launch_loss_grad_t<T, WaveOp<T>, kGradTx32, kGradTy>(
+        fields, grads, pw, nt, nx, scale, P, s, nullptr);
   else
-    st = launch_loss_grad_t<T, WaveOp<T>, 8, 64>(fields, grads, pw, nt, nx,
-                                                 scale, P, s, nullptr);
+    st = launch_loss_grad_t<T, WaveOp<T>, kGradTx64, kGradTy>(
+        fields, grads, pw, nt, nx, scale, P, s, nullptr);
   if (st) return st;
   hipLaunchKernelGGL(k_winv_data<T>, dim3(nbd), dim3(256), 0, s,
                      static_cast<const T*>(u), static_cast<T*>(gu),
