@@ -70,6 +70,32 @@ __device__ inline T block_sum(T v, T* buf) {
   return s;
 }
 
+// Block-level sums of N values at once (one LDS round trip and two
+// barriers for all of them). `buf` holds N * 4 elements (up to 4 waves);
+// results valid in thread 0, written back into v.
+template <int N, class T>
+__device__ inline void block_sum_n(T (&v)[N], T* buf) {
+#pragma unroll
+  for (int e = 0; e < N; ++e) v[e] = wave_sum(v[e]);
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int e = 0; e < N; ++e) buf[e * 4 + wid] = v[e];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int e = 0; e < N; ++e) {
+      T s = 0;
+      for (int w = 0; w < nw; ++w) s += buf[e * 4 + w];
+      v[e] = s;
+    }
+  }
+}
+
 __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 // Halo rule at a wall: the value outside the domain is extrapolated from the

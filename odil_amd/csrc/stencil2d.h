@@ -64,16 +64,6 @@ __device__ inline bool interior_box(int a0, int a1, int b0, int b1, int n0,
   return a0 >= R && a1 <= n0 - R && b0 >= R && b1 <= n1 - R;
 }
 
-template <class T>
-struct LdsAcc {
-  const T* s;
-  int e0, e1;  // extents of one field's tile
-  int li, lj;  // local position of the cell
-  __device__ inline T operator()(int k, int di, int dj) const {
-    return s[(k * e0 + li + di) * e1 + lj + dj];
-  }
-};
-
 template <class T, int NF>
 struct FieldPtrs {
   const T* f[NF];
@@ -104,7 +94,7 @@ __global__ void __launch_bounds__(256)
     k_residual(FieldPtrs<T, Op::NF> in, OutPtrs<T, Op::NE> out,
                double* __restrict__ partial, int n0, int n1,
                typename Op::Params P) {
-  __shared__ double red[4];
+  __shared__ double red[4 * Op::NE];
   unsigned bx, by;
   xcd_block2(bx, by);
   const int j = bx * 64 + (threadIdx.x & 63);
@@ -130,10 +120,9 @@ __global__ void __launch_bounds__(256)
   if (partial) {
     const int b = by * gridDim.x + bx;
     const int nb = gridDim.x * gridDim.y;
-    for (int e = 0; e < Op::NE; ++e) {
-      double s = block_sum(ssq[e], red);
-      if (threadIdx.x == 0) partial[(size_t)e * nb + b] = s;
-    }
+    block_sum_n<Op::NE>(ssq, red);
+    if (threadIdx.x == 0)
+      for (int e = 0; e < Op::NE; ++e) partial[(size_t)e * nb + b] = ssq[e];
   }
 }
 
@@ -314,7 +303,7 @@ __global__ void __launch_bounds__(256)
   constexpr int NE = Op::NE;
   extern __shared__ unsigned char smem_raw[];
   T* ot = reinterpret_cast<T*>(smem_raw);  // [NF][NS-1][E0][E1]
-  __shared__ double red[4];
+  __shared__ double red[4 * Op::NE];
   unsigned bx, by;
   xcd_block2(bx, by);
   double ssq[NE];
@@ -324,10 +313,9 @@ __global__ void __launch_bounds__(256)
   if (partial) {
     const int bidx = by * gridDim.x + bx;
     const int nb = gridDim.x * gridDim.y;
-    for (int e = 0; e < NE; ++e) {
-      double s = block_sum(ssq[e], red);
-      if (threadIdx.x == 0) partial[(size_t)e * nb + bidx] = s;
-    }
+    block_sum_n<NE>(ssq, red);
+    if (threadIdx.x == 0)
+      for (int e = 0; e < NE; ++e) partial[(size_t)e * nb + bidx] = ssq[e];
   }
 }
 
@@ -503,7 +491,7 @@ int launch_residual(const void* const* fields, void* const* res,
   for (int k = 0; k < Op::NF; ++k) in.f[k] = static_cast<const T*>(fields[k]);
   for (int e = 0; e < Op::NE; ++e)
     out.f[e] = res ? static_cast<T*>(res[e]) : nullptr;
-  dim3 grid(ceil_div(n1, 64), ceil_div(n0, 4));
+  dim3 grid(ceil_div(n1, kResTy), ceil_div(n0, kResTx));
   if (nblocks) *nblocks = grid.x * grid.y;
   hipLaunchKernelGGL((k_residual<T, Op>), grid, dim3(256), 0, stream, in, out,
                      partial, n0, n1, P);
