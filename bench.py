@@ -37,7 +37,7 @@ def parse():
     p.add_argument('--re', type=float, default=100.)
     p.add_argument('--maxiter', type=int, default=10)
     p.add_argument('--linsolver', default='pcg', choices=['vcycle', 'pcg'])
-    p.add_argument('--omega', type=float, default=0.5)
+    p.add_argument('--omega', type=float, default=0.6)
     p.add_argument('--nlvl', type=int, default=7)
     p.add_argument('--setup_every', type=int, default=4,
                    help='rebuild the stored multigrid hierarchy every k '
@@ -167,6 +167,27 @@ def measured_traffic():
     return None
 
 
+def copy_ceiling():
+    """Measured device-copy bandwidth (STREAM-copy-like, 512 MiB read +
+    512 MiB written by torch's copy kernel), as a practical ceiling beside
+    the 8 TB/s peak."""
+    import torch
+    x = torch.ones(1 << 27, dtype=torch.float32, device='cuda')
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        y.copy_(x)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / 10
+    del x, y
+    return 2 * (1 << 27) * 4 / (ms * 1e-3) / 1e9
+
+
 def copy_probe():
     """Plain streaming kernel of known size for counter calibration:
     y = -x over 2^26 FP32 values (256 MiB read, 256 MiB written), 5 times."""
@@ -242,10 +263,22 @@ def cpu_baseline(args):
     """One Newton step of the same algorithm in the CPU oracle (NumPy/SciPy,
     float64) on a bounded sample: the cavity at cpu_n x cpu_n with the same
     level count rule and V-cycle count, scaled to 1025^2 by cell count (the
-    multigrid step is linear in the number of cells)."""
+    multigrid step is linear in the number of cells). Timed with one thread
+    and again with all the host's threads."""
+    import torch
+    ncpu = min(os.cpu_count() or 1, 16)
+    one = _cpu_step(args, 1)
+    allc = _cpu_step(args, ncpu)
+    one['all_cores'] = {'value': allc['value'], 'cores': ncpu,
+                        'seconds': allc['seconds']}
+    torch.set_num_threads(ncpu)
+    return one
+
+
+def _cpu_step(args, threads):
     import numpy as np
     import torch
-    torch.set_num_threads(1)
+    torch.set_num_threads(threads)
     from oracle import multigrid as omg
     from oracle import operators as ops
     from oracle import tape
@@ -267,8 +300,8 @@ def cpu_baseline(args):
         sol = M.step(sol, rhs, args.omega, ndirect=3)
     t1 = time.perf_counter()
     sec = (t1 - t0) * (args.n**2) / (n * n)
-    return {'value': 1.0 / sec, 'unit': 'Newton steps/s', 'cores': 1,
-            'kind': 'port',
+    return {'value': 1.0 / sec, 'unit': 'Newton steps/s', 'cores': threads,
+            'kind': 'port', 'seconds': t1 - t0,
             'sample': ('one oracle Newton step (tape Jacobian, stencil J^T J, '
                        'exact Galerkin, {} V(2,2) cycles) at {}x{}, {:.2f} s, '
                        'scaled to {}x{} by cell count').format(
@@ -345,7 +378,8 @@ def main():
                           dom['grid'][0], dom['grid'][1], dom['nent']),
             'achieved': dom['GBps'], 'peak': HBM_PEAK_GBPS, 'unit': 'GB/s',
             'frac': dom['GBps'] / HBM_PEAK_GBPS,
-            'traffic': measured_traffic()},
+            'traffic': measured_traffic(),
+            'measured_copy_GBps': copy_ceiling()},
     }
     if 'mixed' in tol and 'fp64' in tol:
         line['time_to_tol_speedup'] = (tol['fp64']['seconds'] /

@@ -151,7 +151,7 @@ class NewtonMultigrid:
     normal-equation solver. `fields` are FP64 device tensors updated in
     place by `step()`."""
 
-    def __init__(self, op, nlvl=7, smooth_pre=2, smooth_post=2, omega=0.5,
+    def __init__(self, op, nlvl=7, smooth_pre=2, smooth_post=2, omega=0.6,
                  ndirect=3, maxiter=10, restriction='transpose',
                  galerkin_mode='exact', precision='fp64', linsolver='pcg',
                  rtol=0., check_every=10, coarse_reg='auto',
