@@ -108,6 +108,34 @@ int odil_cavity_loss_grad(int dtype, int nx, int ny, double dx, double dy,
                           const void* const* fields, void* const* grad,
                           double* partial, int* nblocks, void* stream);
 int odil_cavity_nblocks(int kind, int dtype, int nx, int ny);
+/* Extra loss terms of the cavity inverse problems (ns_steady.py:396-411),
+ * appended to the cavity residual as six equations on (u, v, p):
+ * mask (u - u_imp) cimp, mask (v - v_imp) cimp (imposed points, cimp =
+ * alpha N / n_imp; mask/uimp/vimp NULL: no imposed points) and
+ * kreg (u_xx, u_yy, v_xx, v_yy) (laplace_split of the halo-corrected live
+ * stencils, ns_steady.py:213-217; kreg already annealed by the caller).
+ * Residual, fused loss + adjoint (ADDS (2/N) J^T f to grad; partials as
+ * odil_cavity_nblocks(0|1, ...) x 6) and Jacobian (14 coefficient fields,
+ * entries from odil_cavity_extras_jac_entries). */
+int odil_cavity_extras_residual(int dtype, int nx, int ny, double dx,
+                                double dy, int bc, double kreg, double cimp,
+                                const void* mask, const void* uimp,
+                                const void* vimp, const void* const* fields,
+                                void* const* res, double* partial,
+                                void* stream);
+int odil_cavity_extras_loss_grad(int dtype, int nx, int ny, double dx,
+                                 double dy, int bc, double kreg, double cimp,
+                                 const void* mask, const void* uimp,
+                                 const void* vimp, const void* const* fields,
+                                 void* const* grad, double* partial,
+                                 int* nblocks, void* stream);
+int odil_cavity_extras_jac(int dtype, int nx, int ny, double dx, double dy,
+                           int bc, double kreg, double cimp, const void* mask,
+                           const void* uimp, const void* vimp,
+                           const void* const* fields, void* coef,
+                           void* const* res, void* stream);
+int odil_cavity_extras_jac_entries(int* ent, int* shifts);
+int odil_cavity_extras_nblocks(int kind, int dtype, int nx, int ny);
 int odil_cavity_normal_apply(int dtype, int nx, int ny, double dx, double dy,
                              double mu, int bc, int rhiechow,
                              const void* const* state, const void* const* x,

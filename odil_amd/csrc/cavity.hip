@@ -341,6 +341,117 @@ struct CavityOp {
   }
 };
 
+// Extra loss terms of the cavity's inverse problems (ns_steady.py:396-411),
+// as additional equations on the unknowns (u, v, p):
+//   e0 = mask (u - u_imp) cimp,  e1 = mask (v - v_imp) cimp   (imposed
+//        points, cimp = alpha N / n_imp),
+//   e2..e5 = kreg (u_xx, u_yy, v_xx, v_yy)   (laplace_split of the live,
+//        halo-corrected 5-point stencils).
+// Terms that are off (no mask, kreg = 0) are zero. Its adjoint is added to
+// the cavity's (ACCUM): both kernels together give the reference's loss
+// gradient with the extra residuals appended.
+struct CavityExtrasParams {
+  double idx2, idy2;   // 1/dx^2, 1/dy^2
+  double kreg, cimp;   // regularisation factor, imposed-point weight
+  int bc;
+  const void* mask;    // T arrays (nx, ny), or null: no imposed points
+  const void* uimp;
+  const void* vimp;
+};
+
+template <class T>
+struct CavityExtrasOp {
+  using Params = CavityExtrasParams;
+  static constexpr int NF = 3, NE = 6, NS = 5, R = 1, RJ = 1;
+  static constexpr bool ACCUM = true;
+  static constexpr int SX[NS] = {0, -1, 1, 0, 0};
+  static constexpr int SY[NS] = {0, 0, 0, -1, 1};
+  static constexpr int NJ = 14;
+  static constexpr int JE[NJ] = {0, 1, 2, 2, 2, 3, 3, 3, 4, 4, 4, 5, 5, 5};
+  static constexpr int JV[NJ] = {0, 1, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1, 1, 1};
+  static constexpr int JS[NJ] = {0, 0, 0, 1, 2, 0, 3, 4, 0, 1, 2, 0, 3, 4};
+
+  // Halo-corrected distance-1 neighbours (c, xm, xp, ym, yp) of field k.
+  template <bool INT, class Acc>
+  __device__ static inline void nbr5(const Acc& q, int k, int i, int j,
+                                     int nx, int ny, const FaceRule& r,
+                                     T* s) {
+    const T c = q(k, 0, 0);
+    const T rxm = q(k, -1, 0), rxp = q(k, 1, 0);
+    const T rym = q(k, 0, -1), ryp = q(k, 0, 1);
+    s[0] = c;
+    if (INT) {
+      s[1] = rxm;
+      s[2] = rxp;
+      s[3] = rym;
+      s[4] = ryp;
+      return;
+    }
+    s[1] = (i == 0) ? ghost(r.kx, rxp, c, T(r.wxm)) : rxm;
+    s[2] = (i == nx - 1) ? ghost(r.kx, rxm, c, T(r.wxp)) : rxp;
+    s[3] = (j == 0) ? ghost(r.ky, ryp, c, T(r.wym)) : rym;
+    s[4] = (j == ny - 1) ? ghost(r.ky, rym, c, T(r.wyp)) : ryp;
+  }
+
+  template <bool INT, class Acc>
+  __device__ static inline void compute(const Acc& q, int i, int j, int nx,
+                                        int ny, const Params& P, T* f,
+                                        T (*J)[NF][NS], bool want_jac) {
+    const FaceRule ru = cavity_rule(P.bc, 0), rv = cavity_rule(P.bc, 1);
+    T U[5], V[5];
+    nbr5<INT>(q, 0, i, j, nx, ny, ru, U);
+    nbr5<INT>(q, 1, i, j, nx, ny, rv, V);
+    T m = 0, ui = 0, vi = 0;
+    if (P.mask) {
+      const size_t c = (size_t)i * ny + j;
+      m = static_cast<const T*>(P.mask)[c];
+      ui = static_cast<const T*>(P.uimp)[c];
+      vi = static_cast<const T*>(P.vimp)[c];
+    }
+    const T ci = T(P.cimp), kr = T(P.kreg);
+    const T ix2 = T(P.idx2), iy2 = T(P.idy2);
+    f[0] = m * (U[0] - ui) * ci;
+    f[1] = m * (V[0] - vi) * ci;
+    f[2] = (U[2] - T(2) * U[0] + U[1]) * ix2 * kr;
+    f[3] = (U[4] - T(2) * U[0] + U[3]) * iy2 * kr;
+    f[4] = (V[2] - T(2) * V[0] + V[1]) * ix2 * kr;
+    f[5] = (V[4] - T(2) * V[0] + V[3]) * iy2 * kr;
+    if (!want_jac) return;
+    for (int e = 0; e < NE; ++e)
+      for (int k = 0; k < NF; ++k)
+        for (int s = 0; s < NS; ++s) J[e][k][s] = 0;
+    J[0][0][0] = m * ci;
+    J[1][1][0] = m * ci;
+    const T ax = ix2 * kr, ay = iy2 * kr;
+    const T exx[5] = {-T(2) * ax, ax, ax, 0, 0};
+    const T eyy[5] = {-T(2) * ay, 0, 0, ay, ay};
+    fold<INT>(J[2][0], exx, i, j, nx, ny, ru.kx, ru.ky);
+    fold<INT>(J[3][0], eyy, i, j, nx, ny, ru.kx, ru.ky);
+    fold<INT>(J[4][1], exx, i, j, nx, ny, rv.kx, rv.ky);
+    fold<INT>(J[5][1], eyy, i, j, nx, ny, rv.kx, rv.ky);
+  }
+
+  template <bool INT = false, class Acc>
+  __device__ static inline void residual(const Acc& q, int i, int j, int nx,
+                                         int ny, const Params& P, T* f) {
+    compute<INT>(q, i, j, nx, ny, P, f, nullptr, false);
+  }
+  template <bool INT = false, class Acc>
+  __device__ static inline void eval(const Acc& q, int i, int j, int nx,
+                                     int ny, const Params& P, T* f,
+                                     T (*J)[NF][NS]) {
+    compute<INT>(q, i, j, nx, ny, P, f, J, true);
+  }
+};
+
+inline CavityExtrasParams extras_params(double dx, double dy, int bc,
+                                        double kreg, double cimp,
+                                        const void* mask, const void* uimp,
+                                        const void* vimp) {
+  return CavityExtrasParams{1.0 / (dx * dx), 1.0 / (dy * dy), kreg, cimp,
+                            bc, mask, uimp, vimp};
+}
+
 }  // namespace odil
 
 using namespace odil;
@@ -440,6 +551,107 @@ int odil_cavity_loss_grad(int dtype, int nx, int ny, double dx, double dy,
                                        nblocks);
   set_error("cavity_loss_grad: bad dtype %d", dtype);
   return kErrArg;
+}
+
+// Extra cavity loss terms (imposed points, regularisation; see
+// CavityExtrasOp): residual fields (6, any may be NULL) with per-block sums
+// of squares; mask/uimp/vimp NULL disables the imposed-point terms.
+int odil_cavity_extras_residual(int dtype, int nx, int ny, double dx,
+                                double dy, int bc, double kreg, double cimp,
+                                const void* mask, const void* uimp,
+                                const void* vimp, const void* const* fields,
+                                void* const* res, double* partial,
+                                void* stream) {
+  if (nx < 3 || ny < 3 || !fields || (mask && (!uimp || !vimp))) {
+    set_error("cavity_extras_residual: bad arguments");
+    return kErrArg;
+  }
+  const CavityExtrasParams P =
+      extras_params(dx, dy, bc, kreg, cimp, mask, uimp, vimp);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    return launch_residual<double, CavityExtrasOp<double>>(
+        fields, res, partial, nx, ny, P, s, nullptr);
+  if (dtype == kF32)
+    return launch_residual<float, CavityExtrasOp<float>>(
+        fields, res, partial, nx, ny, P, s, nullptr);
+  set_error("cavity_extras_residual: bad dtype %d", dtype);
+  return kErrArg;
+}
+
+// Adds the extra terms' adjoint (2/N) J_x^T f_x to grad (u, v, p arrays;
+// the p gradient gets zeros added) with per-block sums of squares of the
+// six extra equations in `partial` (6 x odil_cavity_nblocks(1, ...)).
+int odil_cavity_extras_loss_grad(int dtype, int nx, int ny, double dx,
+                                 double dy, int bc, double kreg, double cimp,
+                                 const void* mask, const void* uimp,
+                                 const void* vimp, const void* const* fields,
+                                 void* const* grad, double* partial,
+                                 int* nblocks, void* stream) {
+  if (nx < 3 || ny < 3 || !fields || !grad || (mask && (!uimp || !vimp))) {
+    set_error("cavity_extras_loss_grad: bad arguments");
+    return kErrArg;
+  }
+  const CavityExtrasParams P =
+      extras_params(dx, dy, bc, kreg, cimp, mask, uimp, vimp);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    return launch_loss_grad_t<double, CavityExtrasOp<double>, kGradTx64,
+                              kGradTy>(fields, grad, partial, nx, ny,
+                                       2.0 / ((double)nx * ny), P, s,
+                                       nblocks);
+  if (dtype == kF32)
+    return launch_loss_grad_t<float, CavityExtrasOp<float>, kGradTx32,
+                              kGradTy>(fields, grad, partial, nx, ny,
+                                       (float)(2.0 / ((double)nx * ny)), P,
+                                       s, nblocks);
+  set_error("cavity_extras_loss_grad: bad dtype %d", dtype);
+  return kErrArg;
+}
+
+// Jacobian coefficient fields of the extra equations (14 x N), entries as
+// odil_cavity_extras_jac_entries lists them; `res` (6 fields or NULL)
+// optionally receives the residuals.
+int odil_cavity_extras_jac(int dtype, int nx, int ny, double dx, double dy,
+                           int bc, double kreg, double cimp, const void* mask,
+                           const void* uimp, const void* vimp,
+                           const void* const* fields, void* coef,
+                           void* const* res, void* stream) {
+  if (nx < 3 || ny < 3 || !fields || !coef || (mask && (!uimp || !vimp))) {
+    set_error("cavity_extras_jac: bad arguments");
+    return kErrArg;
+  }
+  const CavityExtrasParams P =
+      extras_params(dx, dy, bc, kreg, cimp, mask, uimp, vimp);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    return launch_jac<double, CavityExtrasOp<double>>(fields, coef, res,
+                                                      nx, ny, P, s);
+  if (dtype == kF32)
+    return launch_jac<float, CavityExtrasOp<float>>(fields, coef, res, nx,
+                                                    ny, P, s);
+  set_error("cavity_extras_jac: bad dtype %d", dtype);
+  return kErrArg;
+}
+
+// Per-block partial counts of the extras kernels (the cavity's tiles).
+int odil_cavity_extras_nblocks(int kind, int dtype, int nx, int ny) {
+  if (kind == 0) return residual_nblocks(nx, ny);
+  return grad_nblocks(dtype == kF32, nx, ny);
+}
+
+int odil_cavity_extras_jac_entries(int* ent, int* shifts) {
+  using E = CavityExtrasOp<double>;
+  for (int q = 0; q < E::NJ; ++q) {
+    ent[3 * q] = E::JE[q];
+    ent[3 * q + 1] = E::JV[q];
+    ent[3 * q + 2] = E::JS[q];
+  }
+  for (int s = 0; s < E::NS; ++s) {
+    shifts[2 * s] = E::SX[s];
+    shifts[2 * s + 1] = E::SY[s];
+  }
+  return E::NJ;
 }
 
 // Number of per-block partials written by the residual (kind 0) and

@@ -19,6 +19,8 @@
 // whose live dependence is linear).
 #pragma once
 
+#include <type_traits>
+
 #include "common.h"
 
 namespace odil {
@@ -63,6 +65,18 @@ __device__ inline bool interior_box(int a0, int a1, int b0, int b1, int n0,
                                     int n1, int R) {
   return a0 >= R && a1 <= n0 - R && b0 >= R && b1 <= n1 - R;
 }
+
+// Operators with `static constexpr bool ACCUM = true` add their adjoint to
+// the gradient arrays (extra loss terms of another operator's unknowns);
+// all others overwrite them.
+template <class Op, class = void>
+struct op_accumulates {
+  static constexpr bool value = false;
+};
+template <class Op>
+struct op_accumulates<Op, std::void_t<decltype(Op::ACCUM)>> {
+  static constexpr bool value = Op::ACCUM;
+};
 
 template <class T, int NF>
 struct FieldPtrs {
@@ -288,7 +302,12 @@ __device__ inline void loss_grad_tile(const FieldPtrs<T, Op::NF>& in,
 #pragma unroll
       for (int s = 1; s < NS; ++s)
         g += OT(k, s, a + RJ - Op::SX[s], b + RJ - Op::SY[s]);
-      if (grad.f[k]) grad.f[k][c] = g * scale;
+      if (grad.f[k]) {
+        if constexpr (op_accumulates<Op>::value)
+          grad.f[k][c] += g * scale;
+        else
+          grad.f[k][c] = g * scale;
+      }
     }
   }
 }

@@ -259,6 +259,74 @@ class Cavity(Operator):
                 c_int(self.bc), c_int(self.rhiechow)]
 
 
+class CavityExtras(Operator):
+    """Extra loss terms of the cavity's inverse problems
+    (ns_steady.py:396-411) on the cavity's unknowns: imposed points
+    mask (u - u_imp) cimp, mask (v - v_imp) cimp with cimp = alpha N / n_imp
+    (`imposed` = (mask, u_imp, v_imp) arrays of the grid shape, n_imp the
+    nonzeros of mask), and kreg (u_xx, u_yy, v_xx, v_yy) on the
+    halo-corrected stencils (`kreg` already annealed). Its adjoint is added
+    to the gradients it is given; `cavity_inverse_loss_grad` combines it
+    with the cavity loss in the reference's equation order."""
+
+    fieldnames = ('u', 'v', 'p')
+    neq = 6
+    prefix = 'cavity_extras'
+
+    def __init__(self, cavity, imposed=None, alpha=1., kreg=0.,
+                 device=None):
+        super().__init__(cavity.shape)
+        self.cavity = cavity
+        self.kreg = float(kreg)
+        device = device or torch.device('cuda')
+        self.n_imp = 0
+        self._imp = {}
+        if imposed is not None:
+            mask, uimp, vimp = (np.asarray(a, dtype=np.float64)
+                                for a in imposed)
+            if any(a.shape != self.shape for a in (mask, uimp, vimp)):
+                raise ValueError('imposed arrays must have the grid shape')
+            self.n_imp = int(np.count_nonzero(mask))
+            for dt, tag in ((torch.float64, _lib.F64),
+                            (torch.float32, _lib.F32)):
+                self._imp[tag] = [torch.as_tensor(a, dtype=dt,
+                                                  device=device)
+                                  for a in (mask, uimp, vimp)]
+        self.cimp = (float(alpha) * self.ncell / self.n_imp
+                     if self.n_imp else 0.)
+
+    def enabled(self):
+        """Indices of the extra equations the reference would append."""
+        return ([0, 1] if self.n_imp else []) + \
+            ([2, 3, 4, 5] if self.kreg else [])
+
+    def _args_for(self, dtag):
+        c = self.cavity
+        imp = self._imp.get(dtag)
+        return [c_double(c.dx), c_double(c.dy), c_int(c.bc),
+                c_double(self.kreg), c_double(self.cimp)] + (
+                    [ptr(a) for a in imp] if imp else [None, None, None])
+
+    def _args(self):
+        return self._args_for(_lib.F64)
+
+    def loss_grad(self, fields, grads=None):
+        """Adds the extra terms' adjoint to `grads` (zeros if None)."""
+        if grads is None:
+            grads = [torch.zeros_like(f) for f in fields]
+        return super().loss_grad(fields, grads)
+
+
+def cavity_inverse_loss_grad(cavity, extras, fields):
+    """Loss, gradients and loss split of the cavity residuals with the
+    enabled extra equations appended (ns_steady.py:394-411)."""
+    loss, grads, split = cavity.loss_grad(fields)
+    lx, grads, sx = extras.loss_grad(fields, grads)
+    idx = torch.tensor(extras.enabled(), dtype=torch.long,
+                       device=split.device)
+    return loss + lx, grads, torch.cat([split, sx[idx]])
+
+
 def _dev64(a, device):
     return torch.as_tensor(np.asarray(a, dtype=np.float64),
                            device=device).contiguous()

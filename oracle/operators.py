@@ -133,13 +133,20 @@ def wave_gaussian_setup(nt, nx, seed):
 # `ns_steady.py:142-394` (defaults bc=0, rhiechow=1, no imposed points).
 
 
-def make_cavity_operator(mu, bc=0, rhiechow=1):
+def make_cavity_operator(mu, bc=0, rhiechow=1, imposed=None, alpha=1.,
+                         kreg=0.):
     """Residuals (fu, fv, fp) of the steady Navier-Stokes cavity.
 
     Velocity derivatives use deferred correction: second-order upwind of the
     frozen field plus first-order upwind of the live field minus first-order
     upwind of the frozen field (`ns_steady.py:171-204,282-283,320-323`).
     The lid (u=1) is at y=1 (`:224`).
+
+    Inverse-problem terms appended as in `ns_steady.py:396-411`: with
+    `imposed` = (mask, u_imp, v_imp), mask (u - u_imp) coeff and
+    mask (v - v_imp) coeff, coeff = alpha N / n_imp (n_imp the nonzeros of
+    mask); with kreg != 0, kreg times u_xx, u_yy, v_xx, v_yy of the
+    halo-corrected stencils (`laplace_split`, `:213-217`).
     """
 
     def operator(mod, ctx):
@@ -314,7 +321,23 @@ def make_cavity_operator(mu, bc=0, rhiechow=1):
             u_xc = (u_st[2] - u_st[1]) / (2 * dx)
             v_yc = (v_st[4] - v_st[3]) / (2 * dy)
             fp = u_xc + v_yc
-        return [fu, fv, fp]
+        res = [fu, fv, fp]
+        if imposed is not None:
+            mask, uimp, vimp = (torch.as_tensor(np.asarray(a, np.float64))
+                                for a in imposed)
+            n_imp = int(np.count_nonzero(np.asarray(imposed[0])))
+            coeff = alpha * nx * ny / n_imp
+            res += [mask * (u_st[0] - uimp) * coeff,
+                    mask * (v_st[0] - vimp) * coeff]
+        if kreg:
+            def laplace_split(st):
+                q, qxm, qxp, qym, qyp = st
+                return ((qxp - 2 * q + qxm) / dx**2,
+                        (qyp - 2 * q + qym) / dy**2)
+            u_xx, u_yy = laplace_split(u_st)
+            v_xx, v_yy = laplace_split(v_st)
+            res += [u_xx * kreg, u_yy * kreg, v_xx * kreg, v_yy * kreg]
+        return res
 
     return operator
 

@@ -101,3 +101,32 @@ def test_argument_errors_do_not_launch(lib):
         _lib.check(1, 'probe')
     with pytest.raises(_lib.OdilError):
         _lib.check(3, 'probe')
+
+
+def _param_counts():
+    text = open(HEADER).read()
+    out = {}
+    for m in re.finditer(r'\bint\s+(odil_\w+)\s*\(([^;]*?)\)\s*;', text,
+                         re.S):
+        params = m.group(2).strip()
+        out[m.group(1)] = 0 if params in ('', 'void') else \
+            params.count(',') + 1
+    return out
+
+
+def test_operator_entry_points_match_python_calls():
+    """The shared Python operator base passes (dtype, shape..., operator
+    parameters..., then fields, out, partial[, nblocks], stream): every
+    operator's residual / loss_grad / jac declaration takes exactly that
+    many arguments (ctypes does not check arity)."""
+    import torch
+    from odil_amd import _lib
+    from odil_amd.operators import Cavity, CavityExtras
+    counts = _param_counts()
+    cav = Cavity((8, 9), 0.01)
+    ops = [cav, CavityExtras(cav, kreg=0.1, device=torch.device('cpu'))]
+    for op in ops:
+        base = 1 + len(op.shape) + len(op._args_for(_lib.F64))
+        for what, tail in (('residual', 4), ('loss_grad', 5), ('jac', 4)):
+            name = 'odil_{}_{}'.format(op.prefix, what)
+            assert counts[name] == base + tail, (name, counts[name])

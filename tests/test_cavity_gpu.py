@@ -167,3 +167,66 @@ def test_newton_multigrid_options_rejected(dev):
                         refine=1)
     with pytest.raises(ValueError):
         NewtonMultigrid(op, nlvl=3, precision='mixed', krylov='bf16')
+
+
+# -- inverse-problem terms (imposed points, kreg) ----------------------------
+def _imposed(n0, n1, seed):
+    rng = np.random.default_rng(seed)
+    mask = (rng.uniform(size=(n0, n1)) < 0.1).astype(np.float64)
+    mask[0, 0] = 1.
+    return mask, rng.uniform(-1, 1, (n0, n1)), rng.uniform(-1, 1, (n0, n1))
+
+
+@pytest.mark.parametrize('n0,n1,re,bc,rc', CASES)
+@pytest.mark.parametrize('terms', ['imposed', 'kreg', 'both'])
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_cavity_extras_match_oracle(dev, n0, n1, re, bc, rc, terms, dtype):
+    """Loss, split and gradient of the cavity with the extra equations
+    appended, the extras' residuals and Jacobian, against the oracle."""
+    from odil_amd.operators import (Cavity, CavityExtras,
+                                    cavity_inverse_loss_grad)
+    st = _state(n0, n1, seed=5 * n0 + n1)
+    imp = _imposed(n0, n1, n0) if terms in ('imposed', 'both') else None
+    kreg = 0.3 if terms in ('kreg', 'both') else 0.
+    dom = ops.cavity_domain(n0, n1)
+    op_ref = ops.make_cavity_operator(1 / re, bc=bc, rhiechow=rc,
+                                      imposed=imp, alpha=0.8, kreg=kreg)
+    loss, grads, split = tape.eval_loss_grad(op_ref, dom, st)
+    cav = Cavity((n0, n1), 1 / re, bc=bc, rhiechow=rc)
+    ext = CavityExtras(cav, imposed=imp, alpha=0.8, kreg=kreg)
+    f = _dev(st, dev, dtype)
+    l, g, ls = cavity_inverse_loss_grad(cav, ext, f)
+    tol = 1e-11 if dtype == torch.float64 else 1e-4
+    np.testing.assert_allclose(l.item(), loss, rtol=tol)
+    np.testing.assert_allclose(ls.cpu().numpy(), split, rtol=tol,
+                               atol=tol * max(split))
+    rtol = RTOL64 if dtype == torch.float64 else RTOL32
+    for k, name in enumerate('uvp'):
+        _close(g[k].cpu(), grads[name], rtol,
+               scale=np.abs(grads[name]).max())
+    # residuals and Jacobian of the extra equations
+    ff, jg, desc = tape.eval_grad(op_ref, dom, st)
+    res, _ = ext.residual(f)
+    on = ext.enabled()
+    for i, e in enumerate(on):
+        _close(res[e].cpu(), ff[3 + i], rtol,
+               scale=np.abs(ff[3 + i]).max() + 1)
+    jac, _ = ext.jacobian(f)
+    coef = jac.coef.cpu().numpy()
+    for q, (e, k, s) in enumerate(jac.entries):
+        key = (k, tuple(s))
+        gref = None
+        if e in on and key in desc:
+            gref = jg[3 + on.index(e)][desc.index(key)]
+        if gref is None:
+            assert np.abs(coef[q]).max() == 0, (e, k, s)
+        else:
+            _close(coef[q], gref, rtol, scale=np.abs(gref).max() + 1)
+
+
+def test_cavity_extras_rejects_bad_shape(dev):
+    from odil_amd.operators import Cavity, CavityExtras
+    cav = Cavity((8, 8), 0.01)
+    with pytest.raises(ValueError):
+        CavityExtras(cav, imposed=(np.zeros((8, 7)), np.zeros((8, 8)),
+                                   np.zeros((8, 8))))

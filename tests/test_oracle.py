@@ -283,3 +283,66 @@ def test_inverse_wave_oracle_pins():
     assert abs(np.sum(gu * du) + np.sum(gc * dc) - fd) <= 1e-7 * abs(fd)
     with pytest.raises(ValueError):
         inv.wave_inverse_setup(70, 130)  # explicit march CFL violated
+
+
+# -- cavity inverse-problem terms (ns_steady.py:396-411) ---------------------
+def _imposed(n, rng):
+    mask = np.zeros((n, n))
+    for i, j in ((1, 2), (3, 3), (n - 1, 0)):
+        mask[i, j] = 1.
+    return mask, rng.uniform(-1, 1, (n, n)), rng.uniform(-1, 1, (n, n))
+
+
+def test_cavity_extras_closed_forms():
+    """kreg terms of a quadratic field: u = x^2 has u_xx = 2, u_yy = 0 and
+    v = y^2 has v_xx = 0, v_yy = 2 at every cell away from the walls; unit
+    misfits on the imposed cells give a loss term alpha^2 N / n_imp."""
+    n = 8
+    dom = ops.cavity_domain(n)
+    xc = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(xc, xc, indexing='ij')
+    st = {'u': X**2, 'v': Y**2, 'p': np.zeros((n, n))}
+    kreg, alpha = 0.3, 2.5
+    ff = tape.eval_operator(ops.make_cavity_operator(0.01, kreg=kreg),
+                            dom, st)
+    inner = (slice(1, -1), slice(1, -1))
+    np.testing.assert_allclose(ff[3][inner], 2 * kreg, rtol=1e-10)
+    np.testing.assert_allclose(ff[4][inner], 0, atol=1e-9)
+    np.testing.assert_allclose(ff[5][inner], 0, atol=1e-9)
+    np.testing.assert_allclose(ff[6][inner], 2 * kreg, rtol=1e-10)
+    rng = np.random.default_rng(0)
+    mask, uimp, vimp = _imposed(n, rng)
+    st = {'u': uimp + 1, 'v': vimp - 1, 'p': np.zeros((n, n))}
+    ff = tape.eval_operator(
+        ops.make_cavity_operator(0.01, imposed=(mask, uimp, vimp),
+                                 alpha=alpha), dom, st)
+    for f in ff[3:5]:
+        np.testing.assert_allclose(np.mean(f**2), alpha**2 * n * n / 3,
+                                   rtol=1e-12)
+
+
+def test_cavity_extras_gradient_fd():
+    """The extra terms depend on live fields only: their share of the tape
+    gradient (with minus without them) equals FD of their loss."""
+    n = 6
+    rng = np.random.default_rng(4)
+    base = {k: rng.uniform(-1, 1, (n, n)) for k in 'uvp'}
+    dom = ops.cavity_domain(n)
+    imp = _imposed(n, rng)
+    for bc in (0, 1, 2):
+        full = ops.make_cavity_operator(0.05, bc=bc, imposed=imp, alpha=0.7,
+                                        kreg=0.2)
+        plain = ops.make_cavity_operator(0.05, bc=bc)
+        _, g1, _ = tape.eval_loss_grad(full, dom, base)
+        _, g0, _ = tape.eval_loss_grad(plain, dom, base)
+
+        def xloss(name, x):
+            st = dict(base)
+            st[name] = x
+            ff = tape.eval_operator(full, dom, st)
+            return float(sum(np.mean(f**2) for f in ff[3:]))
+
+        for name in 'uvp':
+            fd = _fd_grad(lambda x: xloss(name, x), base[name], 1e-6)
+            np.testing.assert_allclose(g1[name] - g0[name], fd, rtol=1e-5,
+                                       atol=1e-7 * (np.abs(fd).max() + 1))
