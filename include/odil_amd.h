@@ -365,6 +365,17 @@ int odil_stencil_galerkin(int dtype, int ndim, const int* nw, const int* oent,
 int odil_mg_restrict(int dtype, int ndim, const int* nw, int nvar,
                      const double* rw, int rmode, const void* const* fine,
                      void* const* coarse, void* stream);
+/* Cell-centred interpolation of one field from the grid nwc (host int[ndim],
+ * ndim 1..3) to 2 nwc per axis, weights [1,3,3,1]/4 per axis over a halo
+ * 2 SYMMETRIC - REFLECT (MultigridDecomp.interp_cell_field,
+ * multigrid.py:1300-1360); accumulate != 0 adds to fine. The node-grid
+ * interpolation (interp_node_field) is odil_mg_prolong_add. */
+int odil_interp_cell(int dtype, int ndim, const int* nwc, const void* coarse,
+                     void* fine, int accumulate, void* stream);
+/* Its adjoint, coarse = I^T fine (the gradient of MultigridDecomp's
+ * weights_to_field with respect to cell weights). */
+int odil_interp_cell_t(int dtype, int ndim, const int* nwc, const void* fine,
+                       void* coarse, void* stream);
 /* odil_mg_restrict followed by the coarse level's first weighted-Jacobi
  * sweep from the zero initial guess (smoother_jacobi, multigrid.py:543-552,
  * on the zeros of MultigridOp.step, multigrid.py:590-593):

@@ -754,6 +754,103 @@ __global__ void __launch_bounds__(256)
   for (int k = 0; k < NV; ++k) fine.p[k][p] += w * acc[k];
 }
 
+// Cell-centred interpolation n -> 2n per axis (MultigridDecomp.
+// interp_cell_field, multigrid.py:1300-1360): fine cell p takes weights
+// 3/4 on its coarse parent a = p / 2 and 1/4 on the neighbour a -/+ 1
+// (even/odd p), tensor product over axes, from the coarse field padded by
+// one halo cell whose value is 2 u[sym(q)] - u[refl(q)] (sym repeats the
+// edge cell, refl mirrors about it, on every axis at once: a corner halo is
+// 2 u_00 - u_11, as the reference's REFLECT/SYMMETRIC pads give).
+__device__ inline int pad_sym(int q, int n) { return q < 0 ? 0 : (q >= n ? n - 1 : q); }
+__device__ inline int pad_refl(int q, int n) {
+  return q < 0 ? (n > 1 ? 1 : 0) : (q >= n ? (n > 1 ? n - 2 : 0) : q);
+}
+
+template <class T, int DIM>
+__device__ inline T cell_padded(const T* u, const Grid3& c, int q0, int q1,
+                                int q2) {
+  const int s0 = pad_sym(q0, c.n0), s1 = DIM > 1 ? pad_sym(q1, c.n1) : 0,
+            s2 = DIM > 2 ? pad_sym(q2, c.n2) : 0;
+  const int r0 = pad_refl(q0, c.n0), r1 = DIM > 1 ? pad_refl(q1, c.n1) : 0,
+            r2 = DIM > 2 ? pad_refl(q2, c.n2) : 0;
+  const T us = u[((size_t)s0 * c.n1 + s1) * c.n2 + s2];
+  if (s0 == r0 && s1 == r1 && s2 == r2) return us;
+  const T ur = u[((size_t)r0 * c.n1 + r1) * c.n2 + r2];
+  return T(2) * us - ur;
+}
+
+// fine (+)= I coarse, fine grid 2 c per axis.
+template <class T, int DIM>
+__global__ void __launch_bounds__(256)
+    k_interp_cell(const T* __restrict__ u, T* __restrict__ fine, Grid3 c,
+                  Grid3 f, int accumulate) {
+  const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= f.size()) return;
+  int p0, p1, p2;
+  unflat(p, f, p0, p1, p2);
+  const int a0 = p0 >> 1, a1 = p1 >> 1, a2 = p2 >> 1;
+  const int b0 = a0 + ((p0 & 1) ? 1 : -1), b1 = a1 + ((p1 & 1) ? 1 : -1),
+            b2 = a2 + ((p2 & 1) ? 1 : -1);
+  T acc = 0;
+#pragma unroll
+  for (int m = 0; m < (1 << DIM); ++m) {
+    const bool n0 = m & 1, n1 = DIM > 1 && (m & 2), n2 = DIM > 2 && (m & 4);
+    const T w = T((n0 ? 1 : 3) * (DIM > 1 ? (n1 ? 1 : 3) : 1) *
+                  (DIM > 2 ? (n2 ? 1 : 3) : 1));
+    acc += w * cell_padded<T, DIM>(u, c, n0 ? b0 : a0,
+                                   DIM > 1 ? (n1 ? b1 : a1) : 0,
+                                   DIM > 2 ? (n2 ? b2 : a2) : 0);
+  }
+  acc *= T(1) / T(1 << (2 * DIM));
+  fine[p] = accumulate ? fine[p] + acc : acc;
+}
+
+// Adjoint: coarse = I^T fine. Gathered per coarse cell k (no atomics):
+// every fine cell within two cells of 2k in each axis re-derives its
+// stencil and adds the weight it gives u[k], including through the halo
+// (coefficient 2 at sym(q) == k, -1 at refl(q) == k).
+template <class T, int DIM>
+__global__ void __launch_bounds__(256)
+    k_interp_cell_t(const T* __restrict__ fine, T* __restrict__ u, Grid3 c,
+                    Grid3 f) {
+  const size_t kk = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk >= c.size()) return;
+  int k0, k1, k2;
+  unflat(kk, c, k0, k1, k2);
+  double acc = 0;
+  const int lo1 = DIM > 1 ? 2 * k1 - 3 : 0, hi1 = DIM > 1 ? 2 * k1 + 4 : 0;
+  const int lo2 = DIM > 2 ? 2 * k2 - 3 : 0, hi2 = DIM > 2 ? 2 * k2 + 4 : 0;
+  for (int p0 = max(2 * k0 - 3, 0); p0 <= min(2 * k0 + 4, f.n0 - 1); ++p0)
+    for (int p1 = max(lo1, 0); p1 <= min(hi1, f.n1 - 1); ++p1)
+      for (int p2 = max(lo2, 0); p2 <= min(hi2, f.n2 - 1); ++p2) {
+        const int a0 = p0 >> 1, a1 = p1 >> 1, a2 = p2 >> 1;
+        const int b0 = a0 + ((p0 & 1) ? 1 : -1),
+                  b1 = a1 + ((p1 & 1) ? 1 : -1),
+                  b2 = a2 + ((p2 & 1) ? 1 : -1);
+        double coef = 0;
+        for (int m = 0; m < (1 << DIM); ++m) {
+          const bool n0 = m & 1, n1 = DIM > 1 && (m & 2),
+                     n2 = DIM > 2 && (m & 4);
+          const int q0 = n0 ? b0 : a0, q1 = DIM > 1 ? (n1 ? b1 : a1) : 0,
+                    q2 = DIM > 2 ? (n2 ? b2 : a2) : 0;
+          const double w = (n0 ? 1 : 3) * (DIM > 1 ? (n1 ? 1 : 3) : 1) *
+                           (DIM > 2 ? (n2 ? 1 : 3) : 1);
+          const int s0 = pad_sym(q0, c.n0),
+                    s1 = DIM > 1 ? pad_sym(q1, c.n1) : 0,
+                    s2 = DIM > 2 ? pad_sym(q2, c.n2) : 0;
+          const int r0 = pad_refl(q0, c.n0),
+                    r1 = DIM > 1 ? pad_refl(q1, c.n1) : 0,
+                    r2 = DIM > 2 ? pad_refl(q2, c.n2) : 0;
+          const bool inner = s0 == r0 && s1 == r1 && s2 == r2;
+          if (s0 == k0 && s1 == k1 && s2 == k2) coef += inner ? w : 2 * w;
+          if (!inner && r0 == k0 && r1 == k1 && r2 == k2) coef -= w;
+        }
+        if (coef != 0)
+          acc += coef * (double)fine[((size_t)p0 * f.n1 + p1) * f.n2 + p2];
+      }
+  u[kk] = (T)(acc / (double)(1 << (2 * DIM)));
+}
+
 // Dense matrix of a square block stencil operator (stride 1), row-major,
 // n = nvar * N; used for the coarsest-level direct solve. sym != 0: the
 // entries are the upper half of a symmetric operator.
@@ -1337,6 +1434,74 @@ int odil_mg_prolong_add(int dtype, int ndim, const int* nw, int nvar,
 #undef ODIL_PROLONG_NV
 #undef ODIL_PROLONG
   return check_launch("mg_prolong_add");
+}
+
+// Cell-centred interpolation (MultigridDecomp.interp_cell_field) of one
+// field from the coarse grid nwc to 2 nwc per axis: fine = I coarse, or
+// fine += I coarse with accumulate != 0.
+int odil_interp_cell(int dtype, int ndim, const int* nwc, const void* coarse,
+                     void* fine, int accumulate, void* stream) {
+  if (ndim < 1 || ndim > 3 || !nwc || !coarse || !fine) {
+    set_error("interp_cell: bad arguments");
+    return kErrArg;
+  }
+  Grid3 c{nwc[0], ndim > 1 ? nwc[1] : 1, ndim > 2 ? nwc[2] : 1};
+  if (c.n0 < 1 || c.n1 < 1 || c.n2 < 1 || (double)c.size() * (1 << ndim) > 2.0e9) {
+    set_error("interp_cell: bad grid");
+    return kErrArg;
+  }
+  Grid3 f{2 * c.n0, ndim > 1 ? 2 * c.n1 : 1, ndim > 2 ? 2 * c.n2 : 1};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define ODIL_IC(T, D)                                                        \
+  hipLaunchKernelGGL((k_interp_cell<T, D>), dim3(nblk(f.size())), dim3(256), \
+                     0, s, static_cast<const T*>(coarse),                    \
+                     static_cast<T*>(fine), c, f, accumulate)
+#define ODIL_IC_D(T)  \
+  if (ndim == 1) ODIL_IC(T, 1); else if (ndim == 2) ODIL_IC(T, 2); else ODIL_IC(T, 3)
+  if (dtype == kF64) {
+    ODIL_IC_D(double);
+  } else if (dtype == kF32) {
+    ODIL_IC_D(float);
+  } else {
+    set_error("interp_cell: bad dtype %d", dtype);
+    return kErrArg;
+  }
+#undef ODIL_IC_D
+#undef ODIL_IC
+  return check_launch("interp_cell");
+}
+
+// Adjoint of odil_interp_cell: coarse = I^T fine (fine grid 2 nwc).
+int odil_interp_cell_t(int dtype, int ndim, const int* nwc, const void* fine,
+                       void* coarse, void* stream) {
+  if (ndim < 1 || ndim > 3 || !nwc || !coarse || !fine) {
+    set_error("interp_cell_t: bad arguments");
+    return kErrArg;
+  }
+  Grid3 c{nwc[0], ndim > 1 ? nwc[1] : 1, ndim > 2 ? nwc[2] : 1};
+  if (c.n0 < 1 || c.n1 < 1 || c.n2 < 1 || (double)c.size() * (1 << ndim) > 2.0e9) {
+    set_error("interp_cell_t: bad grid");
+    return kErrArg;
+  }
+  Grid3 f{2 * c.n0, ndim > 1 ? 2 * c.n1 : 1, ndim > 2 ? 2 * c.n2 : 1};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define ODIL_ICT(T, D)                                                        \
+  hipLaunchKernelGGL((k_interp_cell_t<T, D>), dim3(nblk(c.size())),           \
+                     dim3(256), 0, s, static_cast<const T*>(fine),            \
+                     static_cast<T*>(coarse), c, f)
+#define ODIL_ICT_D(T) \
+  if (ndim == 1) ODIL_ICT(T, 1); else if (ndim == 2) ODIL_ICT(T, 2); else ODIL_ICT(T, 3)
+  if (dtype == kF64) {
+    ODIL_ICT_D(double);
+  } else if (dtype == kF32) {
+    ODIL_ICT_D(float);
+  } else {
+    set_error("interp_cell_t: bad dtype %d", dtype);
+    return kErrArg;
+  }
+#undef ODIL_ICT_D
+#undef ODIL_ICT
+  return check_launch("interp_cell_t");
 }
 
 // Dense FP64 matrix (zero-initialised by the caller) of a square stride-1

@@ -346,3 +346,46 @@ def test_cavity_extras_gradient_fd():
             fd = _fd_grad(lambda x: xloss(name, x), base[name], 1e-6)
             np.testing.assert_allclose(g1[name] - g0[name], fd, rtol=1e-5,
                                        atol=1e-7 * (np.abs(fd).max() + 1))
+
+
+# -- MultigridDecomp restatement (multigrid.py:1217-1426) --------------------
+@pytest.mark.parametrize('shape', [(5, ), (4, 6), (3, 4, 5)])
+def test_decomp_conv_equals_manual(shape):
+    """The 'conv' and 'manual' forms of the reference are independent
+    restatements of one map: they agree (node grids dims 1-3, cell grids
+    dims 1-2 as the reference's manual form supports)."""
+    from oracle import decomp as od
+    rng = np.random.default_rng(len(shape))
+    u = rng.standard_normal(shape)
+    for depth in (1, 2):
+        np.testing.assert_allclose(od.interp_node_conv(u, depth),
+                                   od.interp_node_manual(u, depth),
+                                   rtol=1e-13, atol=1e-13)
+        if len(shape) <= 2:
+            np.testing.assert_allclose(od.interp_cell_conv(u, depth),
+                                       od.interp_cell_manual(u, depth),
+                                       rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize('shape', [(5, ), (4, 6), (3, 4, 5)])
+def test_decomp_linear_exact(shape):
+    """Both interpolations reproduce linear fields exactly: node values at
+    the fine nodes x_f = x_c / 2, cell values at the fine centres."""
+    from oracle import decomp as od
+    a = np.array([0.3, -1.2, 0.7])[:len(shape)]
+    nodes = np.meshgrid(*[np.arange(n, dtype=float) for n in shape],
+                        indexing='ij')
+    lin = 0.5 + sum(ai * x for ai, x in zip(a, nodes))
+    fine_n = np.meshgrid(*[np.arange(2 * n - 1) / 2. for n in shape],
+                         indexing='ij')
+    np.testing.assert_allclose(od.interp_node_conv(lin),
+                               0.5 + sum(ai * x for ai, x in
+                                         zip(a, fine_n)), atol=1e-13)
+    centres = np.meshgrid(*[np.arange(n) + 0.5 for n in shape],
+                          indexing='ij')
+    lin = 0.5 + sum(ai * x for ai, x in zip(a, centres))
+    fine_c = np.meshgrid(*[(np.arange(2 * n) + 0.5) / 2. for n in shape],
+                         indexing='ij')
+    np.testing.assert_allclose(od.interp_cell_conv(lin),
+                               0.5 + sum(ai * x for ai, x in
+                                         zip(a, fine_c)), atol=1e-13)
