@@ -156,7 +156,7 @@ class NewtonMultigrid:
                  galerkin_mode='exact', precision='fp64', linsolver='pcg',
                  rtol=0., check_every=10, coarse_reg='auto',
                  matrix_free=True, use_graph=True, setup_every=1,
-                 krylov='auto', refine=0, device=None):
+                 krylov='auto', refine=0, overlap_setup=False, device=None):
         if precision not in ('fp64', 'mixed'):
             raise ValueError('precision must be fp64 or mixed')
         if linsolver not in ('vcycle', 'pcg'):
@@ -206,10 +206,19 @@ class NewtonMultigrid:
             # (~1e-5 sigma_max, squared 1e-10): 1e-10 would damp the
             # smoothest coarse modes and slows Newton down threefold.
             coarse_reg = 1e-14 if precision == 'fp64' else 1e-12
-        self.mg = MultigridOp(op.shape, nvar=self.nvar, nlevels=nlvl,
-                              restriction=restriction, dtype=self.wdtype,
-                              device=self.device, coarse_reg=coarse_reg,
-                              galerkin_mode=galerkin_mode)
+        mk = lambda: MultigridOp(op.shape, nvar=self.nvar, nlevels=nlvl,
+                                 restriction=restriction, dtype=self.wdtype,
+                                 device=self.device, coarse_reg=coarse_reg,
+                                 galerkin_mode=galerkin_mode)
+        self.mg = mk()
+        # overlap_setup: rebuilds after the first run on a second stream
+        # into a second hierarchy while the current step's linear solve
+        # runs, and are adopted at the next step (one step more lag).
+        self.overlap_setup = bool(overlap_setup) and matrix_free
+        self._mgs = [self.mg, mk()] if self.overlap_setup else [self.mg]
+        self._side = torch.cuda.Stream(self.device) \
+            if self.overlap_setup else None
+        self._pending = None
         self.vec = VectorOps(self.device)
         self.sc = DeviceScalars(8, self.device)
         self.N = None
@@ -340,7 +349,7 @@ class NewtonMultigrid:
         if not fresh:
             self._pcg_iters(iters)
             return
-        key = (iters, finish)
+        key = (iters, finish, id(self.mg))
         if not self.use_graph:
             self._solve_body(iters, finish)
             return
@@ -395,19 +404,26 @@ class NewtonMultigrid:
         if tol is not None and max(rms) <= tol:
             self.last_linear = 0
             return rms
+        main = torch.cuda.current_stream()
+        if self._pending is not None:
+            ev, mg = self._pending
+            main.wait_event(ev)
+            self.mg = mg
+            self._pending = None
+        side = None
         if (self._nstep % self.setup_every == 0 or not self.mg.AA
                 or not self.matrix_free):
-            jac, _ = self.op.jacobian(fields)
-            J = BlockStencil.from_jacobian(jac)
-            if self.N is None:
-                self.N = normal_operator(J, self.wdtype, sym=self.sym)
+            if self.overlap_setup and self.mg.AA:
+                side = torch.cuda.Event()
+                side.record()
             else:
-                normal_operator(J, out=self.N)
-            if not self.matrix_free:
-                self._N64 = self.N if self.wdtype == torch.float64 else \
-                    normal_operator(J, torch.float64, sym=self.sym)
-            self.mg.update_A(self.N)
-            self.mg.prepare(self.ndirect)
+                self._build(fields, self.mg)
+                for mg in self._mgs:
+                    if not mg.AA:
+                        # First build of the other hierarchy here, on this
+                        # stream: it allocates and tunes its launch shapes.
+                        mg.update_A(self.N)
+                        mg.prepare(self.ndirect)
         self._nstep += 1
         if self.matrix_free:
             if self.state_w is not fields:
@@ -417,10 +433,47 @@ class NewtonMultigrid:
             self.mg.fine = self.fine
         self.timers = {'setup': time.perf_counter() - t0}
         its = self._linear_solve()
+        if side is not None:
+            # Queued after the solve so that its launches do not delay the
+            # solve's; the side Jacobian reads the fields before the update.
+            main.wait_event(self._setup_side(fields, side))
         for f, d in zip(fields, self.x):
             f += d
         self.last_linear = its
         return rms
+
+    def _build(self, fields, mg):
+        """J, N = J^T J (stored), Galerkin hierarchy and coarsest solution
+        operator of `mg`, on the current stream; returns after queueing the
+        Jacobian and the rest (the Jacobian is the only reader of fields)."""
+        jac, _ = self.op.jacobian(fields)
+        ev = torch.cuda.Event()
+        ev.record()
+        J = BlockStencil.from_jacobian(jac)
+        if self.N is None:
+            self.N = normal_operator(J, self.wdtype, sym=self.sym)
+        else:
+            normal_operator(J, out=self.N)
+        if not self.matrix_free:
+            self._N64 = self.N if self.wdtype == torch.float64 else \
+                normal_operator(J, torch.float64, sym=self.sym)
+        mg.update_A(self.N)
+        mg.prepare(self.ndirect)
+        return ev
+
+    def _setup_side(self, fields, after):
+        """Rebuild the inactive hierarchy on the side stream after event
+        `after`; it is adopted at the next step. Returns the event after
+        which the fields may change."""
+        target = self._mgs[1] if self.mg is self._mgs[0] else self._mgs[0]
+        target.fine = self.fine
+        self._side.wait_event(after)
+        with torch.cuda.stream(self._side):
+            jac_ready = self._build(fields, target)
+            done = torch.cuda.Event()
+            done.record()
+        self._pending = (done, target)
+        return jac_ready
 
     def run(self, fields, epochs, tol=None, callback=None):
         """Newton iterations until the max per-equation RMS residual is
