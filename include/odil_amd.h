@@ -482,6 +482,10 @@ int odil_adam_step(int dtype, long long N, void* x, const void* g, void* m,
  * GdOptimizer, optimizer.py:241-268); out may alias x or y. */
 int odil_waxpy(int dtype, long long N, void* out, const void* x,
                const void* y, double a, void* stream);
+/* out = a x + b y (recurrences of the iterative normal-equation solvers
+ * 'lsqr' and 'bicgstab', linsolver.py:59-101); out may alias x or y. */
+int odil_axpby(int dtype, long long N, void* out, const void* x,
+               const void* y, double a, double b, void* stream);
 
 #ifdef __cplusplus
 }

@@ -187,6 +187,16 @@ __global__ void __launch_bounds__(256)
     out[i] = x[i] + aa * y[i];
 }
 
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_axpby(T* out, const T* x, const T* y, size_t N, double a, double b) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const T aa = T(a), bb = T(b);
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+       i += stride)
+    out[i] = aa * x[i] + bb * y[i];
+}
+
 // One weighted-Jacobi sweep from a zero iterate, y_k = omega f_k / d_k
 // (x + omega (f - A x) / d at x = 0, same rounding), optionally writing
 // fw_k = f_k converted to the y precision: the first pre-smoothing sweep of
@@ -518,6 +528,30 @@ int odil_waxpy(int dtype, long long N, void* out, const void* x,
     return kErrArg;
   }
   return check_launch("waxpy");
+}
+
+// out = a x + b y on N values.
+int odil_axpby(int dtype, long long N, void* out, const void* x,
+               const void* y, double a, double b, void* stream) {
+  if (N < 0 || !out || !x || !y) {
+    set_error("axpby: bad arguments");
+    return kErrArg;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_axpby<double>, dim3(grid_for(N)), dim3(256), 0, s,
+                       static_cast<double*>(out),
+                       static_cast<const double*>(x),
+                       static_cast<const double*>(y), (size_t)N, a, b);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_axpby<float>, dim3(grid_for(N)), dim3(256), 0, s,
+                       static_cast<float*>(out), static_cast<const float*>(x),
+                       static_cast<const float*>(y), (size_t)N, a, b);
+  else {
+    set_error("axpby: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  return check_launch("axpby");
 }
 
 }  // extern "C"

@@ -29,7 +29,8 @@ def optimize_newton(problem, state, epochs, args=None, callback=None,
                     tol=None):
     """util.optimize_newton (util.py:240-268): at each epoch linearize
     (residual and stencil Jacobian on the device), solve the normal
-    equations for the correction (`linsolver.solve`, direct on the GPU) and
+    equations for the correction (`linsolver.solve` with `args.linsolver`,
+    default 'direct', every choice on the GPU) and
     update the fields in place (`packed += dpacked`). `callback(epoch,
     state, rms)` runs before each update. Stops early when
     max(rms) <= tol. Returns the history [(epoch, rms, seconds)]."""
@@ -46,7 +47,7 @@ def optimize_newton(problem, state, epochs, args=None, callback=None,
         if epoch == epochs or (tol is not None and max(rms) <= tol):
             break
         dx = ls.solve(J, [-r for r in res], args,
-                      linsolver='direct')
+                      linsolver=getattr(args, 'linsolver', None) or 'direct')
         for k, d in zip(names, dx):
             state.fields[k] += d
     return hist

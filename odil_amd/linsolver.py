@@ -1,27 +1,47 @@
 """Normal-equation solves of the Newton step on the device
 (`linsolver.solve`, linsolver.py:28-107).
 
-`solve(J, rhs, linsolver=...)` solves (J^T J + beta^2 I +
-betadiag^2 diag(J^T J)) dx = J^T rhs for a stencil Jacobian J
-(`BlockStencil`) and residual-shaped rhs fields:
+`solve(J, rhs, args, dhistory, linsolver)` returns dx minimising
+|J dx - rhs|^2 (+ regularisation) for a stencil Jacobian J (`BlockStencil`,
+equations x fields) and residual-shaped rhs fields. The reference's choices,
+all on the GPU:
 
-* 'direct': J^T J in stencil form (`odil_stencil_normal`), its dense FP64
-  matrix (`odil_stencil_to_dense`) and a Cholesky factorisation on the GPU
-  (rocSOLVER through torch.linalg) — the role of SuperLU with MMD_ATA in the
-  reference, for systems up to `max_direct` unknowns. Singular normal
-  equations (e.g. a pressure null mode) need beta > 0 here; the multigrid
-  solver handles them without.
-* 'multigrid': flexible CG on J^T J preconditioned by the GMG V-cycle
-  (`odil_amd.multigrid.MultigridOp`), the solver of
-  `odil_amd.newton.NewtonMultigrid`.
+* 'direct' / 'direct_cu' (SuperLU / cuSolver `spsolve` of
+  A = J^T J + beta^2 I + betadiag^2 diag, linsolver.py:35-53): A in stencil
+  form (`odil_stencil_normal`), its dense FP64 matrix
+  (`odil_stencil_to_dense`) and a Cholesky factorisation, for systems up to
+  `max_direct` unknowns. Singular normal equations (e.g. a pressure null
+  mode) need beta > 0 here.
+* 'lsqr' / 'lsqr_cu' (SciPy / CuPy `lsqr` on J itself with damp = beta,
+  atol = btol = linsolver_tol, linsolver.py:57-75): the Paige-Saunders
+  LSQR bidiagonalisation with J and J^T applied by the stencil kernels and
+  the recurrences in `odil_axpby` / `odil_dot`; records the residual norm
+  of the normal equations, the estimates of |J| and cond(J) and the
+  iteration count like the reference.
+* 'bicgstab' (SciPy `bicgstab` on A with tol = 0, atol = linsolver_tol,
+  linsolver.py:88-101): van der Vorst's BiCGSTAB on the stored symmetric A.
+* 'multigrid' (PyAMG smoothed aggregation as a CG preconditioner,
+  tol = linsolver_tol relative to |b|, linsolver.py:77-87): CG on the stored
+  A preconditioned by one geometric V(smooth_pre, smooth_post) cycle
+  (`odil_amd.multigrid.MultigridOp`: Galerkin levels, weighted Jacobi,
+  dense coarsest solve). Node grids of odd size only (2^k + 1 for full
+  depth). The Newton driver with matrix-free operators and captured graphs
+  is `odil_amd.newton.NewtonMultigrid`.
+
+'sparseqr' (SuiteSparseQR) has no GPU counterpart here and raises
+ValueError.
 """
 
+import ctypes
+import math
 from argparse import Namespace
 
 import numpy as np
 import torch
 
-from odil_amd.multigrid import BlockStencil, normal_operator
+from odil_amd import _lib
+from odil_amd._lib import c_double, c_int, c_vp, check, ptr, ptrs, stream
+from odil_amd.multigrid import BlockStencil, MultigridOp, normal_operator
 
 
 def _arg(args, name, default):
@@ -31,6 +51,274 @@ def _arg(args, name, default):
     return default if v is None else v
 
 
+class _Vec:
+    """FP64 device vectors of k fields stored as one (k, *grid) buffer; the
+    recurrences run in `odil_axpby`, the norms in `odil_dot` (fixed-order
+    FP64 sums)."""
+
+    def __init__(self, device):
+        self.partial = torch.zeros(_lib.lib().odil_dot_partials(),
+                                   dtype=torch.float64, device=device)
+        self.out = torch.zeros(2, dtype=torch.float64, device=device)
+
+    def dot(self, x, y):
+        check(_lib.lib().odil_dot(
+            c_int(_lib.dtype_tag(x.dtype)), c_int(1),
+            ctypes.c_longlong(x.numel()), ptrs([x]), ptrs([y]),
+            c_vp(self.partial.data_ptr()), c_vp(self.out.data_ptr()),
+            c_int(0), stream()), 'dot')
+        return self.out[0].item()
+
+    def norm(self, x):
+        return math.sqrt(self.dot(x, x))
+
+    def axpby(self, out, x, y, a, b):
+        """out = a x + b y (out may alias x or y)."""
+        check(_lib.lib().odil_axpby(
+            c_int(_lib.dtype_tag(x.dtype)), ctypes.c_longlong(x.numel()),
+            ptr(out), ptr(x), ptr(y), c_double(a), c_double(b), stream()),
+            'axpby')
+        return out
+
+    def scale(self, x, a):
+        return self.axpby(x, x, x, a, 0.)
+
+
+def _stack(fields):
+    return torch.stack([f.to(torch.float64) for f in fields]).contiguous()
+
+
+def _regularised_normal(matr, beta, betadiag):
+    """A = J^T J (stored upper half, FP64) with the diagonal regularisation
+    of linsolver.py:37-40: (diag + beta^2) (1 + betadiag^2)."""
+    N = normal_operator(matr, torch.float64, sym=True)
+    if beta or betadiag:
+        for d in N.diagonal():
+            d.add_(beta**2).mul_(1. + betadiag**2)
+    return N
+
+
+def lsqr(A, b, damp=0., atol=1e-6, btol=1e-6, conlim=1e8, iter_lim=None):
+    """LSQR (Paige and Saunders, ACM TOMS 8:43-71, 1982) for
+    min |A x - b|^2 + damp^2 |x|^2 with a stencil operator A (fields in,
+    equations out) and b stacked as (neq, *grid). Returns
+    (x, istop, itn, r1norm, r2norm, anorm, acond, arnorm, xnorm) with x
+    stacked as (nvar, *grid); the stopping rules and estimates are those of
+    the paper (istop 1-2: residual / least-squares tolerance met, 3: cond
+    limit, 4-6: the same at machine precision, 7: iteration limit)."""
+    vec = _Vec(b.device)
+    nvar = A.nvar_in
+    n = nvar * int(np.prod(A.nw))
+    if iter_lim is None:
+        iter_lim = 2 * n
+    eps = np.finfo(np.float64).eps
+    ctol = 1. / conlim if conlim > 0 else 0.
+    shape_x = (nvar, ) + tuple(A.nw)
+    x = torch.zeros(shape_x, dtype=torch.float64, device=b.device)
+    u = b.clone()
+    v = torch.empty_like(x)
+    tmp_u = torch.empty_like(u)
+    tmp_v = torch.empty_like(x)
+    beta = vec.norm(u)
+    alpha = 0.
+    if beta > 0:
+        vec.scale(u, 1. / beta)
+        A.apply_t(list(u), y=list(v))
+        alpha = vec.norm(v)
+    if alpha > 0:
+        vec.scale(v, 1. / alpha)
+    w = v.clone()
+    anorm = acond = ddnorm = res2 = xnorm = xxnorm = z = 0.
+    cs2, sn2 = -1., 0.
+    dampsq = damp**2
+    rhobar, phibar = alpha, beta
+    bnorm = rnorm = r1norm = r2norm = beta
+    arnorm = alpha * beta
+    istop, itn = 0, 0
+    if arnorm == 0:
+        return x, istop, itn, r1norm, r2norm, anorm, acond, arnorm, xnorm
+    while itn < iter_lim:
+        itn += 1
+        # Bidiagonalisation: beta u = A v - alpha u, alpha v = A^T u - beta v.
+        A.apply(list(v), y=list(tmp_u))
+        vec.axpby(u, tmp_u, u, 1., -alpha)
+        beta = vec.norm(u)
+        if beta > 0:
+            vec.scale(u, 1. / beta)
+            anorm = math.sqrt(anorm**2 + alpha**2 + beta**2 + dampsq)
+            A.apply_t(list(u), y=list(tmp_v))
+            vec.axpby(v, tmp_v, v, 1., -beta)
+            alpha = vec.norm(v)
+            if alpha > 0:
+                vec.scale(v, 1. / alpha)
+        # Plane rotation eliminating the damping term, then the one that
+        # eliminates the subdiagonal beta of the bidiagonal matrix.
+        if damp > 0:
+            rhobar1 = math.sqrt(rhobar**2 + dampsq)
+            cs1, sn1 = rhobar / rhobar1, damp / rhobar1
+            psi = sn1 * phibar
+            phibar = cs1 * phibar
+        else:
+            rhobar1, psi = rhobar, 0.
+        rho = math.sqrt(rhobar1**2 + beta**2)
+        cs, sn = rhobar1 / rho, beta / rho
+        theta = sn * alpha
+        rhobar = -cs * alpha
+        phi = cs * phibar
+        phibar = sn * phibar
+        tau = sn * phi
+        # x += (phi / rho) w; w = v - (theta / rho) w; |w / rho|^2 feeds
+        # the condition estimate.
+        ddnorm += vec.dot(w, w) / rho**2
+        vec.axpby(x, x, w, 1., phi / rho)
+        vec.axpby(w, v, w, 1., -theta / rho)
+        # Norm of x from the rotations of the lower bidiagonal system.
+        delta = sn2 * rho
+        gambar = -cs2 * rho
+        rhs = phi - delta * z
+        zbar = rhs / gambar
+        xnorm = math.sqrt(xxnorm + zbar**2)
+        gamma = math.sqrt(gambar**2 + theta**2)
+        cs2, sn2 = gambar / gamma, theta / gamma
+        z = rhs / gamma
+        xxnorm += z**2
+        acond = anorm * math.sqrt(ddnorm)
+        res2 += psi**2
+        rnorm = math.sqrt(phibar**2 + res2)
+        arnorm = alpha * abs(tau)
+        r1sq = rnorm**2 - dampsq * xxnorm
+        r1norm = math.copysign(math.sqrt(abs(r1sq)), r1sq)
+        r2norm = rnorm
+        test1 = rnorm / bnorm
+        test2 = arnorm / (anorm * rnorm + eps)
+        test3 = 1. / (acond + eps)
+        t1 = test1 / (1. + anorm * xnorm / bnorm)
+        rtol = btol + atol * anorm * xnorm / bnorm
+        if itn >= iter_lim:
+            istop = 7
+        if 1. + test3 <= 1.:
+            istop = 6
+        if 1. + test2 <= 1.:
+            istop = 5
+        if 1. + t1 <= 1.:
+            istop = 4
+        if test3 <= ctol:
+            istop = 3
+        if test2 <= atol:
+            istop = 2
+        if test1 <= rtol:
+            istop = 1
+        if istop:
+            break
+    return x, istop, itn, r1norm, r2norm, anorm, acond, arnorm, xnorm
+
+
+def bicgstab(A, b, atol=1e-6, maxiter=50, callback=None):
+    """BiCGSTAB (van der Vorst, SIAM J. Sci. Stat. Comput. 13:631-644, 1992)
+    on a square stencil operator A from x = 0, unpreconditioned, stopping
+    when |r| < atol (relative tolerance 0, as the reference passes).
+    `callback(x)` after every iteration. Returns (x, info): info 0 when
+    converged, else the iterations run."""
+    vec = _Vec(b.device)
+    x = torch.zeros_like(b)
+    r = b.clone()
+    rt = b.clone()
+    p = torch.zeros_like(b)
+    v = torch.zeros_like(b)
+    t = torch.empty_like(b)
+    rho_prev = alpha = omega = 1.
+    if vec.norm(r) < atol:
+        return x, 0
+    for it in range(maxiter):
+        rho = vec.dot(rt, r)
+        if rho == 0.:
+            return x, -10
+        if it == 0:
+            p.copy_(r)
+        else:
+            # p = r + (rho / rho_prev) (alpha / omega) (p - omega v)
+            bb = (rho / rho_prev) * (alpha / omega)
+            vec.axpby(p, p, v, 1., -omega)
+            vec.axpby(p, r, p, 1., bb)
+        A.apply(list(p), y=list(v))
+        alpha = rho / vec.dot(rt, v)
+        vec.axpby(r, r, v, 1., -alpha)  # r holds s = r - alpha v
+        if vec.norm(r) < atol:
+            vec.axpby(x, x, p, 1., alpha)
+            if callback is not None:
+                callback(x)
+            return x, 0
+        A.apply(list(r), y=list(t))
+        omega = vec.dot(t, r) / vec.dot(t, t)
+        vec.axpby(x, x, p, 1., alpha)
+        vec.axpby(x, x, r, 1., omega)
+        vec.axpby(r, r, t, 1., -omega)
+        if callback is not None:
+            callback(x)
+        if vec.norm(r) < atol:
+            return x, 0
+        if omega == 0.:
+            return x, -11
+        rho_prev = rho
+    return x, maxiter
+
+
+def _mg_levels(nw, nlvl):
+    """Levels of a node hierarchy on `nw`: halve while every size is odd
+    and above 3 (at most nlvl levels when given)."""
+    levels, n = 1, list(nw)
+    while all(k % 2 == 1 and k > 3 for k in n):
+        if nlvl and levels >= nlvl:
+            break
+        n = [(k - 1) // 2 + 1 for k in n]
+        levels += 1
+    return levels
+
+
+def pcg_multigrid(A, b, tol=1e-6, maxiter=50, nlvl=None, omega=0.6,
+                  pre=2, post=2, ndirect=3):
+    """CG on the stored symmetric A preconditioned by one V(pre, post) cycle
+    from zero per iteration; stops when |r| <= tol |b|. Returns
+    (x, residual norms including the initial one)."""
+    vec = _Vec(b.device)
+    if any(k % 2 == 0 for k in A.nw):
+        raise ValueError("linsolver='multigrid' needs odd node grids "
+                         '(2^k + 1 for full depth), got {}'.format(A.nw))
+    mg = MultigridOp(A.nw, nvar=A.nvar_in, nlevels=_mg_levels(A.nw, nlvl),
+                     restriction='transpose', dtype=torch.float64,
+                     device=b.device, coarse_reg=1e-14,
+                     galerkin_mode='exact')
+    mg.update_A(A)
+    mg.prepare(ndirect)
+    x = torch.zeros_like(b)
+    r = b.clone()
+    z = torch.zeros_like(b)
+    q = torch.empty_like(b)
+    bnorm = vec.norm(b)
+    res = [bnorm]
+    if bnorm == 0.:
+        return x, res
+    zl = list(z)
+    mg.vcycle(zl, list(r), omega=omega, ndirect=ndirect, pre=pre,
+              post=post, zero=True)
+    p = z.clone()
+    rz = vec.dot(r, z)
+    for _ in range(maxiter):
+        A.apply(list(p), y=list(q))
+        a = rz / vec.dot(p, q)
+        vec.axpby(x, x, p, 1., a)
+        vec.axpby(r, r, q, 1., -a)
+        res.append(vec.norm(r))
+        if res[-1] <= tol * bnorm:
+            break
+        mg.vcycle(zl, list(r), omega=omega, ndirect=ndirect, pre=pre,
+                  post=post, zero=True)
+        rz_new = vec.dot(r, z)
+        vec.axpby(p, z, p, 1., rz_new / rz)
+        rz = rz_new
+    return x, res
+
+
 def solve(matr, rhs, args=None, dhistory=None, linsolver='direct',
           max_direct=40000):
     """Returns the correction fields dx (list of device tensors, FP64).
@@ -38,15 +326,39 @@ def solve(matr, rhs, args=None, dhistory=None, linsolver='direct',
     matr: `BlockStencil` Jacobian (equations x fields), FP64.
     rhs: list of residual-shaped device tensors (one per equation), usually
          -f as in `optimize_newton` (util.py:261-267).
-    args: optional namespace with `beta`, `betadiag`, `linsolver_maxiter`,
-          `nlvl`, `omega` (linsolver.py:110-170)."""
+    args: optional namespace with `beta`, `betadiag`, `linsolver_maxiter`
+          (default 1000 for 'lsqr', else 50, linsolver.py:29-33),
+          `linsolver_tol`, `nlvl`, `omega`, `smooth_pre`, `smooth_post`,
+          `ndirect` (linsolver.py:108-170).
+    dhistory: optional dict receiving `linsolver_niter` and, for the
+          iterative solvers, `linsolver_residual` (and `linsolver_anorm`,
+          `linsolver_acond` for 'lsqr') as in the reference."""
     if not isinstance(matr, BlockStencil):
         raise ValueError('solve expects a BlockStencil Jacobian')
     beta = _arg(args, 'beta', 0.)
     betadiag = _arg(args, 'betadiag', 0.)
+    tol = _arg(args, 'linsolver_tol', 1e-6)
+    lsqr_like = linsolver in ('lsqr', 'lsqr_cu')
+    maxiter = _arg(args, 'linsolver_maxiter', 1000 if lsqr_like else 50)
+    nvar = matr.nvar_in
+    dh = dhistory if dhistory is not None else {}
+    if lsqr_like:
+        out = lsqr(matr, _stack(rhs), damp=beta, atol=tol, btol=tol,
+                   iter_lim=maxiter)
+        x, _, itn, _, _, anorm, acond, arnorm = out[:8]
+        dh['linsolver_residual'] = arnorm
+        dh['linsolver_anorm'] = anorm
+        dh['linsolver_acond'] = acond
+        dh['linsolver_niter'] = itn
+        return [x[k].contiguous() for k in range(nvar)]
+    if linsolver == 'sparseqr':
+        raise ValueError("linsolver='sparseqr' (SuiteSparseQR) is not "
+                         "provided; 'lsqr' solves the same least-squares "
+                         'problem on the GPU')
+    if linsolver not in ('direct', 'direct_cu', 'bicgstab', 'multigrid'):
+        raise ValueError('Unknown linsolver=' + str(linsolver))
     b = matr.apply_t(rhs)
-    if linsolver == 'direct':
-        nvar = matr.nvar_in
+    if linsolver in ('direct', 'direct_cu'):
         n = nvar * int(np.prod(matr.nw))
         if n > max_direct:
             raise ValueError('direct solve limited to {} unknowns, got {}; '
@@ -65,25 +377,46 @@ def solve(matr, rhs, args=None, dhistory=None, linsolver='direct',
         sol = torch.cholesky_solve(
             torch.cat([x.reshape(-1) for x in b]).unsqueeze(1), L)
         sol = sol.reshape(nvar, *matr.nw)
-        if dhistory is not None:
-            dhistory['linsolver_niter'] = 1
+        dh['linsolver_niter'] = 1
         return [sol[k].contiguous() for k in range(nvar)]
-    if linsolver == 'multigrid':
-        raise ValueError("linsolver='multigrid' runs through "
-                         'odil_amd.newton.NewtonMultigrid (it keeps the '
-                         'hierarchy and graphs across Newton steps)')
-    raise ValueError('Unknown linsolver=' + str(linsolver))
+    A = _regularised_normal(matr, beta, betadiag)
+    bb = _stack(b)
+    if linsolver == 'bicgstab':
+        vec = _Vec(bb.device)
+        res = []
+        tmp = torch.empty_like(bb)
+
+        def callback(x):
+            # RMS of A x - b (linsolver.py:90-92)
+            A.residual(list(x), list(bb), y=list(tmp))
+            res.append(vec.norm(tmp) / math.sqrt(tmp.numel()))
+        x, _ = bicgstab(A, bb, atol=tol, maxiter=maxiter, callback=callback)
+        if res:
+            dh['linsolver_residual'] = res[-1]
+        dh['linsolver_niter'] = len(res)
+        return [x[k].contiguous() for k in range(nvar)]
+    x, res = pcg_multigrid(
+        A, bb, tol=tol, maxiter=maxiter, nlvl=_arg(args, 'nlvl', None),
+        omega=_arg(args, 'omega', 0.6), pre=_arg(args, 'smooth_pre', 2),
+        post=_arg(args, 'smooth_post', 2), ndirect=_arg(args, 'ndirect', 3))
+    dh['linsolver_residual'] = res[-1]
+    dh['linsolver_niter'] = len(res)
+    return [x[k].contiguous() for k in range(nvar)]
 
 
 def add_arguments(parser):
-    """Solver options with the reference's names (linsolver.py:110-170)."""
+    """Solver options with the reference's names and defaults
+    (linsolver.py:108-170)."""
     parser.add_argument('--linsolver', default='direct',
-                        choices=['direct', 'multigrid'])
+                        choices=['multigrid', 'direct', 'direct_cu',
+                                 'sparseqr', 'lsqr', 'lsqr_cu', 'bicgstab'])
     parser.add_argument('--linsolver_maxiter', type=int, default=None)
+    parser.add_argument('--linsolver_tol', type=float, default=1e-6)
     parser.add_argument('--beta', type=float, default=0.)
     parser.add_argument('--betadiag', type=float, default=0.)
-    parser.add_argument('--nlvl', type=int, default=None)
-    parser.add_argument('--omega', type=float, default=0.5)
+    parser.add_argument('--linsolver_verbose', type=int, default=0)
+    parser.add_argument('--nlvl', type=int, default=5)
+    parser.add_argument('--omega', type=float, default=0.6)
     parser.add_argument('--smooth_pre', type=int, default=2)
     parser.add_argument('--smooth_post', type=int, default=2)
     parser.add_argument('--ndirect', type=int, default=3)
@@ -91,8 +424,9 @@ def add_arguments(parser):
 
 
 def default_args(**kw):
-    a = Namespace(linsolver='direct', linsolver_maxiter=None, beta=0.,
-                  betadiag=0., nlvl=None, omega=0.5, smooth_pre=2,
+    a = Namespace(linsolver='direct', linsolver_maxiter=None,
+                  linsolver_tol=1e-6, beta=0., betadiag=0.,
+                  linsolver_verbose=0, nlvl=5, omega=0.6, smooth_pre=2,
                   smooth_post=2, ndirect=3)
     for k, v in kw.items():
         setattr(a, k, v)
