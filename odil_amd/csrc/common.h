@@ -96,6 +96,20 @@ __device__ inline void block_sum_n(T (&v)[N], T* buf) {
   }
 }
 
+// Wavefront sums of N values stored as one partial per wavefront,
+// partial[e * nparts + part] (written by lane 0): no LDS round trip and no
+// barrier, so the waves of a workgroup never wait for each other here.
+template <int N>
+__device__ inline void wave_partials(double (&v)[N], double* partial,
+                                     size_t nparts, size_t part) {
+#pragma unroll
+  for (int e = 0; e < N; ++e) v[e] = wave_sum(v[e]);
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int e = 0; e < N; ++e) partial[(size_t)e * nparts + part] = v[e];
+  }
+}
+
 __host__ __device__ inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 // Halo rule at a wall: the value outside the domain is extrapolated from the

@@ -87,38 +87,46 @@ struct OutPtrs {
   T* f[NF];
 };
 
-// Tiles of the residual (64 x 4 threads, one cell each) and of the fused
-// loss-adjoint kernels (cells along axis 0 per precision x 64 along axis 1),
-// with the per-block partial counts every operator's *_nblocks reports.
-constexpr int kResTy = 64, kResTx = 4;
+// Tiles of the residual (64 x 4 threads, kResRows cells each) and of the fused
+// loss-adjoint kernels (cells along axis 0 per precision x 32 along axis 1),
+// and the partial-sum counts every operator's *_nblocks reports: one partial
+// per wavefront (4 per 256-thread workgroup).
+constexpr int kResRows = 1;  // cells per thread along axis 0
+constexpr int kResTy = 64, kResTx = 4 * kResRows;
 // (On the 1025^2 cavity 16 x 32 (FP32) and 8 x 32 (FP64) were the fastest
 // of 4..32 x 32..128.)
 constexpr int kGradTx32 = 16, kGradTx64 = 8, kGradTy = 32;
 
+constexpr int kWavesPerBlock = 4;
 inline int residual_nblocks(int n0, int n1) {
-  return ceil_div(n1, kResTy) * ceil_div(n0, kResTx);
+  return ceil_div(n1, kResTy) * ceil_div(n0, kResTx) * kWavesPerBlock;
 }
 inline int grad_nblocks(bool fp32, int n0, int n1) {
-  return ceil_div(n1, kGradTy) * ceil_div(n0, fp32 ? kGradTx32 : kGradTx64);
+  return ceil_div(n1, kGradTy) * ceil_div(n0, fp32 ? kGradTx32 : kGradTx64) *
+         kWavesPerBlock;
 }
 
-// Residual f_e at every cell, and per-block partial sums of f_e^2.
+// Residual f_e at every cell, and per-wavefront partial sums of f_e^2.
 template <class T, class Op>
 __global__ void __launch_bounds__(256)
     k_residual(FieldPtrs<T, Op::NF> in, OutPtrs<T, Op::NE> out,
                double* __restrict__ partial, int n0, int n1,
                typename Op::Params P) {
-  __shared__ double red[4 * Op::NE];
   unsigned bx, by;
   xcd_block2(bx, by);
-  const int j = bx * 64 + (threadIdx.x & 63);
-  const int i = by * 4 + (threadIdx.x >> 6);
+  const int j = bx * kResTy + (threadIdx.x & 63);
+  const int i0 = by * kResTx + (threadIdx.x >> 6);
   double ssq[Op::NE];
   for (int e = 0; e < Op::NE; ++e) ssq[e] = 0;
-  if (i < n0 && j < n1) {
+  const bool inner = interior_box(by * kResTx, by * kResTx + kResTx,
+                                  bx * kResTy, bx * kResTy + kResTy, n0, n1,
+                                  Op::R);
+#pragma unroll
+  for (int r = 0; r < kResRows; ++r) {
+    const int i = i0 + 4 * r;
+    if (i >= n0 || j >= n1) continue;
     T f[Op::NE];
-    if (interior_box(by * 4, by * 4 + 4, bx * 64, bx * 64 + 64, n0, n1,
-                     Op::R)) {
+    if (inner) {
       GlobalAcc<T, Op::NF, false> acc{in.f, n0, n1, i, j};
       Op::template residual<true>(acc, i, j, n0, n1, P, f);
     } else {
@@ -128,16 +136,13 @@ __global__ void __launch_bounds__(256)
     const size_t c = (size_t)i * n1 + j;
     for (int e = 0; e < Op::NE; ++e) {
       if (out.f[e]) out.f[e][c] = f[e];
-      ssq[e] = (double)f[e] * (double)f[e];
+      ssq[e] += (double)f[e] * (double)f[e];
     }
   }
-  if (partial) {
-    const int b = by * gridDim.x + bx;
-    const int nb = gridDim.x * gridDim.y;
-    block_sum_n<Op::NE>(ssq, red);
-    if (threadIdx.x == 0)
-      for (int e = 0; e < Op::NE; ++e) partial[(size_t)e * nb + b] = ssq[e];
-  }
+  if (partial)
+    wave_partials<Op::NE>(
+        ssq, partial, (size_t)gridDim.x * gridDim.y * kWavesPerBlock,
+        (size_t)(by * gridDim.x + bx) * kWavesPerBlock + (threadIdx.x >> 6));
 }
 
 // Jacobian coefficient fields: coef[q] for q < NJ holds entry
@@ -313,7 +318,7 @@ __device__ inline void loss_grad_tile(const FieldPtrs<T, Op::NF>& in,
 }
 
 // Fused loss + adjoint over the grid: one workgroup per TX x TY tile
-// (XCD-aware order), per-block partial sums of f_e^2 in `partial`.
+// (XCD-aware order), per-wavefront partial sums of f_e^2 in `partial`.
 template <class T, class Op, int TX, int TY>
 __global__ void __launch_bounds__(256)
     k_loss_grad(FieldPtrs<T, Op::NF> in, OutPtrs<T, Op::NF> grad,
@@ -322,20 +327,16 @@ __global__ void __launch_bounds__(256)
   constexpr int NE = Op::NE;
   extern __shared__ unsigned char smem_raw[];
   T* ot = reinterpret_cast<T*>(smem_raw);  // [NF][NS-1][E0][E1]
-  __shared__ double red[4 * Op::NE];
   unsigned bx, by;
   xcd_block2(bx, by);
   double ssq[NE];
   for (int e = 0; e < NE; ++e) ssq[e] = 0;
   loss_grad_tile<T, Op, TX, TY>(in, grad, n0, n1, by * TX, bx * TY, scale, P,
                                 ot, ssq);
-  if (partial) {
-    const int bidx = by * gridDim.x + bx;
-    const int nb = gridDim.x * gridDim.y;
-    block_sum_n<NE>(ssq, red);
-    if (threadIdx.x == 0)
-      for (int e = 0; e < NE; ++e) partial[(size_t)e * nb + bidx] = ssq[e];
-  }
+  if (partial)
+    wave_partials<NE>(
+        ssq, partial, (size_t)gridDim.x * gridDim.y * kWavesPerBlock,
+        (size_t)(by * gridDim.x + bx) * kWavesPerBlock + (threadIdx.x >> 6));
 }
 
 template <class T, class Op, int TX, int TY>
@@ -511,7 +512,7 @@ int launch_residual(const void* const* fields, void* const* res,
   for (int e = 0; e < Op::NE; ++e)
     out.f[e] = res ? static_cast<T*>(res[e]) : nullptr;
   dim3 grid(ceil_div(n1, kResTy), ceil_div(n0, kResTx));
-  if (nblocks) *nblocks = grid.x * grid.y;
+  if (nblocks) *nblocks = grid.x * grid.y * kWavesPerBlock;
   hipLaunchKernelGGL((k_residual<T, Op>), grid, dim3(256), 0, stream, in, out,
                      partial, n0, n1, P);
   return check_launch("residual");
@@ -544,7 +545,7 @@ int launch_loss_grad_t(const void* const* fields, void* const* grad,
     out.f[k] = static_cast<T*>(grad[k]);
   }
   dim3 grid(ceil_div(n1, TY), ceil_div(n0, TX));
-  if (nblocks) *nblocks = grid.x * grid.y;
+  if (nblocks) *nblocks = grid.x * grid.y * kWavesPerBlock;
   constexpr size_t smem = loss_grad_smem<T, Op, TX, TY>();
   static_assert(smem <= 160 * 1024, "LDS tile too large");
   auto kern = k_loss_grad<T, Op, TX, TY>;
