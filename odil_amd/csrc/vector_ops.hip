@@ -220,51 +220,77 @@ __global__ void __launch_bounds__(256)
 // One diagonal block of a right-looking blocked Cholesky factorisation of a
 // dense SPD matrix (row-major, leading dimension lda): the block
 // A[k0:k0+nb, k0:k0+nb] (already updated by the previous panels) is
-// factored in LDS, A_kk = L L^T, L written back (the block's strict upper
-// part zeroed), and W = L^{-1} (lower) written to w (nb x nb, row-major,
-// leading dimension kCholNb). The panel and trailing updates around it are
-// GEMMs. Both steps are right-looking over the block's columns: column j is
-// scaled by 1/sqrt(pivot), then the trailing part takes the rank-1 update;
-// W starts as I and takes the same eliminations. nb <= kCholNb.
+// factored, A_kk = L L^T, L written back (the block's strict upper part
+// zeroed), and W = L^{-1} (lower) written to w (nb x nb, row-major, leading
+// dimension kCholNb). The panel and trailing updates around it are GEMMs.
+// nb <= kCholNb.
 constexpr int kCholNb = 64;
 
-__global__ void __launch_bounds__(256)
-    k_chol_block(double* __restrict__ A, int lda, int k0, int nb,
-                 double* __restrict__ w) {
-  __shared__ double L[kCholNb][kCholNb + 1];
-  __shared__ double W[kCholNb][kCholNb + 1];
-  // 16 x 16 threads; element loops stride by 16 in both directions (no
-  // index divisions on the per-column critical path).
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  for (int i = ty; i < nb; i += 16)
-    for (int j = tx; j < nb; j += 16) {
-      L[i][j] = j <= i ? A[(size_t)(k0 + i) * lda + k0 + j] : 0.0;
-      W[i][j] = i == j ? 1.0 : 0.0;
-    }
-  __syncthreads();
-  for (int j = 0; j < nb; ++j) {
-    const double d = sqrt(L[j][j]);
-    const double inv = 1.0 / d;
-    // Column j of L below the pivot; row j of W (eliminations so far only
-    // touched columns <= j of W's rows).
-    for (int i = j + 1 + threadIdx.x; i < nb; i += blockDim.x) L[i][j] *= inv;
-    for (int c = threadIdx.x; c <= j; c += blockDim.x) W[j][c] *= inv;
+// Double-precision value of another lane (lane index uniform).
+__device__ inline double readlane_f64(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The factorisation and inversion run in one wavefront, in two phases.
+// Factorisation: lane i keeps row i of L in registers (the column loop is
+// unrolled so every register index is a constant); per column j the pivot
+// comes from lane j by readlane, column j reaches every lane through LDS
+// (broadcast reads) and each lane updates its own row. Inversion: L goes to
+// LDS and lane c solves L w = e_c by forward substitution for column c of
+// W = L^{-1}. No workgroup-wide barrier on either 64-step critical path.
+__global__ void __launch_bounds__(64)
+    k_chol_block_wave(double* __restrict__ A, int lda, int k0, int nb,
+                      double* __restrict__ w) {
+  __shared__ double col[kCholNb];
+  __shared__ double Ls[kCholNb][kCholNb + 1];
+  const int i = threadIdx.x;
+  double Lr[kCholNb];
+  const double* __restrict__ arow =
+      A + (size_t)(k0 + (i < nb ? i : 0)) * lda + k0;
+#pragma unroll
+  for (int c = 0; c < kCholNb; ++c)
+    Lr[c] = (i < nb && c <= i && c < nb) ? arow[c] : 0.0;
+  // Columns j >= nb (a short last block) change nothing: lij = 0.
+#pragma unroll
+  for (int j = 0; j < kCholNb; ++j) {
+    const bool live = j < nb;
+    const double d = live ? sqrt(readlane_f64(Lr[j], j)) : 1.0;
+    const double lij = (live && i > j) ? Lr[j] / d : 0.0;
+    Lr[j] = (live && i > j) ? lij : ((live && i == j) ? d : Lr[j]);
+    col[i] = lij;
     __syncthreads();
-    if (threadIdx.x == 0) L[j][j] = d;
-    // Rank-1 updates: L[i][l] -= L[i][j] L[l][j] (j < l <= i) and
-    // W[i][c] -= L[i][j] W[j][c] (i > j, c <= j).
-    for (int i = j + 1 + ty; i < nb; i += 16) {
-      const double lij = L[i][j];
-      for (int l = j + 1 + tx; l <= i; l += 16) L[i][l] -= lij * L[l][j];
-      for (int c = tx; c <= j; c += 16) W[i][c] -= lij * W[j][c];
-    }
+    // L[i][l] -= L[i][j] L[l][j] for l > j (entries right of the diagonal
+    // are never read and are stored as zero).
+#pragma unroll
+    for (int l = j + 1; l < kCholNb; ++l) Lr[l] = fma(-lij, col[l], Lr[l]);
     __syncthreads();
   }
-  for (int i = ty; i < nb; i += 16)
-    for (int j = tx; j < nb; j += 16) {
-      A[(size_t)(k0 + i) * lda + k0 + j] = j <= i ? L[i][j] : 0.0;
-      w[i * kCholNb + j] = W[i][j];
-    }
+#pragma unroll
+  for (int c = 0; c < kCholNb; ++c) Ls[i][c] = (c <= i && i < nb) ? Lr[c] : 0.0;
+  __syncthreads();
+  // Column c of W: w_r = (delta_rc - sum_{k<r} L[r][k] w_k) / L[r][r]; rows
+  // and columns past nb stay zero.
+  const int cc = i;
+  double wc[kCholNb];
+#pragma unroll
+  for (int r = 0; r < kCholNb; ++r) {
+    double sacc = r == cc ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < r; ++k) sacc = fma(-Ls[r][k], wc[k], sacc);
+    wc[r] = r < nb ? sacc / Ls[r][r] : 0.0;
+  }
+  if (i < nb) {
+    double* __restrict__ orow = A + (size_t)(k0 + i) * lda + k0;
+#pragma unroll
+    for (int c = 0; c < kCholNb; ++c)
+      if (c < nb) orow[c] = c <= i ? Lr[c] : 0.0;
+#pragma unroll
+    for (int r = 0; r < kCholNb; ++r)
+      if (r < nb) w[r * kCholNb + cc] = wc[r];
+  }
 }
 
 }  // namespace odil
@@ -501,7 +527,7 @@ int odil_chol_block(double* A, int lda, int k0, int nb, double* W,
     set_error("chol_block: bad arguments");
     return kErrArg;
   }
-  hipLaunchKernelGGL(k_chol_block, dim3(1), dim3(256), 0,
+  hipLaunchKernelGGL(k_chol_block_wave, dim3(1), dim3(64), 0,
                      static_cast<hipStream_t>(stream), A, lda, k0, nb, W);
   return check_launch("chol_block");
 }
