@@ -51,7 +51,7 @@ def parse():
     p.add_argument('--refine', type=int, default=0,
                    help='FP64 iterative-refinement passes per linear solve')
     p.add_argument('--tol', type=float, default=1e-8)
-    p.add_argument('--tol_maxiter', type=int, default=20,
+    p.add_argument('--tol_maxiter', type=int, default=18,
                    help='linear iterations per Newton step of the '
                         'time-to-tolerance runs (both precisions)')
     p.add_argument('--max_epochs', type=int, default=150)
