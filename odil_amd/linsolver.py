@@ -88,6 +88,16 @@ def _stack(fields):
     return torch.stack([f.to(torch.float64) for f in fields]).contiguous()
 
 
+def _vector(b, nvar, nw, what):
+    """b as a contiguous FP64 (nvar, *nw) device tensor (ValueError if the
+    shape differs)."""
+    shape = (nvar, ) + tuple(nw)
+    if tuple(b.shape) != shape:
+        raise ValueError('{}: b has shape {}, expected {}'.format(
+            what, tuple(b.shape), shape))
+    return b.to(torch.float64).contiguous()
+
+
 def _regularised_normal(matr, beta, betadiag):
     """A = J^T J (stored upper half, FP64) with the diagonal regularisation
     of linsolver.py:37-40: (diag + beta^2) (1 + betadiag^2)."""
@@ -106,6 +116,7 @@ def lsqr(A, b, damp=0., atol=1e-6, btol=1e-6, conlim=1e8, iter_lim=None):
     stacked as (nvar, *grid); the stopping rules and estimates are those of
     the paper (istop 1-2: residual / least-squares tolerance met, 3: cond
     limit, 4-6: the same at machine precision, 7: iteration limit)."""
+    b = _vector(b, A.nvar_out, A.nwh, 'lsqr')
     vec = _Vec(b.device)
     nvar = A.nvar_in
     n = nvar * int(np.prod(A.nw))
@@ -219,6 +230,7 @@ def bicgstab(A, b, atol=1e-6, maxiter=50, callback=None):
     when |r| < atol (relative tolerance 0, as the reference passes).
     `callback(x)` after every iteration. Returns (x, info): info 0 when
     converged, else the iterations run."""
+    b = _vector(b, A.nvar_out, A.nw, 'bicgstab')
     vec = _Vec(b.device)
     x = torch.zeros_like(b)
     r = b.clone()
@@ -280,6 +292,7 @@ def pcg_multigrid(A, b, tol=1e-6, maxiter=50, nlvl=None, omega=0.6,
     """CG on the stored symmetric A preconditioned by one V(pre, post) cycle
     from zero per iteration; stops when |r| <= tol |b|. Returns
     (x, residual norms including the initial one)."""
+    b = _vector(b, A.nvar_out, A.nw, 'pcg_multigrid')
     vec = _Vec(b.device)
     if any(k % 2 == 0 for k in A.nw):
         raise ValueError("linsolver='multigrid' needs odd node grids "

@@ -25,3 +25,17 @@ def test_multigrid_levels():
     assert linsolver._mg_levels((33, 33), 3) == 3
     assert linsolver._mg_levels((17, 16), None) == 1   # even: one level
     assert linsolver._mg_levels((3, 3), None) == 1
+
+
+def test_iterative_solvers_check_the_vector_shape():
+    """A right-hand side of the wrong shape is refused before any device
+    work (CPU: the check precedes every kernel call)."""
+    import pytest
+    import torch
+    from odil_amd.multigrid import BlockStencil
+    coef = torch.ones((1, 5, 5), dtype=torch.float64)
+    A = BlockStencil(coef, [(0, 0, (0, 0))], (5, 5))
+    bad = torch.zeros((2, 5, 5), dtype=torch.float64)
+    for fn in (linsolver.lsqr, linsolver.bicgstab, linsolver.pcg_multigrid):
+        with pytest.raises(ValueError):
+            fn(A, bad)
