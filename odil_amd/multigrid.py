@@ -364,6 +364,40 @@ class BlockStencil:
         return BlockStencil(jac.coef, jac.entries, jac.shape,
                             nvar_in=jac.nvar, nvar_out=jac.neq)
 
+    @staticmethod
+    def from_csr(m, nw, nvar_out, nvar_in=None, device=None,
+                 dtype=torch.float64):
+        """Stride-1 block stencil of a sparse matrix whose rows are
+        nvar_out fields and columns nvar_in fields on node grid `nw`, each
+        field-major and C-order (the layout of `Problem.linearize`,
+        util_op.py:877-894, and of the reference's `Multigrid` vectors).
+        Entry (o, i, s) holds row (o, c) -> column (i, c + s); the grid
+        offset s is taken from the actual column, so a coupling that wraps
+        around a periodic grid becomes a long shift (exact as a matrix)."""
+        nvar_in = nvar_out if nvar_in is None else nvar_in
+        nw = tuple(int(n) for n in nw)
+        N = int(np.prod(nw))
+        if m.shape != (nvar_out * N, nvar_in * N):
+            raise ValueError('matrix shape {} does not match {} x {} fields '
+                             'on {}'.format(m.shape, nvar_out, nvar_in, nw))
+        m = m.tocoo()
+        m.sum_duplicates()
+        o, c = np.divmod(m.row.astype(np.int64), N)
+        i, cc = np.divmod(m.col.astype(np.int64), N)
+        ri = np.unravel_index(c, nw)
+        ci = np.unravel_index(cc, nw)
+        key = np.stack([o, i] + [ci[d] - ri[d] for d in range(len(nw))],
+                       axis=1)
+        keys, inv = np.unique(key, axis=0, return_inverse=True)
+        coef = np.zeros((len(keys), N))
+        coef[inv.ravel(), c] = m.data
+        entries = [(int(k[0]), int(k[1]), tuple(int(x) for x in k[2:]))
+                   for k in keys]
+        t = torch.tensor(coef.reshape((len(keys), ) + nw), dtype=dtype,
+                         device=device or torch.device('cuda'))
+        return BlockStencil(t, entries, nw, nvar_in=nvar_in,
+                            nvar_out=nvar_out)
+
 
 _PLANS = {}
 
@@ -858,4 +892,96 @@ class MultigridOp:
         if cur is not u:
             for a, b in zip(u, cur):
                 a.copy_(b)
+        return u
+
+
+class Multigrid:
+    """The reference's sparse-matrix `Multigrid` (multigrid.py:72-339) on
+    the GPU: same constructor (`nw`, `nvar`, `nlevels`, `restriction`; `mod`
+    and `dtype` are accepted and unused), `update_A` with the finest matrix
+    as a SciPy sparse matrix (or a `BlockStencil`), `smoother_jacobi`,
+    `smoother_direct` and `step(u, f, smoother, level, ndirect, pre,
+    post)` on flat FP64 device vectors (fields concatenated, C-order).
+
+    The matrix is converted once to block-stencil form
+    (`BlockStencil.from_csr`), the coarse matrices are R A T
+    (`galerkin_mode='exact'`, what the reference's sparse products compute),
+    smoothing, transfers and the coarsest solve (a dense inverse applied by
+    a GEMV kernel) run in the HIP kernels of `MultigridOp`. Like the
+    reference, `step` passes `ndirect` to the coarser levels but not `pre`
+    and `post` (multigrid.py:331-332). Node grids of odd size only; the
+    block-diagonal (`full=True`) and Gauss-Seidel smoothers are not
+    provided (the Newton driver uses `full=False`, util.py:471-473).
+    """
+
+    def __init__(self, nw, nvar=1, nlevels=None, restriction='full',
+                 mod=None, dtype=None, device=None):
+        self._mg = MultigridOp(nw, nvar=nvar, nlevels=nlevels,
+                               restriction=restriction, dtype=torch.float64,
+                               device=device, coarse_reg=0.,
+                               galerkin_mode='exact')
+        self.nnw = self._mg.nnw
+        self.nlevels = self._mg.nlevels
+        self.nvar = nvar
+        self.dim = len(self.nnw[0])
+        self.restriction = restriction
+        self.AA = None
+
+    def update_A(self, A0):
+        if isinstance(A0, list):
+            raise ValueError('Multigrid.update_A: a list of per-level '
+                             'matrices is not supported; pass the finest')
+        if not isinstance(A0, BlockStencil):
+            A0 = BlockStencil.from_csr(A0, self.nnw[0], self.nvar,
+                                       device=self._mg.device)
+        if any(abs(x) > 2 for _, _, sh in A0.entries for x in sh):
+            raise ValueError('Multigrid.update_A: couplings beyond 2 nodes '
+                             '(e.g. a periodic wrap) are not supported by '
+                             'the stencil Galerkin product')
+        self._mg.update_A(A0)
+        self.AA = self._mg.AA
+        return self
+
+    def _fields(self, v, level):
+        n = (self.nvar, ) + tuple(self.nnw[level])
+        if v.numel() != int(np.prod(n)) or not v.is_contiguous():
+            raise ValueError('level {} vectors hold {} values (contiguous), '
+                             'got {}'.format(level, int(np.prod(n)),
+                                             v.numel()))
+        return list(v.view(n))
+
+    def smoother_jacobi(self, level, u, f, omega=1., full=False):
+        """u += omega (f - A u) / diag(A) in place (multigrid.py:284-299)."""
+        if full:
+            raise ValueError('Multigrid.smoother_jacobi: full=True '
+                             '(block-diagonal) is not provided')
+        self._mg.smoother_jacobi(level, self._fields(u, level),
+                                 self._fields(f, level), omega)
+        return u
+
+    def smoother_direct(self, level, u, f):
+        out = self._mg.smoother_direct(level, self._fields(u, level),
+                                       self._fields(f, level))
+        return torch.stack(out).reshape(-1)
+
+    def step(self, u, f, smoother, level=0, ndirect=5, pre=2, post=2):
+        """One V-cycle from u (multigrid.py:312-339); returns the new u."""
+        if level + 1 >= self.nlevels or min(self.nnw[level]) <= ndirect:
+            return self.smoother_direct(level, u, f)
+        for _ in range(pre):
+            u = smoother(level, u, f)
+        A = self._mg.AA[level]
+        r = A.residual(self._fields(u, level), self._fields(f, level),
+                       [torch.empty(tuple(self.nnw[level]), dtype=u.dtype,
+                                    device=u.device)
+                        for _ in range(self.nvar)])
+        rc = torch.stack(restrict(r, self.nnw[level], self._mg.rw,
+                                  restriction=self.restriction)).reshape(-1)
+        du = self.step(torch.zeros_like(rc), rc, smoother, level=level + 1,
+                       ndirect=ndirect)
+        u = u.clone()
+        prolong_add(self._fields(du, level + 1), self._fields(u, level),
+                    self.nnw[level])
+        for _ in range(post):
+            u = smoother(level, u, f)
         return u

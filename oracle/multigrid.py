@@ -312,3 +312,57 @@ def coeffs_from_tape(grads, field_desc, neqn, nvar):
                 continue
             stf[i][var][tuple(int(x) for x in dw)] = g
     return stf
+
+
+class Multigrid:
+    """The sparse-matrix `Multigrid` (`multigrid.py:72-339`): block-diagonal
+    R and T over nvar fields, coarse matrices R A T by sparse products
+    (`update_A`, `:116-157`), point-Jacobi smoother on the diagonal blocks
+    (`:284-299`, full=False), sparse direct coarsest solve (`:307-310`) and
+    the V-cycle whose recursion forwards `ndirect` but not `pre`/`post`
+    (`:312-339`). Vectors are flat, fields concatenated."""
+
+    def __init__(self, nw, nvar=1, nlevels=None, restriction='full'):
+        nw = np.array(nw)
+        max_nlevels = 0
+        while 2**(max_nlevels + 1) < min(nw):
+            max_nlevels += 1
+        nlevels = max_nlevels if nlevels is None else min(
+            nlevels, max_nlevels)
+        nnw = [nw]
+        for i in range(1, nlevels):
+            nnw.append((nnw[i - 1] - 1) // 2 + 1)
+        self.nnw = [tuple(int(x) for x in n) for n in nnw]
+        self.nlevels = nlevels
+        self.nvar = nvar
+        bd = lambda m: scipy.sparse.block_diag([m] * nvar).tocsr()
+        self.RR = [bd(get_R(n, restriction).tocsr()) for n in self.nnw]
+        self.TT = [bd(get_TT(n).tocsr().T) for n in self.nnw]
+
+    def update_A(self, A0):
+        AA = [A0.tocsr()]
+        for level in range(1, self.nlevels):
+            AA.append((self.RR[level - 1] @ AA[-1] @
+                       self.TT[level - 1]).tocsr())
+        self.AA = AA
+        self.DD = [A.diagonal() for A in AA]
+
+    def smoother_jacobi(self, level, u, f, omega=1.):
+        r = f - self.AA[level] @ u
+        u += r / self.DD[level] * omega
+        return u
+
+    def smoother_direct(self, level, u, f):
+        return scipy.sparse.linalg.spsolve(self.AA[level].tocsc(), f)
+
+    def step(self, u, f, smoother, level=0, ndirect=5, pre=2, post=2):
+        if level + 1 >= self.nlevels or min(self.nnw[level]) <= ndirect:
+            return self.smoother_direct(level, u, f)
+        for _ in range(pre):
+            u = smoother(level, u, f)
+        r = self.RR[level] @ (f - self.AA[level] @ u)
+        u = u + self.TT[level] @ self.step(np.zeros_like(r), r, smoother,
+                                           level=level + 1, ndirect=ndirect)
+        for _ in range(post):
+            u = smoother(level, u, f)
+        return u

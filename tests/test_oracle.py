@@ -209,6 +209,59 @@ def test_vcycle_matches_reference_golden():
             assert np.abs(d).max() <= 1e-10 * np.abs(ref[k]).max()
 
 
+def _cavity17_normal(beta2):
+    """Stencil blocks of J^T J + beta2 I on the 17x17 cavity golden state."""
+    g = np.load(os.path.join(GOLD, 'mg_vcycle.npz'))
+    n = 17
+    st = {k: g['cavity17_state_' + k] for k in 'uvp'}
+    const, grads, desc = tape.eval_grad(ops.make_cavity_operator(0.01),
+                                        ops.cavity_domain(n), st)
+    stf = omg.coeffs_from_tape(grads, desc, 3, 3)
+    mm, matr = omg.normal_operator(stf, (n, n), 3)
+    for i in range(3):
+        c = dict(matr[i][i].coeffs)
+        c[(0, 0)] = c[(0, 0)] + beta2
+        matr[i][i] = omg.StencilOp(c, (n, n))
+    rhs = [-sum(mm[i][j].mul_transpose_field(const[i].ravel())
+                for i in range(3)) for j in range(3)]
+    return matr, rhs
+
+
+def test_sparse_multigrid_equals_stencil_multigrid():
+    """The sparse-matrix Multigrid (multigrid.py:72-339) and the stencil
+    MultigridOp with exact Galerkin products (multigrid.py:342-602) are
+    independent restatements of the same V-cycle when the top level is
+    called with the recursion defaults (ndirect 5, pre = post = 2): their
+    iterates agree to round-off (the two forms sum the Galerkin products in
+    different orders) on the 17x17 Poisson case and on the regularised
+    cavity normal equations."""
+    n = 17
+    dx = 1 / (n - 1)
+    pad = lambda: np.pad(np.ones((n - 2, n - 2)), 1) / dx**2
+    A = omg.StencilOp({(0, 0): 4 * np.ones((n, n)) / dx**2, (1, 0): -pad(),
+                       (-1, 0): -pad(), (0, 1): -pad(), (0, -1): -pad()},
+                      (n, n))
+    cases = [([[A]], [np.full(n * n, 10.)], 1, None, 0.7)]
+    matr, rhs = _cavity17_normal(1e-2)
+    cases.append((matr, rhs, 3, 4, 0.6))
+    for blocks, f, nvar, nlv, omega in cases:
+        M = omg.MultigridOp((n, n), nvar=nvar, nlevels=nlv,
+                            galerkin_mode='exact')
+        M.update_A(blocks)
+        S = omg.Multigrid((n, n), nvar=nvar, nlevels=nlv)
+        S.update_A(scipy.sparse.bmat([[blocks[i][j].tocsr()
+                                       for j in range(nvar)]
+                                      for i in range(nvar)]))
+        smooth = lambda level, u, ff: S.smoother_jacobi(level, u, ff, omega)
+        u = [np.zeros(n * n) for _ in range(nvar)]
+        v = np.zeros(nvar * n * n)
+        for _ in range(3):
+            u = M.step(u, [x.copy() for x in f], omega)
+            v = S.step(v, np.concatenate(f), smooth)
+            ref = np.concatenate(u)
+            assert np.abs(v - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
 def test_poisson_multigrid_converges():
     """Acceptance of the reference's test_multigrid_poisson
     (test_multigrid.py:251-368): 10 V-cycles, |rhs - A u| <= 1e-6."""
