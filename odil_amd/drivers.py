@@ -1,6 +1,8 @@
 """Optimisation drivers with the reference's structure (util.py:240-317,
-509-604): Newton with a direct normal-equation solve, Newton with the
-geometric-multigrid solver, and gradient optimisers over the packed state.
+382-604): Newton with a direct normal-equation solve, Newton with the
+geometric-multigrid solvers (plain V-cycles on the stored normal matrix,
+and flexible CG with a V-cycle preconditioner), and gradient optimisers
+over the packed state.
 
 Every evaluation runs on the device: residual and Jacobian kernels for
 Newton, the fused loss adjoint for the gradient optimisers.
@@ -54,7 +56,55 @@ def optimize_newton(problem, state, epochs, args=None, callback=None,
 
 
 def optimize_multigrid(problem, state, epochs, args=None, callback=None,
-                       tol=None, precision='fp64', maxiter=None):
+                       tol=None):
+    """util.optimize_multigrid -> optimize_multigrid_base (util.py:382-471)
+    on the device: per epoch the stencil Jacobian J and residual f,
+    rhs = -J^T f, the stored normal matrix J^T J (`normal_operator`, stencil
+    form instead of the reference's host CSR product), `Multigrid.update_A`
+    (Galerkin R A T levels) and `linsolver_maxiter` plain V-cycles from zero
+    with the point-Jacobi smoother (omega, smooth_pre, smooth_post, ndirect,
+    restriction and nlvl from `args`), then `packed += sol`.
+    `callback(epoch, state, rms)` runs before each update; stops early when
+    max(rms) <= tol. Returns the history [(epoch, rms, seconds)]."""
+    from odil_amd.multigrid import Multigrid, normal_operator
+    problem.init_missing(state)
+    names = problem.domain.fieldnames
+    a = args or ls.default_args()
+    mg = None
+    hist = []
+    t0 = time.perf_counter()
+    for epoch in range(epochs + 1):
+        J, res = problem.jacobian(state, epoch)
+        rms = _rms(problem, state)
+        hist.append((epoch, rms, time.perf_counter() - t0))
+        if callback is not None:
+            callback(epoch, state, rms)
+        if epoch == epochs or (tol is not None and max(rms) <= tol):
+            break
+        rhs = torch.stack(J.apply_t([-r for r in res])).reshape(-1)
+        N = normal_operator(J, torch.float64)
+        if mg is None:
+            mg = Multigrid(problem.domain.shape, nvar=len(names),
+                           nlevels=getattr(a, 'nlvl', None),
+                           restriction=getattr(a, 'restriction', None) or
+                           'full', device=rhs.device)
+        mg.update_A(N)
+        omega = a.omega
+
+        def smoother(level, u, f):
+            return mg.smoother_jacobi(level, u, f, omega=omega)
+        sol = torch.zeros_like(rhs)
+        for _ in range(a.linsolver_maxiter or 50):
+            sol = mg.step(sol, rhs, smoother, ndirect=a.ndirect,
+                          pre=a.smooth_pre, post=a.smooth_post)
+        sol = sol.view((len(names), ) + tuple(problem.domain.shape))
+        for k, name in enumerate(names):
+            state.fields[name] += sol[k]
+    return hist
+
+
+def optimize_multigridop(problem, state, epochs, args=None, callback=None,
+                         tol=None, precision='fp64', maxiter=None):
     """util.optimize_multigridop_base (util.py:509-604) on the device:
     Newton steps whose normal equations are solved by flexible CG with a
     GMG V-cycle preconditioner (`NewtonMultigrid`)."""

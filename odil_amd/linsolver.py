@@ -433,6 +433,8 @@ def add_arguments(parser):
     parser.add_argument('--smooth_pre', type=int, default=2)
     parser.add_argument('--smooth_post', type=int, default=2)
     parser.add_argument('--ndirect', type=int, default=3)
+    parser.add_argument('--restriction', default='full',
+                        choices=('full', 'half', 'injection', 'transpose'))
     return parser
 
 
@@ -440,7 +442,7 @@ def default_args(**kw):
     a = Namespace(linsolver='direct', linsolver_maxiter=None,
                   linsolver_tol=1e-6, beta=0., betadiag=0.,
                   linsolver_verbose=0, nlvl=5, omega=0.6, smooth_pre=2,
-                  smooth_post=2, ndirect=3)
+                  smooth_post=2, ndirect=3, restriction='full')
     for k, v in kw.items():
         setattr(a, k, v)
     return a

@@ -46,6 +46,36 @@ def optimize_newton(operator, domain, fields, epochs, callback=None,
     return packed
 
 
+def optimize_multigrid(operator, domain, fields, epochs, nlevels=None,
+                       maxiter=50, omega=0.6, ndirect=3, pre=2, post=2,
+                       restriction='full', callback=None):
+    """`util.optimize_multigrid_base` (`util.py:382-471`): linearize,
+    rhs = -m^T c, matr = m^T m, `Multigrid.update_A(matr)`, `maxiter`
+    V-cycles from zero with the point-Jacobi smoother, `packed += sol`.
+    Returns the final packed state."""
+    from oracle.multigrid import Multigrid
+    packed = tape.pack_fields(domain, fields)
+    mg = Multigrid(domain.shape, nvar=len(domain.fieldnames),
+                   nlevels=nlevels, restriction=restriction)
+    for epoch in range(epochs + 1):
+        state = tape.unpack_fields(domain, packed)
+        const, m = tape.linearize(operator, domain, state, epoch=epoch)
+        if callback is not None:
+            callback(epoch, packed, const)
+        if epoch == epochs:
+            break
+        c = np.hstack([eq.flatten() for eq in const])
+        rhs = -m.T @ c
+        mg.update_A((m.T @ m).tocsr())
+        sol = np.zeros_like(rhs)
+        smooth = lambda level, u, f: mg.smoother_jacobi(level, u, f, omega)
+        for _ in range(maxiter):
+            sol = mg.step(sol, rhs, smooth, ndirect=ndirect, pre=pre,
+                          post=post)
+        packed = packed + sol
+    return packed
+
+
 def rms_residuals(const):
     """Per-equation RMS residual, `opt.last_residual` (`util.py:253-254`)."""
     return [float(np.mean(c**2)**0.5) for c in const]

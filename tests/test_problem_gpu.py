@@ -233,3 +233,34 @@ def test_advection_fp32_lbfgs_and_gradient_fd(dev):
     st, info = optimize_opt(pb, st, 'lbfgs', 200, m=10)
     l1 = pb.eval_loss_grad(st)[0].item()
     assert l1 <= 1e-3 * l0, (l0, l1)
+
+
+@pytest.mark.parametrize('restriction', ['full', 'transpose'])
+def test_optimize_multigrid_matches_oracle(dev, restriction):
+    """The sparse-matrix multigrid Newton driver (util.py:382-471: stored
+    J^T J, Galerkin levels, plain V-cycles from zero) gives the oracle's
+    iterates on a 17x17 Poisson problem, and its residual falls."""
+    from odil_amd.drivers import optimize_multigrid
+    from odil_amd.linsolver import default_args
+    from odil_amd.operators import Poisson
+    from odil_amd.problem import Domain, Problem, State
+    from oracle import newton
+    n = 17
+    src = ops.gaussian_mixture_source(n, seed=4)
+    dom = Domain(shape=(n, n), varnames=('x', 'y'), fieldnames=['u'],
+                 device=dev)
+    pb = Problem(Poisson((n, n), torch.tensor(src, device=dev)), dom)
+    st = State()
+    args = default_args(linsolver_maxiter=6, nlvl=3, omega=0.6,
+                        restriction=restriction)
+    hist = optimize_multigrid(pb, st, 3, args)
+    odom = tape.Domain(ndim=2, shape=(n, n), varnames=('x', 'y'),
+                       fieldnames=['u'])
+    packed = newton.optimize_multigrid(ops.make_poisson_operator(src), odom,
+                                       {'u': np.zeros((n, n))}, 3,
+                                       nlevels=3, maxiter=6, omega=0.6,
+                                       restriction=restriction)
+    ref = tape.unpack_fields(odom, packed)['u']
+    u = st.fields['u'].cpu().numpy()
+    assert np.abs(u - ref).max() <= 1e-10 * np.abs(ref).max()
+    assert hist[-1][1][0] < 0.02 * hist[0][1][0]
