@@ -76,9 +76,11 @@ int odil_reduce_partials(const double* partial, int nrows, int nb,
  *                    (util.py:544-566).
  *  odil_X_jac_entries host call: writes the entry table ent[3 q] =
  *                    (equation, field, shift index) and the shift table
- *                    shifts[2 s] (2D); returns the number of entries.
+ *                    shifts[2 s] (2D); returns the number of entries
+ *                    (either array may be NULL to query the count).
  *  odil_X_nblocks    host call: partial-sum count of kind 0 (residual) or
- *                    1 (loss_grad / jtf) for the grid.
+ *                    1 (loss_grad / jtf) for the grid: one partial per
+ *                    wavefront of the launch.
  *  odil_X_normal_apply  matrix-free N = J^T J with J recomputed from
  *                    `state`: mode 0 y = N x, 1 y = b - N x,
  *                    2 y = x + omega (b - N x) / d (weighted Jacobi),

@@ -519,15 +519,17 @@ int odil_cavity_jac(int dtype, int nx, int ny, double dx, double dy,
 // Table of the Jacobian entries: for q < 27, ent[3q..3q+2] =
 // (equation, field, shift index); shifts[2s..2s+1] = (dx, dy).
 int odil_cavity_jac_entries(int* ent, int* shifts) {
-  for (int q = 0; q < CavityOp<double>::NJ; ++q) {
-    ent[3 * q] = CavityOp<double>::JE[q];
-    ent[3 * q + 1] = CavityOp<double>::JV[q];
-    ent[3 * q + 2] = CavityOp<double>::JS[q];
-  }
-  for (int s = 0; s < CavityOp<double>::NS; ++s) {
-    shifts[2 * s] = CavityOp<double>::SX[s];
-    shifts[2 * s + 1] = CavityOp<double>::SY[s];
-  }
+  if (ent)
+    for (int q = 0; q < CavityOp<double>::NJ; ++q) {
+      ent[3 * q] = CavityOp<double>::JE[q];
+      ent[3 * q + 1] = CavityOp<double>::JV[q];
+      ent[3 * q + 2] = CavityOp<double>::JS[q];
+    }
+  if (shifts)
+    for (int s = 0; s < CavityOp<double>::NS; ++s) {
+      shifts[2 * s] = CavityOp<double>::SX[s];
+      shifts[2 * s + 1] = CavityOp<double>::SY[s];
+    }
   return CavityOp<double>::NJ;
 }
 
@@ -642,15 +644,17 @@ int odil_cavity_extras_nblocks(int kind, int dtype, int nx, int ny) {
 
 int odil_cavity_extras_jac_entries(int* ent, int* shifts) {
   using E = CavityExtrasOp<double>;
-  for (int q = 0; q < E::NJ; ++q) {
-    ent[3 * q] = E::JE[q];
-    ent[3 * q + 1] = E::JV[q];
-    ent[3 * q + 2] = E::JS[q];
-  }
-  for (int s = 0; s < E::NS; ++s) {
-    shifts[2 * s] = E::SX[s];
-    shifts[2 * s + 1] = E::SY[s];
-  }
+  if (ent)
+    for (int q = 0; q < E::NJ; ++q) {
+      ent[3 * q] = E::JE[q];
+      ent[3 * q + 1] = E::JV[q];
+      ent[3 * q + 2] = E::JS[q];
+    }
+  if (shifts)
+    for (int s = 0; s < E::NS; ++s) {
+      shifts[2 * s] = E::SX[s];
+      shifts[2 * s + 1] = E::SY[s];
+    }
   return E::NJ;
 }
 

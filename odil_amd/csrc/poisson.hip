@@ -57,15 +57,17 @@ struct LaplaceOp {
 
 template <class Op>
 int entries_table(int* ent, int* shifts) {
-  for (int q = 0; q < Op::NJ; ++q) {
-    ent[3 * q] = Op::JE[q];
-    ent[3 * q + 1] = Op::JV[q];
-    ent[3 * q + 2] = Op::JS[q];
-  }
-  for (int s = 0; s < Op::NS; ++s) {
-    shifts[2 * s] = Op::SX[s];
-    shifts[2 * s + 1] = Op::SY[s];
-  }
+  if (ent)
+    for (int q = 0; q < Op::NJ; ++q) {
+      ent[3 * q] = Op::JE[q];
+      ent[3 * q + 1] = Op::JV[q];
+      ent[3 * q + 2] = Op::JS[q];
+    }
+  if (shifts)
+    for (int s = 0; s < Op::NS; ++s) {
+      shifts[2 * s] = Op::SX[s];
+      shifts[2 * s + 1] = Op::SY[s];
+    }
   return Op::NJ;
 }
 

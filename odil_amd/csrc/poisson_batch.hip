@@ -96,7 +96,7 @@ int odil_poisson_batch_shapes(int dtype, int* shapes, int cap) {
   static const int f64[][2] = {{8, 32}, {8, 64}, {16, 32}, {16, 64}};
   const int(*tab)[2] = dtype == kF32 ? f32 : f64;
   const int n = dtype == kF32 ? 5 : 4;
-  for (int k = 0; k < n && k < cap; ++k) {
+  for (int k = 0; shapes && k < n && k < cap; ++k) {
     shapes[2 * k] = tab[k][0];
     shapes[2 * k + 1] = tab[k][1];
   }

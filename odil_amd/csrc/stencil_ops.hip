@@ -994,6 +994,10 @@ bool bad_dim(int ndim, const int* nw, const char* what) {
     set_error("%s: ndim must be 2 or 3, got %d", what, ndim);
     return true;
   }
+  if (!nw) {
+    set_error("%s: null grid size array", what);
+    return true;
+  }
   double cells = 1;
   for (int d = 0; d < ndim; ++d) {
     if (nw[d] < 1) {
@@ -1104,6 +1108,10 @@ int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
   if (bad_dim(ndim, nw, "stencil_apply_sym") ||
       bad_var(nvar, "stencil_apply_sym"))
     return kErrArg;
+  if (!items || !ostart || !coef || !x || !y) {
+    set_error("stencil_apply_sym: null table, coefficient or field array");
+    return kErrArg;
+  }
   if (mode == 2 && !dent) {
     set_error("stencil_apply_sym: Jacobi needs the diagonal entries");
     return kErrArg;

@@ -57,15 +57,17 @@ int odil_wave_jac(int dtype, int nt, int nx, double dt, double dx,
 }
 
 int odil_wave_jac_entries(int* ent, int* shifts) {
-  for (int q = 0; q < WaveOp<double>::NJ; ++q) {
-    ent[3 * q] = WaveOp<double>::JE[q];
-    ent[3 * q + 1] = WaveOp<double>::JV[q];
-    ent[3 * q + 2] = WaveOp<double>::JS[q];
-  }
-  for (int s = 0; s < WaveOp<double>::NS; ++s) {
-    shifts[2 * s] = WaveOp<double>::SX[s];
-    shifts[2 * s + 1] = WaveOp<double>::SY[s];
-  }
+  if (ent)
+    for (int q = 0; q < WaveOp<double>::NJ; ++q) {
+      ent[3 * q] = WaveOp<double>::JE[q];
+      ent[3 * q + 1] = WaveOp<double>::JV[q];
+      ent[3 * q + 2] = WaveOp<double>::JS[q];
+    }
+  if (shifts)
+    for (int s = 0; s < WaveOp<double>::NS; ++s) {
+      shifts[2 * s] = WaveOp<double>::SX[s];
+      shifts[2 * s + 1] = WaveOp<double>::SY[s];
+    }
   return WaveOp<double>::NJ;
 }
 

@@ -130,3 +130,40 @@ def test_operator_entry_points_match_python_calls():
         for what, tail in (('residual', 4), ('loss_grad', 5), ('jac', 4)):
             name = 'odil_{}_{}'.format(op.prefix, what)
             assert counts[name] == base + tail, (name, counts[name])
+
+
+def test_sweep_and_dense_limits_are_checked_on_the_host(lib):
+    """Limits of the symmetric sweep (tiles per workgroup, products per
+    field, shift reach, 32-bit offsets, lanes), of the Cholesky block and of
+    axpby return their status before any launch (CPU-only host)."""
+    lib.odil_last_error.restype = ctypes.c_char_p
+    i3 = lambda *v: (ctypes.c_int * 3)(*v)
+    ok_items = (ctypes.c_int * 10)()
+    one = ctypes.c_void_p(1)   # never dereferenced: every case fails first
+    fake = (ctypes.c_void_p * 3)(16, 32, 48)
+
+    def sym(nw=(9, 9, 1), reach=(1, 1, 0), max_items=8, nent=5, lanes=1,
+            per=1, mode=0, x=fake):
+        return lib.odil_stencil_apply_sym(
+            ctypes.c_int(0), ctypes.c_int(2), i3(*nw), ctypes.c_int(3),
+            ok_items, ok_items, ok_items, i3(*reach), ctypes.c_int(max_items),
+            ctypes.c_int(nent), ctypes.c_int(lanes), ctypes.c_int(per), one,
+            x, fake, None, ctypes.c_int(mode), ctypes.c_double(0.), None)
+    assert sym(x=None) == 1 and b'null' in lib.odil_last_error()
+    assert sym(per=0) == 1 and b'tiles per workgroup' in lib.odil_last_error()
+    assert sym(max_items=600) == 3
+    assert sym(reach=(600, 0, 0)) == 1
+    assert b'shifts beyond 511' in lib.odil_last_error()
+    assert sym(reach=(1, 1, 1)) == 1          # 2D with a z reach
+    assert sym(nw=(40000, 40000, 1), nent=72) == 3   # 32-bit offsets
+    assert sym(nw=(50000, 50000, 1)) == 1            # 2^31 points
+    assert sym(lanes=3) == 1
+    assert sym(mode=2) in (1, 3) and sym(mode=1) == 1   # no f given
+    assert lib.odil_chol_block_size() == 64
+    assert lib.odil_chol_block(one, ctypes.c_int(128), ctypes.c_int(0),
+                               ctypes.c_int(65), one, None) == 1
+    assert lib.odil_chol_block(one, ctypes.c_int(32), ctypes.c_int(0),
+                               ctypes.c_int(64), one, None) == 1
+    assert lib.odil_axpby(ctypes.c_int(1), ctypes.c_longlong(-1), one, one,
+                          one, ctypes.c_double(1.), ctypes.c_double(1.),
+                          None) == 1
