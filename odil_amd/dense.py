@@ -17,6 +17,7 @@ import torch
 
 from odil_amd import _lib
 from odil_amd._lib import c_int, check, stream
+from odil_amd.graphs import capture
 
 
 def _p(t):
@@ -90,11 +91,7 @@ class SpdSolver:
             # First use: run eagerly (the result stays in Y), then record
             # the same work; capture does not execute it.
             self._body()
-            torch.cuda.synchronize()
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                self._body()
-            self._graph = g
+            self._graph = capture(self._body, self.A.device)
             return self.Y
         self._graph.replay()
         return self.Y

@@ -51,6 +51,7 @@ import torch
 
 from odil_amd import _lib
 from odil_amd._lib import c_double, c_int, c_vp, check, ptrs, stream
+from odil_amd.graphs import capture
 from odil_amd.multigrid import (BlockStencil, MultigridOp, jacobi_zero,
                                normal_operator)
 
@@ -354,15 +355,10 @@ class NewtonMultigrid:
             self._solve_body(iters, finish)
             return
         if key not in self._graph:
-            # First use: run eagerly (the result stays in x), then record the
-            # same work; capture does not execute it.
-            self._solve_body(iters, finish)
-            torch.cuda.synchronize()
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                self._solve_body(iters, finish)
-            self._graph[key] = g
-            return
+            # First use: record the solve (capture does not execute it), then
+            # replay it like every later step.
+            self._graph[key] = capture(
+                lambda: self._solve_body(iters, finish), self.device)
         self._graph[key].replay()
 
     def _linear_solve(self):
