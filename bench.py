@@ -43,6 +43,9 @@ def parse():
                    help='rebuild the stored multigrid hierarchy every k '
                    'Newton steps (the Krylov operator and finest level are '
                    'always current)')
+    p.add_argument('--setup_first', type=int, default=2,
+                   help='also rebuild the hierarchy at each of the first k '
+                   'Newton steps')
     p.add_argument('--overlap_setup', type=int, default=0,
                    help='1: build the next hierarchy on a second stream '
                    'during the linear solve, adopted one step later')
@@ -80,7 +83,9 @@ def make_solver(args, precision):
                          restriction='transpose',
                          use_graph=not args.no_graph,
                          setup_every=args.setup_every,
-                         overlap_setup=bool(args.overlap_setup),
+                         overlap_setup=bool(getattr(args, 'overlap_setup',
+                                                    0)),
+                         setup_first=getattr(args, 'setup_first', 0),
                          krylov=args.krylov if precision == 'mixed' else 'auto',
                          refine=args.refine if precision == 'mixed' else 0)
     fields = [torch.zeros(args.n, args.n, dtype=torch.float64,
@@ -359,12 +364,12 @@ def main():
             'workload': 'lid-driven cavity {0}x{0}, Re={1:g}, Newton with '
                         'GMG normal-equation solve ({5}, {3} iterations/step, '
                         '{2} levels, V(2,2), omega={4}, hierarchy rebuilt '
-                        'every {6} steps{7})'.format(
+                        'every {6} steps and at the first {8}{7})'.format(
                             args.n, args.re, args.nlvl, args.maxiter,
                             args.omega, 'FCG+V-cycle' if args.linsolver ==
                             'pcg' else 'V-cycles', args.setup_every,
                             ' on a second stream' if args.overlap_setup
-                            else ''),
+                            else '', args.setup_first),
             'grid': [args.n, args.n],
             'mixed': 'FP32 flexible CG and V-cycle, FP64 Newton residual, '
                      'update and coarsest solve' + (

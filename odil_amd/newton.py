@@ -157,7 +157,8 @@ class NewtonMultigrid:
                  galerkin_mode='exact', precision='fp64', linsolver='pcg',
                  rtol=0., check_every=10, coarse_reg='auto',
                  matrix_free=True, use_graph=True, setup_every=1,
-                 krylov='auto', refine=0, overlap_setup=False, device=None):
+                 krylov='auto', refine=0, overlap_setup=False,
+                 setup_first=0, device=None):
         if precision not in ('fp64', 'mixed'):
             raise ValueError('precision must be fp64 or mixed')
         if linsolver not in ('vcycle', 'pcg'):
@@ -194,6 +195,9 @@ class NewtonMultigrid:
         # operator and the finest level are always those of the current
         # state, so a lagged hierarchy only changes the preconditioner.
         self.setup_every = max(1, int(setup_every))
+        # ... and at each of the first `setup_first` steps, while the state
+        # still moves far from the one the hierarchy was built for.
+        self.setup_first = max(0, int(setup_first))
         self._nstep = 0
         # J^T J and its Galerkin products R N T with R = T^T / 2^dim are
         # symmetric: store upper halves.
@@ -408,6 +412,7 @@ class NewtonMultigrid:
             self._pending = None
         side = None
         if (self._nstep % self.setup_every == 0 or not self.mg.AA
+                or self._nstep < self.setup_first
                 or not self.matrix_free):
             if self.overlap_setup and self.mg.AA:
                 side = torch.cuda.Event()

@@ -134,6 +134,7 @@ def oracle_cavity65():
     ('mixed', 1e-9, 1e-6, {'krylov': 'fp64'}),      # FP64 Krylov, FP32 MG
     ('mixed', 1e-9, 1e-6, {'refine': 1}),           # + FP64 refinement
     ('mixed', 1e-9, 1e-6, {'setup_every': 4}),      # lagged hierarchy
+    ('mixed', 1e-9, 1e-6, {'setup_every': 4, 'setup_first': 2}),
     ('mixed', 1e-9, 1e-6, {'overlap_setup': True}),  # built on a 2nd stream
     ('fp64', 1e-11, 1e-8, {'setup_every': 2, 'overlap_setup': True}),
 ])
