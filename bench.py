@@ -39,7 +39,7 @@ def parse():
     p.add_argument('--linsolver', default='pcg', choices=['vcycle', 'pcg'])
     p.add_argument('--omega', type=float, default=0.6)
     p.add_argument('--nlvl', type=int, default=7)
-    p.add_argument('--setup_every', type=int, default=4,
+    p.add_argument('--setup_every', type=int, default=12,
                    help='rebuild the stored multigrid hierarchy every k '
                    'Newton steps (the Krylov operator and finest level are '
                    'always current)')
