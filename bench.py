@@ -43,6 +43,9 @@ def parse():
                    help='rebuild the stored multigrid hierarchy every k '
                    'Newton steps (the Krylov operator and finest level are '
                    'always current)')
+    p.add_argument('--coarse_smooth', default='1,1',
+                   help='pre,post Jacobi sweeps on the levels below the '
+                   'finest (the reference recursion uses 2,2)')
     p.add_argument('--setup_first', type=int, default=2,
                    help='also rebuild the hierarchy at each of the first k '
                    'Newton steps')
@@ -54,7 +57,7 @@ def parse():
     p.add_argument('--refine', type=int, default=0,
                    help='FP64 iterative-refinement passes per linear solve')
     p.add_argument('--tol', type=float, default=1e-8)
-    p.add_argument('--tol_maxiter', type=int, default=18,
+    p.add_argument('--tol_maxiter', type=int, default=20,
                    help='linear iterations per Newton step of the '
                         'time-to-tolerance runs (both precisions)')
     p.add_argument('--max_epochs', type=int, default=150)
@@ -86,6 +89,9 @@ def make_solver(args, precision):
                          overlap_setup=bool(getattr(args, 'overlap_setup',
                                                     0)),
                          setup_first=getattr(args, 'setup_first', 0),
+                         coarse_smooth=tuple(
+                             int(x) for x in
+                             getattr(args, 'coarse_smooth', '2,2').split(',')),
                          krylov=args.krylov if precision == 'mixed' else 'auto',
                          refine=args.refine if precision == 'mixed' else 0)
     fields = [torch.zeros(args.n, args.n, dtype=torch.float64,
@@ -363,13 +369,14 @@ def main():
         'config': {
             'workload': 'lid-driven cavity {0}x{0}, Re={1:g}, Newton with '
                         'GMG normal-equation solve ({5}, {3} iterations/step, '
-                        '{2} levels, V(2,2), omega={4}, hierarchy rebuilt '
+                        '{2} levels, V(2,2) on the finest and V({9}) on '
+                        'the coarser levels, omega={4}, hierarchy rebuilt '
                         'every {6} steps and at the first {8}{7})'.format(
                             args.n, args.re, args.nlvl, args.maxiter,
                             args.omega, 'FCG+V-cycle' if args.linsolver ==
                             'pcg' else 'V-cycles', args.setup_every,
                             ' on a second stream' if args.overlap_setup
-                            else '', args.setup_first),
+                            else '', args.setup_first, args.coarse_smooth),
             'grid': [args.n, args.n],
             'mixed': 'FP32 flexible CG and V-cycle, FP64 Newton residual, '
                      'update and coarsest solve' + (

@@ -666,6 +666,9 @@ class MultigridOp:
                                device=self.device)
         self.AA = None
         self._buf = None
+        # Sweeps of the levels below the finest in `vcycle`: the reference's
+        # recursion uses its defaults 2 and 2 (multigrid.py:590-593).
+        self.coarse_pre, self.coarse_post = 2, 2
         # Optional finest-level operator used instead of AA[0] for
         # smoothing and residuals (same apply/residual/jacobi interface),
         # e.g. a matrix-free J^T J; AA[0] still seeds the Galerkin levels.
@@ -875,7 +878,7 @@ class MultigridOp:
         nxt = buf[level + 1]
         # The coarse cycle starts from zero (pre = 2 there): unless it is
         # the direct solve, the restriction also does its first sweep.
-        fused = not self._is_direct(level + 1, 5)
+        fused = not self._is_direct(level + 1, 5) and self.coarse_pre > 0
         if fused:
             restrict_smooth(r, self.nnw[level], self.rw, nxt['f'],
                             self.AA[level + 1].diagonal(), nxt['v'], omega,
@@ -883,8 +886,9 @@ class MultigridOp:
         else:
             restrict(r, self.nnw[level], self.rw, nxt['f'],
                      restriction=self.restriction)
-        res = self._vc(level + 1, nxt['u'], nxt['f'], nxt['v'], omega, 5, 2,
-                       2, zero=True, first_done=fused)
+        res = self._vc(level + 1, nxt['u'], nxt['f'], nxt['v'], omega, 5,
+                       self.coarse_pre, self.coarse_post, zero=True,
+                       first_done=fused and self.coarse_pre > 0)
         prolong_add(res, cur, self.nnw[level])
         for _ in range(post):
             A.jacobi(cur, f, oth, omega)

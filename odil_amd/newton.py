@@ -158,7 +158,7 @@ class NewtonMultigrid:
                  rtol=0., check_every=10, coarse_reg='auto',
                  matrix_free=True, use_graph=True, setup_every=1,
                  krylov='auto', refine=0, overlap_setup=False,
-                 setup_first=0, device=None):
+                 setup_first=0, coarse_smooth=(2, 2), device=None):
         if precision not in ('fp64', 'mixed'):
             raise ValueError('precision must be fp64 or mixed')
         if linsolver not in ('vcycle', 'pcg'):
@@ -211,10 +211,17 @@ class NewtonMultigrid:
             # (~1e-5 sigma_max, squared 1e-10): 1e-10 would damp the
             # smoothest coarse modes and slows Newton down threefold.
             coarse_reg = 1e-14 if precision == 'fp64' else 1e-12
-        mk = lambda: MultigridOp(op.shape, nvar=self.nvar, nlevels=nlvl,
-                                 restriction=restriction, dtype=self.wdtype,
-                                 device=self.device, coarse_reg=coarse_reg,
-                                 galerkin_mode=galerkin_mode)
+        self.coarse_smooth = tuple(int(x) for x in coarse_smooth)
+        if len(self.coarse_smooth) != 2 or min(self.coarse_smooth) < 0:
+            raise ValueError('coarse_smooth must be (pre, post) >= 0')
+
+        def mk():
+            m = MultigridOp(op.shape, nvar=self.nvar, nlevels=nlvl,
+                            restriction=restriction, dtype=self.wdtype,
+                            device=self.device, coarse_reg=coarse_reg,
+                            galerkin_mode=galerkin_mode)
+            m.coarse_pre, m.coarse_post = self.coarse_smooth
+            return m
         self.mg = mk()
         # overlap_setup: rebuilds after the first run on a second stream
         # into a second hierarchy while the current step's linear solve

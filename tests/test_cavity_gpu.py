@@ -135,6 +135,9 @@ def oracle_cavity65():
     ('mixed', 1e-9, 1e-6, {'refine': 1}),           # + FP64 refinement
     ('mixed', 1e-9, 1e-6, {'setup_every': 4}),      # lagged hierarchy
     ('mixed', 1e-9, 1e-6, {'setup_every': 4, 'setup_first': 2}),
+    ('mixed', 1e-9, 1e-6, {'setup_every': 12, 'setup_first': 2,
+                           'coarse_smooth': (1, 1)}),
+    ('fp64', 1e-11, 1e-8, {'coarse_smooth': (0, 1)}),
     ('mixed', 1e-9, 1e-6, {'overlap_setup': True}),  # built on a 2nd stream
     ('fp64', 1e-11, 1e-8, {'setup_every': 2, 'overlap_setup': True}),
 ])
@@ -170,6 +173,8 @@ def test_newton_multigrid_options_rejected(dev):
                         refine=1)
     with pytest.raises(ValueError):
         NewtonMultigrid(op, nlvl=3, precision='mixed', krylov='bf16')
+    with pytest.raises(ValueError):
+        NewtonMultigrid(op, nlvl=3, coarse_smooth=(1, -1))
 
 
 # -- inverse-problem terms (imposed points, kreg) ----------------------------
