@@ -916,8 +916,14 @@ __global__ void __launch_bounds__(256)
 }
 
 // y = M x for a dense row-major FP64 matrix, one wavefront per row; x and y
-// may be FP32 or FP64 (the coarse solve stays FP64 inside).
-template <class Tx, class Ty>
+// may be FP32 or FP64 (the coarse solve stays FP64 inside). CONTIG: the x
+// fields are one allocation (x_k = x_0 + k N), so the row and x are both
+// read as contiguous vectors of nvar N values in chunks of 16 columns per
+// lane, fully unrolled with masked tails (a column past the row reads
+// element 0 and adds an exact zero): all of a chunk's loads are in flight
+// at once, and a coarsest cavity row of 867 columns is one chunk. Otherwise
+// field by field.
+template <class Tx, class Ty, bool CONTIG>
 __global__ void __launch_bounds__(256)
     k_gemv(const double* __restrict__ M, CVecPtrs<Tx> x, VecPtrs<Ty> y,
            int nvar, int N) {
@@ -927,20 +933,25 @@ __global__ void __launch_bounds__(256)
   if (row >= n) return;
   double acc = 0;
   const double* __restrict__ m = M + (size_t)row * n;
-  // The row is contiguous over all fields; the field of column j is found
-  // by comparisons (no division), so one unrolled loop keeps all of a
-  // lane's loads in flight at once (a coarsest cavity row of 867 columns
-  // is 14 loads per lane: one memory round trip).
-#pragma unroll 16
-  for (int j = lane; j < n; j += 64) {
-    int k = 0, jj = j;
+  if (CONTIG) {
+    const Tx* __restrict__ xv = x.p[0];
+    for (int j0 = 0; j0 < n; j0 += 16 * 64) {
 #pragma unroll
-    for (int t = 1; t < kMaxVar; ++t)
-      if (t < nvar && jj >= N) {
-        jj -= N;
-        k = t;
+      for (int t = 0; t < 16; ++t) {
+        const int j = j0 + t * 64 + lane;
+        const bool ok = j < n;
+        const double a = m[ok ? j : 0];
+        const double b = (double)xv[ok ? j : 0];
+        acc += ok ? a * b : 0.0;
       }
-    acc += m[j] * (double)pick(x, k)[jj];
+    }
+  } else {
+    for (int k = 0; k < nvar; ++k) {
+      const double* __restrict__ mk = m + (size_t)k * N;
+      const Tx* __restrict__ xk = x.p[k];
+#pragma unroll 8
+      for (int j = lane; j < N; j += 64) acc += mk[j] * (double)xk[j];
+    }
   }
   acc = wave_sum(acc);
   if (lane == 0) {
@@ -1583,18 +1594,31 @@ int odil_dense_gemv(int dtype_x, int dtype_y, int nvar, int N,
                     const double* M, const void* const* x, void* const* y,
                     void* stream) {
   if (bad_var(nvar, "dense_gemv")) return kErrArg;
+  if (!M || !x || !y) {
+    set_error("dense_gemv: null matrix or field array");
+    return kErrArg;
+  }
   hipStream_t s = static_cast<hipStream_t>(stream);
   dim3 grid(ceil_div(nvar * N, 4));
-  if (dtype_x == kF64 && dtype_y == kF64)
-    hipLaunchKernelGGL((k_gemv<double, double>), grid, dim3(256), 0, s, M,
-                       cvp<double>(x, nvar), vp<double>(y, nvar), nvar, N);
-  else if (dtype_x == kF32 && dtype_y == kF32)
-    hipLaunchKernelGGL((k_gemv<float, float>), grid, dim3(256), 0, s, M,
-                       cvp<float>(x, nvar), vp<float>(y, nvar), nvar, N);
-  else {
+  const size_t esz = dtype_x == kF64 ? 8 : 4;
+  bool contig = true;
+  for (int k = 1; k < nvar; ++k)
+    contig = contig && static_cast<const char*>(x[k]) ==
+                           static_cast<const char*>(x[0]) + k * (size_t)N * esz;
+#define ODIL_GEMV(TX, TY, C)                                                  \
+  hipLaunchKernelGGL((k_gemv<TX, TY, C>), grid, dim3(256), 0, s, M,          \
+                     cvp<TX>(x, nvar), vp<TY>(y, nvar), nvar, N)
+  if (dtype_x == kF64 && dtype_y == kF64) {
+    if (contig) ODIL_GEMV(double, double, true);
+    else ODIL_GEMV(double, double, false);
+  } else if (dtype_x == kF32 && dtype_y == kF32) {
+    if (contig) ODIL_GEMV(float, float, true);
+    else ODIL_GEMV(float, float, false);
+  } else {
     set_error("dense_gemv: unsupported dtypes");
     return kErrArg;
   }
+#undef ODIL_GEMV
   return check_launch("dense_gemv");
 }
 
