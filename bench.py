@@ -145,8 +145,8 @@ def _event_time(fn, reps, use_graph):
     return e0.elapsed_time(e1) / reps
 
 
-def dominant_kernel(args, nm):
-    """The kernel with the largest share of the mixed Newton step: the
+def level1_sweep_kernel(args, nm):
+    """The largest stored-stencil kernel of the mixed Newton step: the
     weighted-Jacobi sweep of the stored (upper-half symmetric) Galerkin
     operator on multigrid level 1 (513 x 513 nodes, FP32), run on the
     solver's own operator and level buffers. Algorithmic bytes per launch:
@@ -166,8 +166,33 @@ def dominant_kernel(args, nm):
             'nent': len(A.entries), 'grid': list(A.nw)}
 
 
+def normal_sweep_kernel(args, nm):
+    """The kernel with the largest share of the mixed Newton step: the
+    matrix-free weighted-Jacobi sweep of N = J^T J on the finest level
+    (`k_normal_apply<float>`, mode 2, 1025 x 1025 cells), run on the
+    solver's FP32 state copy and diagonal with synthetic x and b. Algorithmic
+    bytes per launch: state, x, b and diag(N) read and y written, three
+    fields each: 15 * N * 4."""
+    import torch
+    st = nm.state_w
+    g = torch.Generator(device='cuda').manual_seed(1)
+    x = [torch.rand(a.shape, dtype=a.dtype, device='cuda', generator=g)
+         for a in st]
+    b = [torch.rand(a.shape, dtype=a.dtype, device='cuda', generator=g)
+         for a in st]
+    y = [torch.empty_like(a) for a in st]
+    ms = _event_time(lambda: nm.op.normal_apply(st, x, y, b=b,
+                                                d=nm.fine.diag,
+                                                omega=args.omega, mode=2),
+                     args.kernel_reps, not args.no_graph)
+    n = st[0].numel()
+    nbytes = 15 * n * st[0].element_size()
+    return {'ms': ms, 'GBps': nbytes / (ms * 1e-3) / 1e9, 'bytes': nbytes,
+            'grid': list(st[0].shape)}
+
+
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles',
-                            'traffic_apply_sym_level1_cavity1025.txt')
+                            'traffic_normal_apply_cavity1025.txt')
 
 
 def measured_traffic():
@@ -336,7 +361,8 @@ def main():
     if args.copy_probe:
         copy_probe()
     sps_mixed, ms_mixed, nm_mixed = steps_per_second(args, 'mixed')
-    dom = dominant_kernel(args, nm_mixed)
+    lvl1 = level1_sweep_kernel(args, nm_mixed)
+    dom = normal_sweep_kernel(args, nm_mixed)
     del nm_mixed
     sps_64, ms_64, _ = steps_per_second(args, 'fp64')
     tol = {}
@@ -351,7 +377,8 @@ def main():
         d.append(torch.max(torch.abs(pm - p6)).item())
         tol['mixed_vs_fp64_maxabs'] = {'u': d[0], 'v': d[1], 'p': d[2]}
     kb = kernel_bandwidth(args)
-    kb['mg_level1_jacobi_fp32'] = dom
+    kb['mg_level1_jacobi_fp32'] = lvl1
+    kb['normal_jacobi_fp32'] = dom
     line = {
         'metric': 'Newton steps/s + residual/adjoint GB/s at 1024^2, '
                   'mixed vs FP64, same tol',
@@ -391,9 +418,9 @@ def main():
         'kernels': kb,
         'roofline': {
             'bound': 'hbm',
-            'kernel': 'k_apply_sym<float> (level-1 Galerkin Jacobi sweep, '
-                      '{}x{}, {} stored coefficient fields, FP32)'.format(
-                          dom['grid'][0], dom['grid'][1], dom['nent']),
+            'kernel': 'k_normal_apply<float> (matrix-free J^T J '
+                      'weighted-Jacobi sweep on the finest level, {}x{} '
+                      'cells, FP32)'.format(dom['grid'][0], dom['grid'][1]),
             'achieved': dom['GBps'], 'peak': HBM_PEAK_GBPS, 'unit': 'GB/s',
             'frac': dom['GBps'] / HBM_PEAK_GBPS,
             'traffic': measured_traffic(),
