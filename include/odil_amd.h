@@ -110,6 +110,20 @@ int odil_cavity_loss_grad(int dtype, int nx, int ny, double dx, double dy,
                           const void* const* fields, void* const* grad,
                           double* partial, int* nblocks, void* stream);
 int odil_cavity_nblocks(int kind, int dtype, int nx, int ny);
+/* Allocation-free variant of odil_cavity_loss_grad. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops). It takes no workspace: the
+ * per-wavefront partial sums of f^2 live in a device array of the library
+ * (3 x 65536 doubles per device, held until the process ends) and their
+ * fixed-order sums go to ssq[0..2] (device, the values odil_reduce_partials
+ * gives for odil_cavity_loss_grad's partials). Grids that need more than
+ * 65536 partials per equation (FP64 above about 2048 x 2048 cells) return
+ * ODIL_ERR_CAPACITY. Calls that overlap on one device race on the array. */
+int odil_cavity_loss_grad_noalloc(int dtype, int nx, int ny, double dx,
+                                  double dy, double mu, int bc, int rhiechow,
+                                  const void* const* fields,
+                                  void* const* grad, double* ssq,
+                                  void* stream);
 /* Extra loss terms of the cavity inverse problems (ns_steady.py:396-411),
  * appended to the cavity residual as six equations on (u, v, p):
  * mask (u - u_imp) cimp, mask (v - v_imp) cimp (imposed points, cimp =

@@ -12,6 +12,8 @@
 //   fu <- u: lo1 upwind + Laplacian (5-point), fu <- p: central (3-point)
 //   fp <- u, v: face averages, fp <- p: Rhie-Chow pressure Laplacian
 // and every equation is linear in the live fields.
+#include <mutex>
+
 #include "stencil2d.h"
 
 namespace odil {
@@ -698,6 +700,110 @@ int odil_cavity_jtf(int dtype, int nx, int ny, double dx, double dy,
   return dispatch_jtf<CavityOp<double>, CavityOp<float>>(
       dtype, fields, out, partial, nx, ny, scale, P,
       static_cast<hipStream_t>(stream), nblocks);
+}
+
+}  // extern "C"
+
+// ---- allocation-free loss + adjoint ----------------------------------------
+// The per-wavefront partial sums of squares of odil_cavity_loss_grad_noalloc
+// live in this file-scope device array (one copy per device, held until the
+// process ends): 3 equations x kNoAllocParts doubles = 1.5 MiB. Host code
+// never passes `scratch` itself to a kernel (its host-side name is not a
+// device address); it uses the per-device address from hipGetSymbolAddress.
+namespace odil {
+namespace {
+constexpr int kNoAllocParts = 65536;
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace
+}  // namespace odil
+
+static __device__ __align__(256) double scratch[3 * odil::kNoAllocParts];
+
+namespace odil {
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+int scratch_address(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error("cavity_loss_grad_noalloc: hipGetDevice: %s",
+              hipGetErrorString(e));
+    return kErrLaunch;
+  }
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    set_error("cavity_loss_grad_noalloc: device %d beyond %d", dev,
+              kNoAllocMaxDevices);
+    return kErrCapacity;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("cavity_loss_grad_noalloc: hipGetSymbolAddress: %s",
+                hipGetErrorString(e));
+      return kErrLaunch;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return kOk;
+}
+}  // namespace
+}  // namespace odil
+
+extern "C" {
+
+// As odil_cavity_loss_grad, but allocates nothing and needs no workspace:
+// the per-wavefront partials go to the library's device array and their
+// fixed-order sums to ssq[0..2] (device; sum of f_e^2 per equation, the same
+// values odil_reduce_partials gives for odil_cavity_loss_grad's partials).
+// Grids needing more than 65536 partials per equation (FP64 above about
+// 2048 x 2048 cells) are rejected with ODIL_ERR_CAPACITY. Calls that overlap
+// on one device race on the shared array.
+int odil_cavity_loss_grad_noalloc(int dtype, int nx, int ny, double dx,
+                                  double dy, double mu, int bc, int rhiechow,
+                                  const void* const* fields,
+                                  void* const* grad, double* ssq,
+                                  void* stream) {
+  if (nx < 3 || ny < 3 || !fields || !grad || !ssq) {
+    set_error("cavity_loss_grad_noalloc: need nx, ny >= 3, fields, grad "
+              "and ssq");
+    return kErrArg;
+  }
+  if (dtype != kF32 && dtype != kF64) {
+    set_error("cavity_loss_grad_noalloc: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  const int need = grad_nblocks(dtype == kF32, nx, ny);
+  if (need > kNoAllocParts) {
+    set_error("cavity_loss_grad_noalloc: %d x %d needs %d partials per "
+              "equation, more than the %d of the library's scratch",
+              nx, ny, need, kNoAllocParts);
+    return kErrCapacity;
+  }
+  double* part = nullptr;
+  int st = scratch_address(&part);
+  if (st != kOk) return st;
+  CavityParams P = cavity_params(dx, dy, mu, bc, rhiechow);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int nb = 0;
+  st = dtype == kF64 ? cavity_dispatch_grad<double>(fields, grad, part, nx,
+                                                    ny, P, s, &nb)
+                     : cavity_dispatch_grad<float>(fields, grad, part, nx,
+                                                   ny, P, s, &nb);
+  if (st != kOk) return st;
+  if (nb != need) {
+    set_error("cavity_loss_grad_noalloc: partial count %d != %d", nb, need);
+    return kErrArg;
+  }
+  hipLaunchKernelGGL(k_reduce_partials, dim3(3), dim3(256), 0, s, part, nb,
+                     ssq);
+  return check_launch("cavity_loss_grad_noalloc reduce");
 }
 
 }  // extern "C"
