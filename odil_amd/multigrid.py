@@ -8,6 +8,7 @@
 """
 
 import ctypes
+import math
 import itertools
 
 import numpy as np
@@ -673,6 +674,71 @@ class MultigridOp:
         # smoothing and residuals (same apply/residual/jacobi interface),
         # e.g. a matrix-free J^T J; AA[0] still seeds the Galerkin levels.
         self.fine = None
+        # Per-level upper bounds of the Jacobi damping (see
+        # estimate_damping); None: the caller's omega on every level.
+        self.omega_cap = None
+
+    # -- Jacobi damping bounds -------------------------------------------
+    def estimate_damping(self, limit=1.8, iters=20, seed=0):
+        """Cap the Jacobi damping of each level at limit / lambda_l, with
+        lambda_l the largest eigenvalue of D^-1 A on that level (A
+        symmetric positive semi-definite, D its diagonal).
+
+        Weighted Jacobi on such an A is stable only for omega lambda_max <
+        2. The normal equations J^T J of the cavity have lambda_max of
+        3.0-3.3 at cell Reynolds numbers below ~4 (omega = 0.6 is stable)
+        but 3.7 and 4.4 at Re = 1000 on 129^2 and 65^2 cells, where a fixed
+        omega = 0.6 smoother diverges on the highest modes and the
+        preconditioned Krylov solve stalls.
+
+        Estimate: `iters` power steps x <- (I + D^-1 A) x from a seeded
+        random vector, each one Jacobi sweep with omega = -1 and a zero
+        right-hand side (no normalisation: (1 + lambda)^iters stays far
+        below the FP32 range), then 1 + lambda_l = |x_k| / |x_(k-1)|. The
+        estimate approaches lambda_max from below; a limit of 1.8 leaves
+        10 % for that. One host synchronisation for all levels. The caps
+        are rounded down to steps of 2^(1/16) so that small changes between
+        hierarchy rebuilds keep the same value (a captured solve that
+        bakes the damping in stays valid). Returns the caps (also stored in
+        omega_cap; the coarsest level, solved directly, has none)."""
+        buf = self._buffers()
+        g = torch.Generator(device=self.device).manual_seed(seed)
+        nrm = []
+        for level in range(self.nlevels - 1):
+            A = self.AA[level]
+            b = buf[level]
+            zero = b['r']
+            for z in zero:
+                z.zero_()
+            x, y = b['u'], b['v']
+            for a in x:
+                a.copy_(torch.rand(a.shape, dtype=a.dtype, device=a.device,
+                                   generator=g))
+            for _ in range(iters):
+                A.jacobi(x, zero, y, -1.0)
+                x, y = y, x
+            nrm.append(torch.stack([
+                sum((a.double()**2).sum() for a in x),
+                sum((a.double()**2).sum() for a in y)]))
+        vals = torch.stack(nrm).cpu().tolist() if nrm else []
+        lam, caps = [], []
+        for last, prev in vals:
+            v = (last / prev)**0.5 - 1 if prev > 0 else 0.
+            lam.append(v)
+            c = limit / max(v, 1e-300)
+            caps.append(2.0**(math.floor(16 * math.log2(c)) / 16))
+        self.lambda_max = lam + [None]
+        self.omega_cap = caps + [None]
+        return self.omega_cap
+
+    def level_omega(self, level, omega):
+        """Damping used on `level` for the caller's omega."""
+        if self.omega_cap is None or self.omega_cap[level] is None:
+            return omega
+        return min(omega, self.omega_cap[level])
+
+    def omega_key(self):
+        return None if self.omega_cap is None else tuple(self.omega_cap)
 
     # -- setup -----------------------------------------------------------
     def update_A(self, A0):
@@ -860,6 +926,8 @@ class MultigridOp:
             return u
         A = self._op(level)
         buf = self._buffers()
+        omega_in = omega
+        omega = self.level_omega(level, omega_in)
         cur, oth = u, spare
         k0 = 0
         if zero and pre > 0:
@@ -881,12 +949,13 @@ class MultigridOp:
         fused = not self._is_direct(level + 1, 5) and self.coarse_pre > 0
         if fused:
             restrict_smooth(r, self.nnw[level], self.rw, nxt['f'],
-                            self.AA[level + 1].diagonal(), nxt['v'], omega,
+                            self.AA[level + 1].diagonal(), nxt['v'],
+                            self.level_omega(level + 1, omega_in),
                             restriction=self.restriction)
         else:
             restrict(r, self.nnw[level], self.rw, nxt['f'],
                      restriction=self.restriction)
-        res = self._vc(level + 1, nxt['u'], nxt['f'], nxt['v'], omega, 5,
+        res = self._vc(level + 1, nxt['u'], nxt['f'], nxt['v'], omega_in, 5,
                        self.coarse_pre, self.coarse_post, zero=True,
                        first_done=fused and self.coarse_pre > 0)
         prolong_add(res, cur, self.nnw[level])

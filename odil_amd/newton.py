@@ -158,7 +158,8 @@ class NewtonMultigrid:
                  rtol=0., check_every=10, coarse_reg='auto',
                  matrix_free=True, use_graph=True, setup_every=1,
                  krylov='auto', refine=0, overlap_setup=False,
-                 setup_first=0, coarse_smooth=(2, 2), device=None):
+                 setup_first=0, coarse_smooth=(2, 2), omega_auto=True,
+                 device=None):
         if precision not in ('fp64', 'mixed'):
             raise ValueError('precision must be fp64 or mixed')
         if linsolver not in ('vcycle', 'pcg'):
@@ -187,6 +188,11 @@ class NewtonMultigrid:
         self.rtol = float(rtol)
         self.check_every = max(1, int(check_every))
         self.omega = float(omega)
+        # Cap the damping per level at 1.8 / lambda_max(D^-1 A), estimated
+        # at every hierarchy build (MultigridOp.estimate_damping): a fixed
+        # omega diverges on the normal equations of high cell Reynolds
+        # numbers.
+        self.omega_auto = bool(omega_auto)
         self.pre, self.post, self.ndirect = smooth_pre, smooth_post, ndirect
         self.matrix_free = matrix_free
         self.use_graph = use_graph
@@ -334,6 +340,7 @@ class NewtonMultigrid:
                 a.copy_(b)
         iters = self.maxiter
         for _ in range(self.refine):
+            # (With rtol the passes run eagerly with the full budget.)
             # FP64 residual of the current solution, solved again in FP32.
             self.op.normal_apply(self._fields, self.x, self.r64, b=self.rhs,
                                  mode=1)
@@ -365,12 +372,17 @@ class NewtonMultigrid:
         if not self.use_graph:
             self._solve_body(iters, finish)
             return
+        # The damping of each level is a kernel argument inside the graph:
+        # recapture when the hierarchy's caps changed.
+        om = self.mg.omega_key()
+        if key in self._graph and self._graph[key][0] != om:
+            del self._graph[key]
         if key not in self._graph:
             # First use: record the solve (capture does not execute it), then
             # replay it like every later step.
-            self._graph[key] = capture(
-                lambda: self._solve_body(iters, finish), self.device)
-        self._graph[key].replay()
+            self._graph[key] = (om, capture(
+                lambda: self._solve_body(iters, finish), self.device))
+        self._graph[key][1].replay()
 
     def _linear_solve(self):
         """Krylov iterations run (refinement passes included)."""
@@ -385,10 +397,9 @@ class NewtonMultigrid:
             rr, bb = self.sc.t[6].item(), self.sc.t[4].item()
             if bb == 0 or (rr / bb)**0.5 <= self.rtol:
                 break
-        if self.xk is not self.x:
-            for a, b in zip(self.x, self.xk):
-                a.copy_(b)
-        return done
+        # Same FP64 solution copy and refinement passes as a fixed budget.
+        self._finish()
+        return done + self.refine * self.maxiter
 
     # -- Newton ------------------------------------------------------------
     def residual_rms(self, fields):
@@ -432,6 +443,8 @@ class NewtonMultigrid:
                         # stream: it allocates and tunes its launch shapes.
                         mg.update_A(self.N)
                         mg.prepare(self.ndirect)
+                        if self.omega_auto:
+                            mg.estimate_damping()
         self._nstep += 1
         if self.matrix_free:
             if self.state_w is not fields:
@@ -467,6 +480,8 @@ class NewtonMultigrid:
                 normal_operator(J, torch.float64, sym=self.sym)
         mg.update_A(self.N)
         mg.prepare(self.ndirect)
+        if self.omega_auto:
+            mg.estimate_damping()
         return ev
 
     def _setup_side(self, fields, after):

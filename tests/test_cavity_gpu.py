@@ -93,59 +93,95 @@ def test_loss_grad(dev, n0, n1, re, bc, rc, dtype):
                scale=np.abs(grads[name]).max())
 
 
-def test_fixture_state_residual(dev):
-    """Converged Re=100 state shipped by the reference (128x128): the FP64
-    residual kernel agrees with the oracle and is small."""
+@pytest.mark.parametrize('re_', [100, 1000])
+def test_fixture_state_residual(dev, re_):
+    """States shipped by the reference (128x128, 20 direct Newton steps of
+    `cases/cavity_stream/run` at Re 100 and 1000): the FP64 residual kernel
+    agrees with the oracle cell by cell and the states are near converged
+    (Re 100: RMS ~1e-6; Re 1000: ~4e-5 after its 20 steps)."""
     import os
     from odil_amd.operators import Cavity
     path = os.path.join(os.path.dirname(__file__), 'golden',
-                        'cavity_N128_Re100.npz')
+                        'cavity_N128_Re{}.npz'.format(re_))
     d = np.load(path)
     st = {k: d[k] for k in 'uvp'}
-    ff = tape.eval_operator(ops.make_cavity_operator(0.01),
+    mu = 1.0 / re_
+    ff = tape.eval_operator(ops.make_cavity_operator(mu),
                             ops.cavity_domain(128), st)
-    res, ssq = Cavity((128, 128), 0.01).residual(_dev(st, dev,
-                                                      torch.float64))
-    # Round-off is relative to the largest term, mu/h^2 * |u| ~ 164.
+    res, ssq = Cavity((128, 128), mu).residual(_dev(st, dev, torch.float64))
+    # Round-off is relative to the largest term: the convective terms
+    # |u| |u_x| ~ 1/h and the viscous mu |u| / h^2.
+    scale = 128 + mu * 128**2
     for e in range(3):
-        _close(res[e].cpu(), ff[e], 1e-12, scale=0.01 * 128**2)
-        assert np.sqrt(ssq[e].item() / 128**2) < 1e-5
+        _close(res[e].cpu(), ff[e], 1e-12, scale=scale)
+        # sum f^2 of a near-zero residual: each cell carries round-off
+        # ~1e-16 scale, so |d sum f^2| <= 2 |f|_2 sqrt(N) 1e-14 scale.
+        ref = (ff[e]**2).sum()
+        np.testing.assert_allclose(
+            ssq[e].item(), ref, rtol=1e-12,
+            atol=2 * np.sqrt(ref) * 128 * 1e-14 * scale)
+        assert np.sqrt(ssq[e].item() / 128**2) < (1e-5 if re_ == 100
+                                                  else 1e-4)
+    res32, _ = Cavity((128, 128), mu).residual(_dev(st, dev, torch.float32))
+    for e in range(3):
+        _close(res32[e].cpu(), ff[e], RTOL32, scale=128 + mu * 128**2)
 
 
 # -- end to end: Newton with the GMG normal-equation solver ------------------
 
-@pytest.fixture(scope='module')
-def oracle_cavity65():
-    """Converged Re=100 cavity on 65x65 cells from the oracle's Newton with
-    the reference's direct normal-equation solve (linsolver.py:35-45), 40
-    steps from zero (max RMS residual ~2e-12)."""
-    from oracle import newton
-    n = 65
-    dom = ops.cavity_domain(n)
-    zero = {k: np.zeros((n, n)) for k in 'uvp'}
-    packed = newton.optimize_newton(ops.make_cavity_operator(0.01), dom,
-                                    zero, 40)
-    return tape.unpack_fields(dom, packed)
+def _converged(n, re_):
+    """Converged cavity from the oracle's direct Newton (the reference's
+    `optimize_newton` with the SuperLU normal-equation solve,
+    util.py:242-268, linsolver.py:35-45), written by
+    tests/golden/make_cavity_converged.py: max RMS residual <= 1e-13."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), 'golden',
+                        'cavity_converged_N{}_Re{}.npz'.format(n, re_))
+    d = np.load(path)
+    assert d['rms'][int(d['epoch'])].max() <= 1e-13
+    return {k: d[k] for k in 'uvp'}
 
 
-@pytest.mark.parametrize('precision,tol,rtol,kw', [
-    ('fp64', 1e-11, 1e-8, {}),
-    ('mixed', 1e-9, 1e-6, {}),                      # FP32 Krylov
-    ('mixed', 1e-9, 1e-6, {'krylov': 'fp64'}),      # FP64 Krylov, FP32 MG
-    ('mixed', 1e-9, 1e-6, {'refine': 1}),           # + FP64 refinement
-    ('mixed', 1e-9, 1e-6, {'setup_every': 4}),      # lagged hierarchy
-    ('mixed', 1e-9, 1e-6, {'setup_every': 4, 'setup_first': 2}),
-    ('mixed', 1e-9, 1e-6, {'setup_every': 12, 'setup_first': 2,
-                           'coarse_smooth': (1, 1)}),
-    ('fp64', 1e-11, 1e-8, {'coarse_smooth': (0, 1)}),
-    ('mixed', 1e-9, 1e-6, {'overlap_setup': True}),  # built on a 2nd stream
-    ('fp64', 1e-11, 1e-8, {'setup_every': 2, 'overlap_setup': True}),
+# Agreement with the direct solve is limited by the GPU run's own residual:
+# |x - x*| <= |J^-1| |r| with |r| the RMS residual times sqrt(3 n^2). The
+# FP64 and mixed runs stop at max RMS <= 1e-12 (FP64 round-off of the
+# residual at 65^2 is ~1e-14); the assert is 1e-10 of max |x*|.
+NEWTON_TOL = 1e-12
+NEWTON_RTOL = 1e-10
+
+
+def _check_solution(fields, ref, rtol):
+    errs = []
+    for k, name in enumerate('uv'):
+        a = fields[k].cpu().numpy()
+        errs.append(np.abs(a - ref[name]).max() / np.abs(ref[name]).max())
+    p = fields[2].cpu().numpy()
+    pr = ref['p'] - ref['p'].mean()
+    errs.append(np.abs(p - p.mean() - pr).max() / np.abs(pr).max())
+    assert max(errs) <= rtol, 'relative errors u, v, p: {}'.format(
+        ['{:.2e}'.format(e) for e in errs])
+    return errs
+
+
+@pytest.mark.parametrize('precision,kw', [
+    ('fp64', {}),
+    ('mixed', {}),                                  # FP32 Krylov
+    ('mixed', {'krylov': 'fp64'}),                  # FP64 Krylov, FP32 MG
+    ('mixed', {'refine': 1}),                       # + FP64 refinement
+    ('mixed', {'refine': 1, 'rtol': 1e-3, 'check_every': 5}),
+    ('mixed', {'setup_every': 4}),                  # lagged hierarchy
+    ('mixed', {'setup_every': 4, 'setup_first': 2}),
+    ('mixed', {'setup_every': 12, 'setup_first': 2,
+               'coarse_smooth': (1, 1)}),
+    ('fp64', {'coarse_smooth': (0, 1)}),
+    ('mixed', {'overlap_setup': True}),             # built on a 2nd stream
+    ('fp64', {'setup_every': 2, 'overlap_setup': True}),
 ])
-def test_newton_multigrid_matches_oracle_direct(dev, oracle_cavity65,
-                                                precision, tol, rtol, kw):
+def test_newton_multigrid_matches_oracle_direct(dev, precision, kw):
     """The GPU Newton-multigrid solve (FCG on J^T J, V(2,2) preconditioner,
-    5 levels) reaches the same converged cavity solution as the oracle's
-    direct Newton; pressure is compared up to its constant (null mode)."""
+    5 levels) of the Re=100 cavity at 65x65 reaches the converged solution
+    of the oracle's direct Newton; pressure is compared up to its constant
+    (null mode)."""
     from odil_amd.newton import NewtonMultigrid
     from odil_amd.operators import Cavity
     n = 65
@@ -153,13 +189,64 @@ def test_newton_multigrid_matches_oracle_direct(dev, oracle_cavity65,
     nm = NewtonMultigrid(op, nlvl=5, maxiter=10, precision=precision, **kw)
     fields = [torch.zeros(n, n, dtype=torch.float64, device=dev)
               for _ in range(3)]
-    hist = nm.run(fields, 80, tol=tol)
-    assert max(hist[-1][1]) <= tol, hist[-1]
-    ref = oracle_cavity65
-    for k, name in enumerate('uv'):
-        _close(fields[k].cpu().numpy(), ref[name], rtol)
-    p = fields[2].cpu().numpy()
-    _close(p - p.mean(), ref['p'] - ref['p'].mean(), rtol)
+    hist = nm.run(fields, 150, tol=NEWTON_TOL)
+    assert max(hist[-1][1]) <= NEWTON_TOL, hist[-1]
+    _check_solution(fields, _converged(65, 100), NEWTON_RTOL)
+
+
+@pytest.mark.parametrize('n,nlvl,epochs', [(65, 5, 300), (129, 4, 200),
+                                           (257, 5, 200)])
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_newton_multigrid_re1000(dev, n, nlvl, epochs, precision):
+    """Re = 1000 (mu = 1/Re, ns_steady.py:867): the Newton-multigrid solve
+    with the bench's settings (FCG with a V(2,2) cycle on every level,
+    hierarchy rebuilt at the first 2 steps and then every 12) reaches the
+    oracle's direct-Newton solution. The reference's Newton is a Picard
+    iteration (frozen advecting velocity): at Re = 1000 it contracts by
+    ~0.8 (65^2) to ~0.6 (257^2) per step against ~0.5 at Re = 100. At
+    cell Reynolds numbers of 8-15 (65^2, 129^2) a fixed Jacobi damping of
+    0.6 diverges on J^T J and the solve stalls near 1e-2; the per-level
+    damping bound (MultigridOp.estimate_damping) is what converges here."""
+    import os
+    from odil_amd.newton import NewtonMultigrid
+    from odil_amd.operators import Cavity
+    path = os.path.join(os.path.dirname(__file__), 'golden',
+                        'cavity_converged_N{}_Re1000.npz'.format(n))
+    if not os.path.exists(path):
+        pytest.skip('fixture not generated: ' + os.path.basename(path))
+    op = Cavity((n, n), 1e-3)
+    nm = NewtonMultigrid(op, nlvl=nlvl, maxiter=18, precision=precision,
+                         setup_every=12, setup_first=2)
+    fields = [torch.zeros(n, n, dtype=torch.float64, device=dev)
+              for _ in range(3)]
+    hist = nm.run(fields, epochs, tol=NEWTON_TOL)
+    assert max(hist[-1][1]) <= NEWTON_TOL, (len(hist), hist[-1])
+    _check_solution(fields, _converged(n, 1000), NEWTON_RTOL)
+
+
+def test_fixed_damping_stalls_at_high_cell_reynolds(dev):
+    """Without the damping bound (omega = 0.6 on every level) the same
+    65^2 Re = 1000 solve stalls: the smoother is unstable there
+    (lambda_max(D^-1 J^T J) ~ 4.5 > 2 / 0.6), which is why the bound is on
+    by default. Guards the diagnosis, not a result."""
+    from odil_amd.newton import NewtonMultigrid
+    from odil_amd.operators import Cavity
+    n = 65
+    op = Cavity((n, n), 1e-3)
+    res = {}
+    for auto in (False, True):
+        nm = NewtonMultigrid(op, nlvl=3, maxiter=18, precision='fp64',
+                             setup_every=12, setup_first=2,
+                             omega_auto=auto)
+        fields = [torch.zeros(n, n, dtype=torch.float64, device=dev)
+                  for _ in range(3)]
+        hist = nm.run(fields, 60)
+        res[auto] = max(hist[-1][1])
+        if auto:
+            lam = nm.mg.lambda_max[0]
+            assert 4.0 < lam < 5.0, lam
+            assert nm.mg.omega_cap[0] <= 1.8 / lam
+    assert res[True] < 1e-6 < 1e-3 < res[False], res
 
 
 def test_newton_multigrid_options_rejected(dev):

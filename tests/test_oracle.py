@@ -55,12 +55,38 @@ def test_cavity_newton_trajectory(re_, epochs):
         assert abs(dv - g['dv'][epoch]) <= 1e-9 * g['dv'][epoch]
 
 
-def test_cavity_fixture_is_converged():
-    d = np.load(os.path.join(GOLD, 'cavity_N128_Re100.npz'))
-    ff = tape.eval_operator(ops.make_cavity_operator(0.01),
-                            ops.cavity_domain(128), {k: d[k] for k in 'uvp'})
+@pytest.mark.parametrize('re_,bound', [(100, 1e-5), (1000, 1e-4)])
+def test_cavity_fixture_is_converged(re_, bound):
+    """The reference's own N=128 states (20 direct Newton steps at Re 100 and
+    1000, cases/cavity_stream/run) are near zeros of the oracle's operator
+    at mu = 1/Re (ns_steady.py:867): RMS residual 1e-6 / 4e-5 against ~1 at
+    the zero state. With mu off by a factor 2 it would be ~1e-2 / 1e-3."""
+    d = np.load(os.path.join(GOLD, 'cavity_N128_Re{}.npz'.format(re_)))
+    st = {k: d[k] for k in 'uvp'}
+    ff = tape.eval_operator(ops.make_cavity_operator(1.0 / re_),
+                            ops.cavity_domain(128), st)
     for f in ff:
-        assert np.sqrt(np.mean(f**2)) < 1e-5
+        assert np.sqrt(np.mean(f**2)) < bound
+    wrong = tape.eval_operator(ops.make_cavity_operator(2.0 / re_),
+                               ops.cavity_domain(128), st)
+    assert max(np.sqrt(np.mean(f**2)) for f in wrong) > 10 * bound
+
+
+@pytest.mark.parametrize('n,re_', [(65, 100), (65, 1000), (129, 1000)])
+def test_cavity_converged_fixtures(n, re_):
+    """The direct-Newton solutions the GPU Newton tests compare against
+    (written by tests/golden/make_cavity_converged.py) are zeros of the
+    oracle's operator to round-off and have the cavity's flow pattern."""
+    d = np.load(os.path.join(GOLD, 'cavity_converged_N{}_Re{}.npz'.format(
+        n, re_)))
+    st = {k: d[k] for k in 'uvp'}
+    ff = tape.eval_operator(ops.make_cavity_operator(1.0 / re_),
+                            ops.cavity_domain(n), st)
+    for f in ff:
+        assert np.sqrt(np.mean(f**2)) <= 1e-13
+    # The lid drives a clockwise primary vortex: u > 0 near the lid, < 0
+    # below the centre.
+    assert st['u'][:, -1].mean() > 0.5 and st['u'][n // 2, n // 4] < 0
 
 
 # -- tape gradients against central finite differences ---------------------
@@ -442,3 +468,117 @@ def test_decomp_linear_exact(shape):
     np.testing.assert_allclose(od.interp_cell_conv(lin),
                                0.5 + sum(ai * x for ai, x in
                                          zip(a, fine_c)), atol=1e-13)
+
+
+# -- closed forms of the Laplace, Poisson and advection operators ------------
+def _grid2(n0, n1):
+    dom = tape.Domain(ndim=2, shape=(n0, n1), varnames=('x', 'y'),
+                      fieldnames=['u', 'v'])
+    x, y = dom.cell_center_all()
+    return dom, x, y
+
+
+@pytest.mark.parametrize('n0,n1', [(7, 11), (12, 5)])
+def test_laplace_closed_form(n0, n1):
+    """`operator_laplace` (util_op.py:1038-1079) of quadratic fields: the
+    5-point second difference is exact (2a, 2b) in interior cells, and the
+    linear-extrapolation halo (util_op.py:1032-1035) makes the normal second
+    difference of any field vanish in wall cells. So
+    lap(a x^2 + b y^2 + c x y + d x + e y) = 2a [x interior] + 2b [y
+    interior] cell by cell, and the third equation is zero."""
+    dom, x, y = _grid2(n0, n1)
+    u = 3 * x**2 - 5 * y**2 + 0.7 * x * y + 2 * x - y + 1
+    v = -x**2 + 4 * y**2 + 1.3 * x * y - 3 * x + 0.5
+    ff = tape.eval_operator(ops.operator_laplace, dom, {'u': u, 'v': v})
+    ix = np.arange(n0)[:, None] * np.ones((1, n1))
+    iy = np.ones((n0, 1)) * np.arange(n1)[None, :]
+    inx = ((ix > 0) & (ix < n0 - 1)).astype(float)
+    iny = ((iy > 0) & (iy < n1 - 1)).astype(float)
+    np.testing.assert_allclose(ff[0], 2 * 3 * inx + 2 * (-5) * iny,
+                               atol=1e-9)
+    np.testing.assert_allclose(ff[1], 2 * (-1) * inx + 2 * 4 * iny,
+                               atol=1e-9)
+    np.testing.assert_array_equal(ff[2], 0)
+    # A linear field has zero Laplacian everywhere, walls included.
+    lin = {'u': 2 * x - 3 * y + 1, 'v': -x + 0.25 * y}
+    for f in tape.eval_operator(ops.operator_laplace, dom, lin):
+        np.testing.assert_allclose(f, 0, atol=1e-9)
+
+
+@pytest.mark.parametrize('n0,n1', [(8, 13), (17, 6)])
+def test_poisson_closed_form(n0, n1):
+    """`make_poisson_operator`: u = x(1-x) y(1-y) vanishes on every wall of
+    the unit square and is quadratic along each axis, so both the interior
+    second differences and the wall halo (quadratic through the wall value,
+    util_op.py:1017-1023) are exact: f = -2 (x(1-x) + y(1-y)) - src in every
+    cell, wall cells included."""
+    dom, x, y = _grid2(n0, n1)
+    dom.fieldnames = ['u']
+    src = np.random.default_rng(n0).normal(size=(n0, n1))
+    u = x * (1 - x) * y * (1 - y)
+    ff = tape.eval_operator(ops.make_poisson_operator(src), dom, {'u': u})
+    np.testing.assert_allclose(ff[0], -2 * (x * (1 - x) + y * (1 - y)) - src,
+                               atol=1e-10)
+    # A field that is quadratic but not zero on the walls shows the wall rule:
+    # u = 1 (constant) has halo (1 - 6 + 0)/3 = -5/3 at each wall, so a wall
+    # cell's second difference along that axis is (-5/3 - 2 + 1)/h^2.
+    ff = tape.eval_operator(ops.make_poisson_operator(0 * src), dom,
+                            {'u': np.ones((n0, n1))})
+    hx, hy = 1. / n0, 1. / n1
+    want = np.zeros((n0, n1))
+    want[0, :] += (-5 / 3 - 1) / hx**2
+    want[-1, :] += (-5 / 3 - 1) / hx**2
+    want[:, 0] += (-5 / 3 - 1) / hy**2
+    want[:, -1] += (-5 / 3 - 1) / hy**2
+    np.testing.assert_allclose(ff[0], want, rtol=1e-12, atol=1e-9)
+
+
+@pytest.mark.parametrize('ax,ay', [(0.7, -0.4), (-0.9, 0.3), (0.5, 0.8),
+                                   (-0.2, -0.6)])
+def test_advection_closed_form(ax, ay):
+    """Advection residual (advection_2d.py:129-138,200-220) of
+    p = beta t + x^2 + y^2 with first-order upwinding on the velocity sign
+    s = sign(a): (x_i^2 - x_{i-1}^2)/dx = 2x - dx for a > 0 and
+    (x_{i+1}^2 - x_i^2)/dx = 2x + dx otherwise, and (p - p(t - dt))/dt =
+    beta exactly, so f = beta + ax (2x - s_x dx) + ay (2y - s_y dy) in every
+    cell except on the periodic seam; the t = 0 row uses p_init, here
+    p(t_0 - dt). On the seam the upwind neighbour wraps around (x_{-1} :=
+    x_{n-1})."""
+    nt, nx, ny = 5, 9, 7
+    beta = 1.7
+    from oracle.tape import Domain
+    dom = Domain(ndim=3, shape=(nt, nx, ny), upper=(1., 1., 1.),
+                 varnames=('t', 'x', 'y'), fieldnames=['p'])
+    t, x, y = dom.cell_center_all()
+    dt, dx, dy = 1. / nt, 1. / nx, 1. / ny
+    p = beta * t + x**2 + y**2
+    p_init = beta * (t[0] - dt) + x[0]**2 + y[0]**2
+    ff = tape.eval_operator(ops.make_advection_operator(ax, ay, p_init),
+                            dom, {'p': p})[0]
+    sx = 1. if ax > 0 else -1.
+    sy = 1. if ay > 0 else -1.
+    want = beta + ax * (2 * x - sx * dx) + ay * (2 * y - sy * dy)
+    seam_x = 0 if ax > 0 else nx - 1
+    seam_y = 0 if ay > 0 else ny - 1
+    keep = np.ones(p.shape, bool)
+    keep[:, seam_x, :] = False
+    keep[:, :, seam_y] = False
+    np.testing.assert_allclose(ff[keep], want[keep], rtol=1e-12, atol=1e-11)
+    # Seam cells: the upwind neighbour is the periodic image.
+    xs = x[:, seam_x, :]
+    xn = x[:, nx - 1 if ax > 0 else 0, :]
+    px = (xs**2 - xn**2) / dx if ax > 0 else (xn**2 - xs**2) / dx
+    ys = y[:, :, seam_y]
+    yn = y[:, :, ny - 1 if ay > 0 else 0]
+    py = (ys**2 - yn**2) / dy if ay > 0 else (yn**2 - ys**2) / dy
+    wx = beta + ax * px + ay * (2 * y[:, seam_x, :] -
+                                sy * dy)
+    sel = np.ones(ny, bool)
+    sel[seam_y] = False
+    np.testing.assert_allclose(ff[:, seam_x, :][:, sel], wx[:, sel],
+                               rtol=1e-12, atol=1e-11)
+    wy = beta + ax * (2 * x[:, :, seam_y] - sx * dx) + ay * py
+    sel = np.ones(nx, bool)
+    sel[seam_x] = False
+    np.testing.assert_allclose(ff[:, :, seam_y][:, sel], wy[:, sel],
+                               rtol=1e-12, atol=1e-11)
