@@ -197,12 +197,26 @@ class PoissonBatchSolver:
     iterations on J^T J (FP64, matrix-free) preconditioned by one V(2,2)
     cycle of the shared hierarchy of the instance's size (FP32 in 'mixed').
     `solve()` replays the batch's HIP graphs (at most `graph_instances`
-    instances per graph)."""
+    instances per graph).
+
+    A graph of the whole 256-instance batch (about 180 000 kernel nodes in
+    one dependency chain) ended the process with a host segmentation fault
+    during its first use; graphs of at most MAX_GRAPH_INSTANCES instances
+    (about 6 000 nodes) are the largest verified, so larger values of
+    `graph_instances` are refused."""
+
+    MAX_GRAPH_INSTANCES = 8
 
     def __init__(self, batch, precision='mixed', maxiter=10, omega=0.5,
                  nlvl=None, ndirect=3, use_graph=True, graph_instances=8):
         self.batch = batch
-        self.graph_instances = max(1, int(graph_instances))
+        gi = int(graph_instances)
+        if gi < 1 or gi > self.MAX_GRAPH_INSTANCES:
+            raise ValueError(
+                'graph_instances must be 1..{} (larger single graphs are '
+                'not verified: a whole-batch graph crashed the host '
+                'process)'.format(self.MAX_GRAPH_INSTANCES))
+        self.graph_instances = gi
         self.maxiter = int(maxiter)
         self.omega = float(omega)
         self.ndirect = ndirect

@@ -187,21 +187,18 @@ __device__ inline T ld_b32(const T* base, unsigned bytes) {
 }
 
 // B32: every coefficient and x offset fits in 32 bits as bytes (checked on
-// the host); the staged offsets are then byte offsets.
-// LANES == 1: the item index is uniform across the wavefront, so items are
-// read straight from the global table with scalar loads (scalar cache,
-// offsets formed in scalar ALU): no LDS staging and no barrier ahead of the
-// products. The epilogue operands (f, x, diagonal) are loaded before the
-// products so that their latency overlaps them.
+// the host); the staged offsets are then byte offsets. The item table of
+// the workgroup's output field is staged in LDS once (flat offsets and
+// packed shifts); the epilogue operands (f, x, diagonal) are loaded before
+// the products so that their latency overlaps them.
 template <class T, int LANES, bool CONTIG, bool B32>
 __global__ void __launch_bounds__(256)
     k_apply_sym(const int* __restrict__ items, const int* __restrict__ ostart,
                 const int* __restrict__ dent, const T* __restrict__ coef,
                 CVecPtrs<T> x, VecPtrs<T> y, CVecPtrs<T> f, Grid3 nw,
                 Grid3 reach, int mode, T omega, int per) {
-  constexpr bool SCALAR = false;
-  __shared__ int2 fast[SCALAR ? 1 : kMaxItems];
-  __shared__ int2 shf[SCALAR ? 1 : kMaxItems];
+  __shared__ int2 fast[kMaxItems];
+  __shared__ int2 shf[kMaxItems];
   // Logical block: a chunk of `per` consecutive tiles, output field o
   // fastest (one XCD per chunk); the item list is staged once per chunk.
   const unsigned g = xcd_block(blockIdx.x + blockIdx.y * gridDim.x,
@@ -212,37 +209,13 @@ __global__ void __launch_bounds__(256)
   const int ne = ostart[o + 1] - i0e;
   const int N = (int)nw.size();
   constexpr unsigned S = B32 ? sizeof(T) : 1;
-  const int* __restrict__ itab = items + 10 * i0e;
-  if constexpr (!SCALAR) {
-    stage_items<CONTIG>(items, i0e, ne, N, fast, shf, S);
-    __syncthreads();
-  }
+  stage_items<CONTIG>(items, i0e, ne, N, fast, shf, S);
+  __syncthreads();
   // Flat (scaled) offsets and packed shifts of item e.
-  auto item = [&](int e) -> int2 {
-    if constexpr (SCALAR) {
-      const int* it = itab + 10 * e;
-      return make_int2(
-          (int)((unsigned)(it[0] * N + it[2]) * S),
-          (int)((unsigned)((CONTIG ? it[1] * N : 0) + it[3]) * S));
-    } else {
-      return fast[e];
-    }
-  };
-  auto shifts = [&](int e) -> int2 {
-    if constexpr (SCALAR) {
-      const int* it = itab + 10 * e;
-      return make_int2(pack_shift(it[4], it[5], it[6]),
-                       pack_shift(it[7], it[8], it[9]) |
-                           (int)((unsigned)it[1] << 30));
-    } else {
-      return shf[e];
-    }
-  };
+  auto item = [&](int e) -> int2 { return fast[e]; };
+  auto shifts = [&](int e) -> int2 { return shf[e]; };
   auto var_of = [&](int e) -> int {
-    if constexpr (SCALAR)
-      return itab[10 * e + 1];
-    else
-      return (int)((unsigned)shf[e].y >> 30);
+    return (int)((unsigned)shf[e].y >> 30);
   };
   const int lane = threadIdx.x % LANES;
   for (int r = 0; r < per; ++r) {

@@ -25,6 +25,9 @@ from functools import partial
 
 import numpy as np
 
+# Importing the reference must not write bytecode next to its sources.
+sys.dont_write_bytecode = True
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
