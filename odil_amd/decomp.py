@@ -98,7 +98,39 @@ def interp_cell_t(g):
 
 class MultigridDecomp:
     """Static methods with the reference's names and arguments (the `mod`
-    and `method` arguments do not apply: one GPU implementation)."""
+    and `method` arguments do not apply: one GPU implementation).
+
+    `check_nnw` and `field_to_weights` are called by the reference's
+    `Domain` (util_op.py:77,303) but not defined in its `MultigridDecomp`;
+    here `check_nnw` raises ValueError unless every level refines to the
+    next finer one (n -> 2n - 1 for nodes, n -> 2n for cells), and
+    `field_to_weights` returns the weights that put the field on the finest
+    level (divided by its factor) and zeros on the coarser ones, the exact
+    right inverse of `weights_to_field`."""
+
+    @staticmethod
+    def check_nnw(nnw, use_axes=None, cell=False):
+        if use_axes is not None and not all(use_axes):
+            raise ValueError('decomposition along a subset of the axes is '
+                             'not provided')
+        for a, b in zip(nnw[:-1], nnw[1:]):
+            want = tuple(2 * x if cell else 2 * x - 1 for x in b)
+            if len(a) != len(b) or tuple(a) != want:
+                raise ValueError('level {} does not refine to {} ({} grids '
+                                 'need {})'.format(tuple(b), tuple(a),
+                                                   'cell' if cell else
+                                                   'node', want))
+
+    @staticmethod
+    def field_to_weights(u, nnw, mod=None, factors=None, cell=False):
+        f0 = float(factors[0]) if factors else 1.
+        if tuple(u.shape) != tuple(nnw[0]) and u.numel() != int(
+                np.prod(nnw[0])):
+            raise ValueError('field of shape {} for the finest level {}'
+                             .format(tuple(u.shape), tuple(nnw[0])))
+        rest = sum(int(np.prod(nw)) for nw in nnw[1:])
+        return torch.cat([u.reshape(-1) / f0,
+                          torch.zeros(rest, dtype=u.dtype, device=u.device)])
 
     @staticmethod
     def interp_node_field(u, depth=1):

@@ -400,6 +400,190 @@ class BlockStencil:
                             nvar_out=nvar_out)
 
 
+
+def _torch_dtype(dtype):
+    """torch dtype for a torch, NumPy or string dtype (None: FP64)."""
+    if dtype is None:
+        return torch.float64
+    if isinstance(dtype, torch.dtype):
+        return dtype
+    name = np.dtype(dtype).name
+    if name == 'float64':
+        return torch.float64
+    if name == 'float32':
+        return torch.float32
+    raise ValueError('unsupported dtype {}'.format(dtype))
+
+
+class SparseOperator:
+    """The reference's `SparseOperator` (multigrid.py:877-1214) on the GPU:
+    A = sum_s diag(c_s) Shift_s over one field, zero outside the grid
+    (non-circular, `apply_shift`, multigrid.py:941-947), optional output
+    stride. `shift_to_field` maps shift tuples to coefficient fields (device
+    tensors of the output grid, any layout of that size). Backed by a
+    single-block `BlockStencil`: `mul_field` / `mul_transpose_field` are the
+    stencil apply kernels, `mul_transpose_op` the stored J^T J kernel on the
+    two-field block [A B] (its (0, 1) block is A^T B), `add_elementwise`
+    sums coefficient fields on the device. Vectors are returned flat like the
+    reference's. `mod` is accepted for the reference's signature and unused.
+    """
+
+    def __init__(self, shift_to_field, nw, stride=1, mod=None, dtype=None,
+                 device=None):
+        self.mod = mod
+        self.nw = np.array([int(n) for n in nw])
+        self.stride = int(stride)
+        self.dim = len(self.nw)
+        self.nwh = (self.nw - 1) // self.stride + 1
+        self._tdtype = _torch_dtype(dtype)
+        self.dtype = dtype if dtype is not None else self._tdtype
+        nwh = tuple(int(n) for n in self.nwh)
+        keys, vals = [], []
+        for dw, c in dict(shift_to_field).items():
+            dw = tuple(int(x) for x in np.atleast_1d(dw))
+            if len(dw) != self.dim:
+                raise ValueError('shift {} has not {} axes'.format(
+                    dw, self.dim))
+            c = torch.as_tensor(c, dtype=self._tdtype, device=device)
+            if c.numel() != int(np.prod(nwh)):
+                raise ValueError('coefficient of {} has {} values, the '
+                                 'output grid {}'.format(dw, c.numel(), nwh))
+            keys.append(dw)
+            vals.append(c.reshape(nwh))
+        self.device = vals[0].device if vals else (
+            torch.device(device) if device is not None else
+            torch.device('cuda'))
+        self.shift_to_field = dict(zip(keys, vals))
+        if vals:
+            coef = torch.stack(vals).contiguous()
+        else:
+            coef = torch.zeros((0, ) + nwh, dtype=self._tdtype,
+                               device=self.device)
+        self._op = BlockStencil(coef, [(0, 0, k) for k in keys],
+                                tuple(int(n) for n in self.nw),
+                                stride=self.stride, nvar_in=1, nvar_out=1) \
+            if keys else None
+
+    def _vec(self, u, shape):
+        u = torch.as_tensor(u, dtype=self._tdtype, device=self.device)
+        if u.numel() != int(np.prod(shape)):
+            raise ValueError('vector of {} values for a grid {}'.format(
+                u.numel(), tuple(shape)))
+        return u.contiguous().reshape(tuple(int(n) for n in shape))
+
+    def mul_field(self, u):
+        """A u, flat (multigrid.py:958-969)."""
+        u = self._vec(u, self.nw)
+        if self._op is None:
+            return torch.zeros(int(np.prod(self.nwh)), dtype=self._tdtype,
+                               device=self.device)
+        return self._op.apply([u])[0].reshape(-1)
+
+    def mul_transpose_field(self, uh):
+        """A^T uh, flat (multigrid.py:1006-1033)."""
+        uh = self._vec(uh, self.nwh)
+        if self._op is None:
+            return torch.zeros(int(np.prod(self.nw)), dtype=self._tdtype,
+                               device=self.device)
+        return self._op.apply_t([uh])[0].reshape(-1)
+
+    def mul_transpose_op(self, op):
+        """A^T B as a SparseOperator, coefficient of shift sb - sa:
+        sum c^A_sa(x - sa) c^B_sb(x - sa) (multigrid.py:1035-1071)."""
+        if self.stride != 1 or op.stride != 1:
+            raise ValueError('mul_transpose_op needs stride 1')
+        if tuple(self.nw) != tuple(op.nw):
+            raise ValueError('grids differ')
+        if self._op is None or op._op is None:
+            return SparseOperator({}, self.nw, mod=self.mod,
+                                  dtype=self.dtype, device=self.device)
+        same = op is self
+        ents = [(0, 0, k) for k in self.shift_to_field]
+        coefs = list(self.shift_to_field.values())
+        if not same:
+            ents += [(0, 1, k) for k in op.shift_to_field]
+            coefs += list(op.shift_to_field.values())
+        J = BlockStencil(torch.stack(coefs).contiguous(), ents,
+                         tuple(int(n) for n in self.nw), nvar_in=1 + (
+                             not same), nvar_out=1)
+        N = normal_operator(J, torch.float64)
+        want = 0 if same else 1
+        out = {}
+        for q, (o, i, sh) in enumerate(N.entries):
+            if o == 0 and i == want:
+                out[sh] = N.coef[q].to(self._tdtype)
+        return SparseOperator(out, self.nw, mod=self.mod, dtype=self.dtype,
+                              device=self.device)
+
+    def mul_self_transpose(self):
+        return self.mul_transpose_op(self)
+
+    def add_elementwise(self, op):
+        """A + B (union of the shifts, multigrid.py:1138-1184)."""
+        if tuple(self.nw) != tuple(op.nw) or self.stride != op.stride:
+            raise ValueError('operators on different grids')
+        out = {k: v.clone() for k, v in self.shift_to_field.items()}
+        for k, v in op.shift_to_field.items():
+            if k in out:
+                out[k] += v
+            else:
+                out[k] = v.clone()
+        return SparseOperator(out, self.nw, stride=self.stride,
+                              mod=self.mod, dtype=self.dtype,
+                              device=self.device)
+
+    def eliminate_zeros(self):
+        """Drops shifts whose coefficient field is zero."""
+        keep = {k: v for k, v in self.shift_to_field.items()
+                if bool((v != 0).any())}
+        self.__init__(keep, self.nw, stride=self.stride, mod=self.mod,
+                      dtype=self.dtype, device=self.device)
+
+    def tocsr(self):
+        """SciPy CSR copy (multigrid.py:906-920), explicit zeros dropped."""
+        if self._op is None:
+            import scipy.sparse
+            return scipy.sparse.csr_matrix(
+                (int(np.prod(self.nwh)), int(np.prod(self.nw))))
+        m = self._op.tocsr()
+        m.eliminate_zeros()
+        return m
+
+    @staticmethod
+    def eye(nw, stride=1, mod=None, dtype=None, device=None):
+        nwh = tuple((int(n) - 1) // stride + 1 for n in nw)
+        return SparseOperator(
+            {(0, ) * len(nw): torch.ones(nwh, dtype=_torch_dtype(dtype),
+                                        device=device)},
+            nw, stride=stride, mod=mod, dtype=dtype, device=device)
+
+    @staticmethod
+    def diag(c, nw, stride=1, mod=None):
+        c = torch.as_tensor(c)
+        return SparseOperator({(0, ) * len(nw): c}, nw, stride=stride,
+                              mod=mod, dtype=c.dtype, device=c.device)
+
+
+def blocks_to_stencil(A, nw, dtype=None):
+    """nvar x nvar list of SparseOperators (the reference's block matrix of
+    `MultigridOp.update_A`, util.py:577-586) as one BlockStencil with
+    entries (i, j, shift) for block A[i][j]."""
+    nvar = len(A)
+    ents, coefs = [], []
+    for i in range(nvar):
+        if len(A[i]) != nvar:
+            raise ValueError('update_A expects an nvar x nvar block list')
+        for j in range(nvar):
+            for k, c in A[i][j].shift_to_field.items():
+                ents.append((i, j, tuple(k)))
+                coefs.append(c)
+    if not coefs:
+        raise ValueError('empty operator')
+    coef = torch.stack([c.to(dtype or coefs[0].dtype) for c in coefs])
+    return BlockStencil(coef.contiguous(), ents, tuple(int(n) for n in nw),
+                        nvar_in=nvar, nvar_out=nvar)
+
+
 _PLANS = {}
 
 
@@ -646,9 +830,14 @@ class MultigridOp:
     constant pressure, 0 otherwise.
     """
 
-    def __init__(self, nw, nvar=1, nlevels=None, restriction='full',
-                 dtype=torch.float64, device=None, coarse_reg=0.,
+    def __init__(self, nw, nvar=1, nlevels=None, mod=None, dtype=None,
+                 restriction='full', device=None, coarse_reg=0.,
                  galerkin_mode='reference'):
+        # Positional order of the reference (nw, nvar, nlevels, mod, dtype,
+        # restriction); `mod` is accepted and unused, `dtype` may be a NumPy
+        # or torch dtype (default FP64).
+        self.mod = mod
+        dtype = _torch_dtype(dtype)
         self.nnw = level_sizes(nw, nlevels)
         self.nlevels = len(self.nnw)
         self._spd = {}
@@ -741,12 +930,16 @@ class MultigridOp:
         return None if self.omega_cap is None else tuple(self.omega_cap)
 
     # -- setup -----------------------------------------------------------
-    def update_A(self, A0):
-        """A0: BlockStencil of the finest operator (cast to self.dtype).
+    def update_A(self, A0, set_lower=False):
+        """A0: BlockStencil of the finest operator (cast to self.dtype), or
+        the reference's nvar x nvar list of SparseOperators (util.py:
+        577-586).
 
         A second call with an operator of the same structure refills the
         level operators and coarse inverses in place, so device addresses
         stay fixed (a captured V-cycle graph stays valid)."""
+        if isinstance(A0, (list, tuple)):
+            A0 = blocks_to_stencil(A0, self.nnw[0], self.dtype)
         if A0.nw != self.nnw[0] or A0.nvar_in != self.nvar:
             raise ValueError('operator grid/fields do not match the levels')
         same = (self.AA is not None and self.AA[0].entries == A0.entries
@@ -828,46 +1021,77 @@ class MultigridOp:
         return self.AA[level]
 
     # -- reference-shaped operations -------------------------------------
+    def _lvl(self, v, level):
+        """Views of a list of level vectors (flat or shaped) as fields."""
+        shp = tuple(self.nnw[level])
+        return [x.reshape(shp) for x in v]
+
     def residual(self, A, u, f):
-        return A.residual(u, f, [torch.empty_like(x) for x in f])
+        """r = f - A u (multigrid.py:536-541); A is a level's BlockStencil
+        or the reference's block list. Returns vectors shaped like f."""
+        if isinstance(A, (list, tuple)):
+            A = blocks_to_stencil(A, tuple(A[0][0].nw), self.dtype)
+        level = self.nnw.index(tuple(A.nw))
+        r = A.residual(self._lvl(u, level), self._lvl(f, level),
+                       [torch.empty(tuple(self.nnw[level]), dtype=A.dtype,
+                                    device=self.device) for _ in f])
+        return [a.reshape(b.shape) for a, b in zip(r, f)]
 
     def smoother_jacobi(self, level, u, f, omega=1.):
-        """In-place weighted Jacobi (u is modified and returned)."""
-        tmp = [torch.empty_like(x) for x in u]
-        self._op(level).jacobi(u, f, tmp, omega)
-        for a, b in zip(u, tmp):
+        """In-place weighted Jacobi u += omega (f - A u) / D
+        (multigrid.py:543-552): u is modified and returned."""
+        uu = self._lvl(u, level)
+        tmp = [torch.empty_like(x) for x in uu]
+        self._op(level).jacobi(uu, self._lvl(f, level), tmp, omega)
+        for a, b in zip(uu, tmp):
             a.copy_(b)
         return u
 
     def smoother_direct(self, level, u, f):
+        """Direct solve on `level` (multigrid.py:554-565): the dense
+        solution operator applied by the GEMV kernel. Returns new vectors
+        shaped like f."""
         Minv = self._coarse(level)
-        out = [torch.empty_like(x) for x in f]
+        ff = [x.contiguous() for x in self._lvl(f, level)]
+        out = [torch.empty_like(x) for x in ff]
         N = int(np.prod(self.nnw[level]))
-        dt = _lib.dtype_tag(f[0].dtype)
+        dt = _lib.dtype_tag(ff[0].dtype)
         st = _lib.lib().odil_dense_gemv(c_int(dt), c_int(dt),
                                         c_int(self.nvar), c_int(N), ptr(Minv),
-                                        ptrs(f), ptrs(out), stream())
+                                        ptrs(ff), ptrs(out), stream())
         check(st, 'dense_gemv')
-        return out
+        return [a.reshape(b.shape) for a, b in zip(out, f)]
 
-    def step(self, u, f, omega=0.6, level=0, ndirect=5, pre=2, post=2):
-        """One V-cycle. Like the reference (multigrid.py:589-593) the
-        recursive call does not pass `ndirect`, `pre`, `post`: coarser levels
-        use 5, 2, 2."""
+    def step(self, u, f, smoother=None, level=0, ndirect=5, pre=2, post=2,
+             omega=None):
+        """One V-cycle (multigrid.py:567-602). `smoother(level, u, f)`
+        returns the smoothed u, e.g. partial(mg.smoother_jacobi, omega=w)
+        as util.py:588-595 passes it; `omega=` alone builds that smoother.
+        Like the reference, the recursive call does not pass `ndirect`,
+        `pre`, `post`: coarser levels use 5, 2, 2. u and f are lists of
+        level vectors (flat or shaped); the result has their shape."""
+        if smoother is None:
+            w = 0.6 if omega is None else omega
+            smoother = lambda lv, a, b: self.smoother_jacobi(lv, a, b, w)
+        elif not callable(smoother):
+            raise ValueError('smoother must be callable(level, u, f)')
         if level + 1 >= self.nlevels or min(self.nnw[level]) <= ndirect:
             return self.smoother_direct(level, u, f)
         for _ in range(pre):
-            u = self.smoother_jacobi(level, u, f, omega)
+            u = smoother(level, u, f)
         A = self._op(level)
-        r = A.residual(u, f, [torch.empty_like(x) for x in f])
+        r = A.residual(self._lvl(u, level), self._lvl(f, level),
+                       [torch.empty(tuple(self.nnw[level]), dtype=self.dtype,
+                                    device=self.device) for _ in f])
         rc = restrict(r, self.nnw[level], self.rw,
                       restriction=self.restriction)
-        du = self.step([torch.zeros_like(x) for x in rc], rc, omega,
+        du = self.step([torch.zeros_like(x) for x in rc], rc, smoother,
                        level=level + 1)
         u = [x.clone() for x in u]
-        prolong_add(du, u, self.nnw[level])
+        prolong_add(self._lvl(du, level + 1), self._lvl(u, level),
+                    self.nnw[level])
         for _ in range(post):
-            u = self.smoother_jacobi(level, u, f, omega)
+            u = smoother(level, u, f)
         return u
 
     # -- allocation-free V-cycle on preallocated buffers ----------------
