@@ -87,14 +87,79 @@ struct OutPtrs {
   T* f[NF];
 };
 
-// Tiles of the residual (64 x 4 threads, kResRows cells each) and of the fused
-// loss-adjoint kernels (cells along axis 0 per precision x 32 along axis 1),
-// and the partial-sum counts every operator's *_nblocks reports: one partial
-// per wavefront (4 per 256-thread workgroup).
-constexpr int kResRows = 1;  // cells per thread along axis 0
-constexpr int kResTy = 64, kResTx = 4 * kResRows;
+// Accessor of a field tile staged in LDS: `t` points at cell (0, 0) of
+// field 0 in a [NF][E0][E1] tile, so neighbour (di, dj) of field k is one
+// ds_read at a compile-time offset from the cell's address.
+template <class T, int E0, int E1>
+struct LdsAcc {
+  const T* t;
+  __device__ inline T operator()(int k, int di, int dj) const {
+    return t[k * (E0 * E1) + di * E1 + dj];
+  }
+};
+
+// Stages fields[k] over rows [x0 - H, x0 + TX + H) x cols [y0 - H, y0 + TY +
+// H) into `tile` ([NF][E0][E1]), indices clamped into the grid exactly as
+// the clamping global accessor reads them (wall rules override the clamped
+// values where they apply). Consecutive threads read consecutive columns.
+// All of a thread's loads are issued before any LDS write, so they are in
+// flight together (a load-then-store loop would wait for each in turn);
+// blocks whose grown tile lies inside the grid skip the clamps.
+template <class T, int NF, int TX, int TY, int H>
+__device__ inline void stage_tile(const FieldPtrs<T, NF>& in, T* tile,
+                                  int x0, int y0, int n0, int n1) {
+  constexpr int E0 = TX + 2 * H, E1 = TY + 2 * H;
+  constexpr int PER = E0 * E1;
+  constexpr int NL = (PER + 255) / 256;
+  const bool inside = x0 - H >= 0 && x0 + TX + H <= n0 && y0 - H >= 0 &&
+                      y0 + TY + H <= n1;
+  T v[NF][NL];
+  if (inside) {
+    const T* base[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k)
+      base[k] = in.f[k] + (size_t)(x0 - H) * n1 + (y0 - H);
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      const int idx = threadIdx.x + 256 * r;
+      if (r + 1 < NL || idx < PER) {
+        const int a = idx / E1, b = idx - a * E1;
+        const int off = a * n1 + b;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) v[k][r] = base[k][off];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < NL; ++r) {
+      const int idx = threadIdx.x + 256 * r;
+      if (r + 1 < NL || idx < PER) {
+        const int a = idx / E1, b = idx - a * E1;
+        const int gi = min(max(x0 - H + a, 0), n0 - 1);
+        const int gj = min(max(y0 - H + b, 0), n1 - 1);
+        const size_t off = (size_t)gi * n1 + gj;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) v[k][r] = in.f[k][off];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NL; ++r) {
+    const int idx = threadIdx.x + 256 * r;
+    if (r + 1 < NL || idx < PER) {
+#pragma unroll
+      for (int k = 0; k < NF; ++k) tile[k * PER + idx] = v[k][r];
+    }
+  }
+}
+
+// Tiles of the residual (TX rows x TY columns per 256-thread workgroup) and
+// of the fused loss-adjoint kernels (cells along axis 0 per precision x 32
+// along axis 1), and the partial-sum counts every operator's *_nblocks
+// reports: one partial per wavefront (4 per 256-thread workgroup).
 // (On the 1025^2 cavity 16 x 32 (FP32) and 8 x 32 (FP64) were the fastest
-// of 4..32 x 32..128.)
+// loss-adjoint tiles of 4..32 x 32..128.)
+constexpr int kResTx = 16, kResTy = 64;
 constexpr int kGradTx32 = 16, kGradTx64 = 8, kGradTy = 32;
 
 constexpr int kWavesPerBlock = 4;
@@ -106,33 +171,49 @@ inline int grad_nblocks(bool fp32, int n0, int n1) {
          kWavesPerBlock;
 }
 
-// Residual f_e at every cell, and per-wavefront partial sums of f_e^2.
+template <class T, class Op>
+constexpr size_t residual_smem() {
+  return sizeof(T) * (size_t)Op::NF * (kResTx + 2 * Op::R) *
+         (kResTy + 2 * Op::R);
+}
+
+// Residual f_e at every cell, and per-wavefront partial sums of f_e^2. The
+// workgroup's TX x TY tile of the fields, grown by the operator reach R, is
+// staged in LDS with one coalesced read per value; every cell then reads its
+// stencil from LDS. A wavefront covers one row of 64 cells.
 template <class T, class Op>
 __global__ void __launch_bounds__(256)
     k_residual(FieldPtrs<T, Op::NF> in, OutPtrs<T, Op::NE> out,
                double* __restrict__ partial, int n0, int n1,
                typename Op::Params P) {
+  constexpr int R = Op::R;
+  constexpr int E0 = kResTx + 2 * R, E1 = kResTy + 2 * R;
+  constexpr int NPT = kResTx * kResTy / 256;
+  static_assert(kResTy == 64, "one wavefront per tile row");
+  extern __shared__ unsigned char smem_raw[];
+  T* tile = reinterpret_cast<T*>(smem_raw);
   unsigned bx, by;
   xcd_block2(bx, by);
-  const int j = bx * kResTy + (threadIdx.x & 63);
-  const int i0 = by * kResTx + (threadIdx.x >> 6);
+  const int x0 = by * kResTx, y0 = bx * kResTy;
+  stage_tile<T, Op::NF, kResTx, kResTy, R>(in, tile, x0, y0, n0, n1);
   double ssq[Op::NE];
   for (int e = 0; e < Op::NE; ++e) ssq[e] = 0;
-  const bool inner = interior_box(by * kResTx, by * kResTx + kResTx,
-                                  bx * kResTy, bx * kResTy + kResTy, n0, n1,
-                                  Op::R);
+  const bool inner = interior_box(x0, x0 + kResTx, y0, y0 + kResTy, n0, n1,
+                                  R);
+  __syncthreads();
+  const int b = threadIdx.x & 63;
+  const int j = y0 + b;
 #pragma unroll
-  for (int r = 0; r < kResRows; ++r) {
-    const int i = i0 + 4 * r;
+  for (int r = 0; r < NPT; ++r) {
+    const int a = (threadIdx.x >> 6) + 4 * r;
+    const int i = x0 + a;
     if (i >= n0 || j >= n1) continue;
+    LdsAcc<T, E0, E1> acc{tile + (a + R) * E1 + (b + R)};
     T f[Op::NE];
-    if (inner) {
-      GlobalAcc<T, Op::NF, false> acc{in.f, n0, n1, i, j};
+    if (inner)
       Op::template residual<true>(acc, i, j, n0, n1, P, f);
-    } else {
-      GlobalAcc<T, Op::NF> acc{in.f, n0, n1, i, j};
+    else
       Op::template residual<false>(acc, i, j, n0, n1, P, f);
-    }
     const size_t c = (size_t)i * n1 + j;
     for (int e = 0; e < Op::NE; ++e) {
       if (out.f[e]) out.f[e][c] = f[e];
@@ -196,29 +277,27 @@ __device__ inline void ring_cell(int q, int& ea, int& eb) {
   }
 }
 
-// Evaluate residual and Jacobian of cell (gi, gj) reading the state from
-// global memory (neighbour reuse through L1/L2); INT blocks skip clamping
-// and wall rules.
-template <class T, class Op>
-__device__ inline void eval_global(const FieldPtrs<T, Op::NF>& in, int gi,
-                                   int gj, int n0, int n1, bool inner,
-                                   const typename Op::Params& P, T* f,
-                                   T (*J)[Op::NF][Op::NS]) {
-  if (inner) {
-    GlobalAcc<T, Op::NF, false> acc{in.f, n0, n1, gi, gj};
+// Residual and Jacobian of one cell from a state tile staged in LDS
+// ([NF][S0][S1], see stage_tile); INT blocks skip wall rules:
+// `cell` points at the cell's own position in field 0 of the tile.
+template <class T, class Op, int S0, int S1>
+__device__ inline void eval_lds(const T* cell, int gi, int gj, int n0, int n1,
+                                bool inner, const typename Op::Params& P,
+                                T* f, T (*J)[Op::NF][Op::NS]) {
+  LdsAcc<T, S0, S1> acc{cell};
+  if (inner)
     Op::template eval<true>(acc, gi, gj, n0, n1, P, f, J);
-  } else {
-    GlobalAcc<T, Op::NF> acc{in.f, n0, n1, gi, gj};
+  else
     Op::template eval<false>(acc, gi, gj, n0, n1, P, f, J);
-  }
 }
 
 // Fused loss + adjoint of one TX x TY tile of output cells at (x0, y0),
 // computed by the whole 256-thread workgroup:
-//  1. evaluate f and J on the tile grown by RJ (RJ = Jacobian reach),
-//     reading the state from global memory; every off-centre product
-//     J[e][k][s] f_e goes to LDS `ot`, the centre product of inner cells
-//     stays in registers;
+//  0. stage the state over the tile grown by RJ + R in LDS (after the
+//     products buffer `ot`);
+//  1. evaluate f and J on the tile grown by RJ (RJ = Jacobian reach) from
+//     that tile; every off-centre product J[e][k][s] f_e goes to LDS `ot`,
+//     the centre product of inner cells stays in registers;
 //  2. gather grad_k(c) = scale * sum_s out_{k,s}(c - shift_s).
 // Adds this thread's f_e^2 to ssq[e]. Starts with a barrier so a workgroup
 // may run it on several tiles in turn.
@@ -230,6 +309,7 @@ __device__ inline void loss_grad_tile(const FieldPtrs<T, Op::NF>& in,
                                       double* ssq) {
   constexpr int NF = Op::NF, NE = Op::NE, NS = Op::NS, RJ = Op::RJ;
   constexpr int E0 = TX + 2 * RJ, E1 = TY + 2 * RJ;
+  constexpr int H = RJ + Op::R, S0 = TX + 2 * H, S1 = TY + 2 * H;
   constexpr int NT = 256;
   constexpr int NPT = (TX * TY) / NT;  // inner cells per thread
   static_assert((TX * TY) % NT == 0, "tile must be a multiple of 256");
@@ -239,7 +319,10 @@ __device__ inline void loss_grad_tile(const FieldPtrs<T, Op::NF>& in,
   auto OT = [&](int k, int s, int ea, int eb) -> T& {
     return ot[((k * (NS - 1) + s - 1) * E0 + ea) * E1 + eb];
   };
-  __syncthreads();  // previous tile's gathers are done with `ot`
+  T* st = ot + NF * (NS - 1) * E0 * E1;  // state tile [NF][S0][S1]
+  __syncthreads();  // previous tile's gathers are done with `ot`, `st`
+  stage_tile<T, NF, TX, TY, H>(in, st, x0, y0, n0, n1);
+  __syncthreads();
 
   T cen[NPT][NF];
   for (int p = 0; p < NPT; ++p) {
@@ -251,7 +334,8 @@ __device__ inline void loss_grad_tile(const FieldPtrs<T, Op::NF>& in,
     if (gi < n0 && gj < n1) {
       T f[NE];
       T J[NE][NF][NS];
-      eval_global<T, Op>(in, gi, gj, n0, n1, inner, P, f, J);
+      eval_lds<T, Op, S0, S1>(st + (a + H) * S1 + (b + H), gi, gj, n0, n1,
+                              inner, P, f, J);
       for (int e = 0; e < NE; ++e) ssq[e] += (double)f[e] * (double)f[e];
 #pragma unroll
       for (int k = 0; k < NF; ++k)
@@ -278,7 +362,8 @@ __device__ inline void loss_grad_tile(const FieldPtrs<T, Op::NF>& in,
     if (gi >= 0 && gi < n0 && gj >= 0 && gj < n1) {
       T f[NE];
       T J[NE][NF][NS];
-      eval_global<T, Op>(in, gi, gj, n0, n1, inner, P, f, J);
+      eval_lds<T, Op, S0, S1>(st + (ea + Op::R) * S1 + (eb + Op::R), gi, gj,
+                              n0, n1, inner, P, f, J);
 #pragma unroll
       for (int k = 0; k < NF; ++k)
 #pragma unroll
@@ -341,8 +426,10 @@ __global__ void __launch_bounds__(256)
 
 template <class T, class Op, int TX, int TY>
 constexpr size_t loss_grad_smem() {
+  constexpr int H = Op::RJ + Op::R;
   return sizeof(T) * (size_t)(Op::NF * (Op::NS - 1) * (TX + 2 * Op::RJ) *
-                              (TY + 2 * Op::RJ));
+                                  (TY + 2 * Op::RJ) +
+                              Op::NF * (TX + 2 * H) * (TY + 2 * H));
 }
 
 // Fused matrix-free normal operator N = J^T J with the Jacobian recomputed
@@ -350,8 +437,8 @@ constexpr size_t loss_grad_smem() {
 //   mode 0: y = N x          mode 1: y = b - N x
 //   mode 2: y = x + omega (b - N x) / d   (weighted Jacobi, d = diag N)
 //   mode 3: y = diag(N) = sum_e sum_s J[e][k][s](c - shift_s)^2
-// Same tile scheme as k_loss_grad with f replaced by v = J x; state and x
-// are read from global memory.
+// Same tile scheme as k_loss_grad with f replaced by v = J x; the state
+// (halo RJ + R) and x (halo 2 RJ) are staged in LDS after the products.
 template <class T, class Op, int TX, int TY>
 __global__ void __launch_bounds__(256)
     k_normal_apply(FieldPtrs<T, Op::NF> st_in, FieldPtrs<T, Op::NF> x_in,
@@ -363,8 +450,12 @@ __global__ void __launch_bounds__(256)
   constexpr int NT = 256;
   constexpr int NPT = (TX * TY) / NT;
   static_assert((TX * TY) % NT == 0, "tile must be a multiple of 256");
+  constexpr int H = RJ + Op::R, S0 = TX + 2 * H, S1 = TY + 2 * H;
+  constexpr int X0 = TX + 4 * RJ, X1 = TY + 4 * RJ;
   extern __shared__ unsigned char smem_raw[];
   T* ot = reinterpret_cast<T*>(smem_raw);  // [NF][NS-1][E0][E1]
+  T* st = ot + NF * (NS - 1) * E0 * E1;    // state [NF][S0][S1]
+  T* xt = st + NF * S0 * S1;               // x [NF][X0][X1]
 
   unsigned bx, by;
   xcd_block2(bx, by);
@@ -377,20 +468,20 @@ __global__ void __launch_bounds__(256)
   auto OT = [&](int k, int s, int ea, int eb) -> T& {
     return ot[((k * (NS - 1) + s - 1) * E0 + ea) * E1 + eb];
   };
-  auto xat = [&](int k, int i, int j) -> T {
-    if (!inner) {  // coefficients reaching outside are zero; clamp reads
-      i = min(max(i, 0), n0 - 1);
-      j = min(max(j, 0), n1 - 1);
-    }
-    return x_in.f[k][(size_t)i * n1 + j];
-  };
+  // Coefficients reaching outside the grid are zero, so the clamped values
+  // the staging stores there are never weighted.
+  stage_tile<T, NF, TX, TY, H>(st_in, st, x0, y0, n0, n1);
+  if (!diag) stage_tile<T, NF, TX, TY, 2 * RJ>(x_in, xt, x0, y0, n0, n1);
+  __syncthreads();
 
-  // Products of one cell: o[k][s] = sum_e J[e][k][s] v_e with v = J x (or
-  // J[e][k][s]^2 summed over e for the diagonal).
-  auto cell = [&](int gi, int gj, T (*o)[NS]) {
+  // Products of the cell at grown-tile position (ea, eb):
+  // o[k][s] = sum_e J[e][k][s] v_e with v = J x (or J[e][k][s]^2 summed
+  // over e for the diagonal).
+  auto cell = [&](int ea, int eb, int gi, int gj, T (*o)[NS]) {
     T f[NE];
     T J[NE][NF][NS];
-    eval_global<T, Op>(st_in, gi, gj, n0, n1, inner, P, f, J);
+    eval_lds<T, Op, S0, S1>(st + (ea + Op::R) * S1 + (eb + Op::R), gi, gj,
+                            n0, n1, inner, P, f, J);
     if (diag) {
 #pragma unroll
       for (int k = 0; k < NF; ++k)
@@ -404,11 +495,12 @@ __global__ void __launch_bounds__(256)
       return;
     }
     T xv[NF][NS];
+    const T* xc = xt + (ea + RJ) * X1 + (eb + RJ);
 #pragma unroll
     for (int k = 0; k < NF; ++k)
 #pragma unroll
       for (int s = 0; s < NS; ++s)
-        xv[k][s] = xat(k, gi + Op::SX[s], gj + Op::SY[s]);
+        xv[k][s] = xc[k * (X0 * X1) + Op::SX[s] * X1 + Op::SY[s]];
     T v[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
@@ -438,7 +530,7 @@ __global__ void __launch_bounds__(256)
     const int ea = a + RJ, eb = b + RJ;
     T o[NF][NS];
     if (gi < n0 && gj < n1) {
-      cell(gi, gj, o);
+      cell(ea, eb, gi, gj, o);
     } else {
       for (int k = 0; k < NF; ++k)
         for (int s = 0; s < NS; ++s) o[k][s] = 0;
@@ -455,7 +547,7 @@ __global__ void __launch_bounds__(256)
     const int gi = x0 - RJ + ea, gj = y0 - RJ + eb;
     T o[NF][NS];
     if (gi >= 0 && gi < n0 && gj >= 0 && gj < n1) {
-      cell(gi, gj, o);
+      cell(ea, eb, gi, gj, o);
     } else {
       for (int k = 0; k < NF; ++k)
         for (int s = 0; s < NS; ++s) o[k][s] = 0;
@@ -483,7 +575,8 @@ __global__ void __launch_bounds__(256)
       else if (mode == 1)
         out = b_in.f[k][c] - g;
       else
-        out = x_in.f[k][c] + omega * (b_in.f[k][c] - g) / d_in.f[k][c];
+        out = xt[k * (X0 * X1) + (a + 2 * RJ) * X1 + (b + 2 * RJ)] +
+              omega * (b_in.f[k][c] - g) / d_in.f[k][c];
       y.f[k][c] = out;
     }
   }
@@ -491,8 +584,12 @@ __global__ void __launch_bounds__(256)
 
 template <class T, class Op, int TX, int TY>
 constexpr size_t normal_apply_smem() {
+  constexpr int H = Op::RJ + Op::R;
   return sizeof(T) * (size_t)(Op::NF * (Op::NS - 1) * (TX + 2 * Op::RJ) *
-                              (TY + 2 * Op::RJ));
+                                  (TY + 2 * Op::RJ) +
+                              Op::NF * (TX + 2 * H) * (TY + 2 * H) +
+                              Op::NF * (TX + 4 * Op::RJ) *
+                                  (TY + 4 * Op::RJ));
 }
 
 // Deterministic final reduction of per-block partials: out[e] = sum_b
@@ -513,8 +610,10 @@ int launch_residual(const void* const* fields, void* const* res,
     out.f[e] = res ? static_cast<T*>(res[e]) : nullptr;
   dim3 grid(ceil_div(n1, kResTy), ceil_div(n0, kResTx));
   if (nblocks) *nblocks = grid.x * grid.y * kWavesPerBlock;
-  hipLaunchKernelGGL((k_residual<T, Op>), grid, dim3(256), 0, stream, in, out,
-                     partial, n0, n1, P);
+  constexpr size_t smem = residual_smem<T, Op>();
+  static_assert(smem <= 64 * 1024, "residual tile too large");
+  hipLaunchKernelGGL((k_residual<T, Op>), grid, dim3(256), smem, stream, in,
+                     out, partial, n0, n1, P);
   return check_launch("residual");
 }
 
