@@ -286,6 +286,39 @@ int odil_advection_loss_grad(int dtype, int nt, int nx, int ny, double dt,
                              const void* p_init, const void* p, void* grad,
                              double* partial, void* stream);
 
+/* Inverse advection problem (advection_2d.py:103-250, problem 'inverse',
+ * central 0, implicit 0) on an nt x nx x ny space-time grid (nx, ny >= 3):
+ * unknowns u, v, p (fields[3]); equations in the reference's order
+ * fe = (p - p_exact) alpha m(t) (alpha already annealed by the caller,
+ * get_anneal_factor :25-26; m = 1, or with onlyfinal 1 only at t = 0 and
+ * nt - 1), fp = (p - p_prev)/dt + u Dx p + v Dy p (upwind on the sign of the
+ * frozen velocity, p_prev = p_exact at t = 0), then kreg lap_xy(u),
+ * kreg lap_xy(v) if kreg != 0, ktreg (u - u(t-1))/dt and the v form if
+ * ktreg != 0 (0 at t = 0), divfree div_c(u, v) if divfree != 0. x, y
+ * periodic (periodic != 0) or zero outside. p_exact [nt][nx][ny] in the
+ * field dtype. *_neq gives the equation count, *_nblocks the partials per
+ * equation (partial [neq][nblocks]); res may be NULL (sums only) and its
+ * entries NULL. loss_grad writes grad[3] = (2/N) sum_e J_e^T f_e (the
+ * reference's tape gradient: the upwind direction carries no derivative). */
+int odil_advection_inverse_neq(double kreg, double ktreg, double divfree);
+int odil_advection_inverse_nblocks(int nt, int nx, int ny);
+int odil_advection_inverse_residual(int dtype, int nt, int nx, int ny,
+                                    double dt, double dx, double dy,
+                                    double alpha, int onlyfinal, double kreg,
+                                    double ktreg, double divfree,
+                                    int periodic, const void* p_exact,
+                                    const void* const* fields,
+                                    void* const* res, double* partial,
+                                    void* stream);
+int odil_advection_inverse_loss_grad(int dtype, int nt, int nx, int ny,
+                                     double dt, double dx, double dy,
+                                     double alpha, int onlyfinal, double kreg,
+                                     double ktreg, double divfree,
+                                     int periodic, const void* p_exact,
+                                     const void* const* fields,
+                                     void* const* grad, double* partial,
+                                     void* stream);
+
 /* Batched Poisson (many independent instances of uneven size in one
  * persistent launch; the reference solves one problem per process).
  * Instance i owns cells [off[i], off[i] + n0[i] n1[i]) of the packed u, src,

@@ -487,6 +487,118 @@ class Advection:
         return split.sum(), [g], split
 
 
+
+class AdvectionInverse:
+    """Inverse advection problem of advection_2d.py:103-250
+    (problem='inverse', central=0, implicit=0) on an (nt, nx, ny)
+    space-time grid, t in [0, tmax], x, y in [0, 1]: unknowns u, v, p;
+    equations [fe, fp] + [fsu, fsv] (kreg != 0) + [ftregu, ftregv]
+    (ktreg != 0) + [fdiv] (divfree != 0) in the reference's order. fe is
+    the misfit to `p_exact` annealed by 0.5^(epoch / anneal_half) (only at
+    the first and last time rows with onlyfinal); fp is the advection
+    residual with the live velocity and first-order upwinding on the sign of
+    the frozen one; x, y periodic or zero outside (periodic=False). One
+    kernel each for the residuals (`odil_advection_inverse_residual`) and
+    the fused loss + adjoint (`odil_advection_inverse_loss_grad`)."""
+
+    fieldnames = ('u', 'v', 'p')
+
+    def __init__(self, shape, p_exact, alpha=1., anneal_half=0.,
+                 onlyfinal=False, kreg=1e-4, ktreg=0., divfree=0.,
+                 periodic=True, tmax=1.):
+        self.shape = tuple(int(s) for s in shape)
+        if len(self.shape) != 3 or min(self.shape[1:]) < 3:
+            raise ValueError('shape must be (nt, nx, ny) with nx, ny >= 3')
+        self.ncell = int(np.prod(self.shape))
+        nt, nx, ny = self.shape
+        self.dt, self.dx, self.dy = tmax / nt, 1. / nx, 1. / ny
+        _lib.require_device(p_exact)
+        if tuple(p_exact.shape) != self.shape:
+            raise ValueError('p_exact must have shape {}'.format(self.shape))
+        self.p_exact = p_exact
+        self.alpha, self.anneal_half = float(alpha), float(anneal_half)
+        self.onlyfinal, self.periodic = bool(onlyfinal), bool(periodic)
+        self.kreg, self.ktreg = float(kreg), float(ktreg)
+        self.divfree = float(divfree)
+        self.neq = _lib.lib().odil_advection_inverse_neq(
+            c_double(self.kreg), c_double(self.ktreg),
+            c_double(self.divfree))
+
+    def anneal(self, epoch):
+        """get_anneal_factor (advection_2d.py:25-26)."""
+        return 0.5**(epoch / self.anneal_half) if self.anneal_half else 1.
+
+    def _args(self, dtag, epoch):
+        return [c_int(dtag)] + [c_int(s) for s in self.shape] + [
+            c_double(self.dt), c_double(self.dx), c_double(self.dy),
+            c_double(self.alpha * self.anneal(epoch)),
+            c_int(int(self.onlyfinal)), c_double(self.kreg),
+            c_double(self.ktreg), c_double(self.divfree),
+            c_int(int(self.periodic)), ptr(self._pex(dtag))]
+
+    def _pex(self, dtag):
+        want = torch.float64 if dtag == _lib.F64 else torch.float32
+        if self.p_exact.dtype == want:
+            return self.p_exact
+        if getattr(self, '_pex_cast', None) is None or \
+                self._pex_cast.dtype != want:
+            self._pex_cast = self.p_exact.to(want)
+        return self._pex_cast
+
+    def _check(self, fields):
+        if len(fields) != 3:
+            raise ValueError('fields are (u, v, p)')
+        dt = fields[0].dtype
+        for f in fields:
+            _lib.require_device(f)
+            if tuple(f.shape) != self.shape or f.dtype != dt:
+                raise ValueError('fields must have shape {} and one dtype'
+                                 .format(self.shape))
+        return _lib.dtype_tag(dt)
+
+    def _reduce(self, partial, nb, dev):
+        ssq = torch.empty(self.neq, dtype=torch.float64, device=dev)
+        check(_lib.lib().odil_reduce_partials(ptr(partial), c_int(self.neq),
+                                              c_int(nb), ptr(ssq),
+                                              _lib.stream()),
+              'reduce_partials')
+        return ssq
+
+    def _nb(self):
+        return _lib.lib().odil_advection_inverse_nblocks(
+            *[c_int(s) for s in self.shape])
+
+    def residual(self, fields, epoch=0, want_fields=True):
+        """(residual fields or None, per-equation sums of squares)."""
+        dtag = self._check(fields)
+        dev = fields[0].device
+        res = [torch.empty_like(fields[0]) for _ in range(self.neq)] \
+            if want_fields else None
+        nb = self._nb()
+        partial = torch.empty((self.neq, nb), dtype=torch.float64,
+                              device=dev)
+        check(_lib.lib().odil_advection_inverse_residual(
+            *self._args(dtag, epoch), ptrs(fields),
+            ptrs(res) if res else None, ptr(partial), _lib.stream()),
+            'advection_inverse_residual')
+        return res, self._reduce(partial, nb, dev)
+
+    def loss_grad(self, fields, grads=None, epoch=0):
+        """(loss, [grad u, grad v, grad p], loss_split): loss =
+        sum_e mean(f_e^2), its gradient with the upwind direction held."""
+        dtag = self._check(fields)
+        dev = fields[0].device
+        grads = grads or [torch.empty_like(f) for f in fields]
+        nb = self._nb()
+        partial = torch.empty((self.neq, nb), dtype=torch.float64,
+                              device=dev)
+        check(_lib.lib().odil_advection_inverse_loss_grad(
+            *self._args(dtag, epoch), ptrs(fields), ptrs(grads),
+            ptr(partial), _lib.stream()), 'advection_inverse_loss_grad')
+        split = self._reduce(partial, nb, dev) / self.ncell
+        return split.sum(), grads, split
+
+
 class WaveInverse:
     """Inverse wave problem: unknown field u on the (nt, nx) space-time grid
     and unknown speed c(x) [nx]. Loss = mean(f_wave^2) + mean(f_data^2) +

@@ -380,6 +380,90 @@ def make_advection_operator(ax, ay, p_init):
     return operator
 
 
+def make_advection_inverse_operator(p_exact, alpha=1., anneal_half=0.,
+                                    onlyfinal=False, kreg=1e-4, ktreg=0.,
+                                    divfree=0., periodic=True):
+    """`operator` of advection_2d.py:103-250 with problem='inverse',
+    central=0, implicit=0: unknowns u, v, p on (t, x, y), equations
+    [fe, fp] + [fsu, fsv] (kreg) + [ftregu, ftregv] (ktreg) + [fdiv]
+    (divfree). The upwind direction of p_x, p_y comes from the frozen
+    velocity (`uf`, `vf`), the advecting velocity is live; `fe` is annealed
+    by 0.5^(epoch / anneal_half) (`get_anneal_factor`, :25-26). Non-periodic
+    grids zero the x/y halo of every stencil (`apply_bc_zero`); at it = 0
+    the previous time level of p is p_exact (:200-202)."""
+    pex = torch.as_tensor(np.asarray(p_exact, dtype=np.float64))
+
+    def operator(mod, ctx):
+        dt = ctx.step('t')
+        dx = ctx.step('x')
+        dy = ctx.step('y')
+        it = ctx.cell_index('t')
+        ix = ctx.cell_index('x')
+        iy = ctx.cell_index('y')
+        nt = ctx.size('t')
+        nx = ctx.size('x')
+        ny = ctx.size('y')
+        zeros = ctx.field('zeros')
+
+        def st(key, freeze=False, shift_t=0):
+            return [ctx.field(key, shift_t, 0, 0, freeze=freeze),
+                    ctx.field(key, shift_t, -1, 0, freeze=freeze),
+                    ctx.field(key, shift_t, 1, 0, freeze=freeze),
+                    ctx.field(key, shift_t, 0, -1, freeze=freeze),
+                    ctx.field(key, shift_t, 0, 1, freeze=freeze)]
+
+        def bc_zero(s):
+            if periodic:
+                return s
+            return [s[0], mod.where(ix == 0, zeros, s[1]),
+                    mod.where(ix == nx - 1, zeros, s[2]),
+                    mod.where(iy == 0, zeros, s[3]),
+                    mod.where(iy == ny - 1, zeros, s[4])]
+
+        u_st = bc_zero(st('u'))
+        v_st = bc_zero(st('v'))
+        uf = ctx.field('u', freeze=True)
+        vf = ctx.field('v', freeze=True)
+        p_st = bc_zero(st('p'))
+        ptm = ctx.field('p', -1, 0, 0)
+        ptm = mod.where(it == 0, pex, ptm)
+        p, pxm, pxp, pym, pyp = p_st
+        p_x = mod.where(uf > 0, (p - pxm) / dx, (pxp - p) / dx)
+        p_y = mod.where(vf > 0, (p - pym) / dy, (pyp - p) / dy)
+        u, v = u_st[0], v_st[0]
+        fp = (p - ptm) / dt + u * p_x + v * p_y
+        anneal = 0.5**(ctx.epoch / anneal_half) if anneal_half else 1.
+        fe = (p - pex) * alpha * anneal
+        if onlyfinal:
+            fe = mod.where((it == 0) | (it == nt - 1), fe, zeros)
+        res = [fe, fp]
+
+        def lap(s):
+            q, qxm, qxp, qym, qyp = s
+            return (qxp - 2 * q + qxm) / dx**2 + (qyp - 2 * q + qym) / dy**2
+
+        if kreg:
+            res += [lap(u_st) * kreg, lap(v_st) * kreg]
+        if ktreg:
+            fu = (u - ctx.field('u', -1, 0, 0)) / dt * ktreg
+            fv = (v - ctx.field('v', -1, 0, 0)) / dt * ktreg
+            res += [mod.where(it == 0, zeros, fu),
+                    mod.where(it == 0, zeros, fv)]
+        if divfree:
+            u_xc = (u_st[2] - u_st[1]) / (2 * dx)
+            v_yc = (v_st[4] - v_st[3]) / (2 * dy)
+            res.append((u_xc + v_yc) * divfree)
+        return res
+
+    return operator
+
+
+def advection_inverse_domain(nt, nx, ny, tmax=1.0):
+    from oracle.tape import Domain
+    return Domain(ndim=3, shape=(nt, nx, ny), upper=(tmax, 1., 1.),
+                  varnames=('t', 'x', 'y'), fieldnames=['u', 'v', 'p'])
+
+
 def advection_setup(nt, nx, ny, seed, tmax=1.0):
     """Seeded advection case: a ~ U(-1,1)^2, Gaussian blob p_init with centre
     U(0.3,0.7)^2 and sigma 0.1 on x,y in [0,1]."""

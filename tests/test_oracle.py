@@ -582,3 +582,86 @@ def test_advection_closed_form(ax, ay):
     sel[seam_x] = False
     np.testing.assert_allclose(ff[:, :, seam_y][:, sel], wy[:, sel],
                                rtol=1e-12, atol=1e-11)
+
+
+# -- inverse advection (advection_2d.py:224-250) -----------------------------
+def test_advection_inverse_closed_form():
+    """u = a + b x (a > |b|, so upwinding is backward everywhere),
+    v = c y + g t, p = beta t + x^2 + y^2, p_exact = p: off the periodic
+    seams fp = beta + u (2x - dx) + v (2y - s_v dy), fe = 0, the kreg
+    Laplacians of the linear fields vanish, ktreg gives g ktreg (0 at
+    t = 0) for v and 0 for u, and fdiv = divfree (b + c)."""
+    nt, nx, ny = 4, 7, 9
+    dom = ops.advection_inverse_domain(nt, nx, ny)
+    t, x, y = dom.cell_center_all()
+    dt, dx, dy = 1. / nt, 1. / nx, 1. / ny
+    a, b, c, g, beta = 2.0, 0.5, -0.8, 0.3, 1.7
+    u = a + b * x
+    v = c * y + g * t
+    p = beta * t + x**2 + y**2
+    pex = p.copy()
+    pex[0] = beta * (t[0] - dt) + x[0]**2 + y[0]**2  # previous level at t0
+    op = ops.make_advection_inverse_operator(pex, alpha=1.3, kreg=0.4,
+                                             ktreg=0.6, divfree=0.9)
+    fe, fp, fsu, fsv, ftu, ftv, fdiv = tape.eval_operator(
+        op, dom, {'u': u, 'v': v, 'p': p})
+    sv = np.where(v > 0, 1., -1.)
+    keep = np.ones(p.shape, bool)
+    keep[:, 0, :] = keep[:, -1, :] = False
+    keep[:, :, 0] = keep[:, :, -1] = False
+    want = beta + u * (2 * x - dx) + v * (2 * y - sv * dy)
+    np.testing.assert_allclose(fp[keep], want[keep], rtol=1e-12, atol=1e-11)
+    np.testing.assert_allclose(fe[1:], 0, atol=1e-13)
+    np.testing.assert_allclose(fe[0], 1.3 * beta * dt, rtol=1e-12)
+    np.testing.assert_allclose(fsu[keep], 0, atol=1e-9)
+    np.testing.assert_allclose(fsv[keep], 0, atol=1e-9)
+    np.testing.assert_allclose(ftu, 0, atol=1e-12)
+    np.testing.assert_allclose(ftv[1:], 0.6 * g, rtol=1e-12)
+    np.testing.assert_array_equal(ftv[0], 0)
+    np.testing.assert_allclose(fdiv[keep], 0.9 * (b + c), rtol=1e-12)
+
+
+def test_advection_inverse_matches_direct_operator():
+    """With a constant velocity field the inverse operator's fp equals the
+    residual of the direct advection oracle (pinned above) whose initial
+    level is p_exact at t = 0."""
+    dom_i = ops.advection_inverse_domain(5, 6, 7)
+    dom_d, op_d, (ax, ay, p0) = ops.advection_setup(5, 6, 7, seed=2)
+    rng = np.random.default_rng(1)
+    p = rng.normal(size=dom_i.shape)
+    pex = rng.normal(size=dom_i.shape)
+    pex[0] = p0
+    op_i = ops.make_advection_inverse_operator(pex, kreg=0.)
+    ff = tape.eval_operator(op_i, dom_i, {
+        'u': np.full(dom_i.shape, ax), 'v': np.full(dom_i.shape, ay),
+        'p': p})
+    fd = tape.eval_operator(op_d, dom_d, {'p': p})[0]
+    np.testing.assert_allclose(ff[1], fd, rtol=1e-12, atol=1e-11)
+    np.testing.assert_allclose(ff[0], p - pex, rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize('periodic,onlyfinal', [(True, False), (False, True)])
+def test_advection_inverse_gradient_fd(periodic, onlyfinal):
+    """Tape gradient (frozen upwind direction) against central differences:
+    velocities kept away from zero so the direction does not flip."""
+    dom = ops.advection_inverse_domain(3, 5, 4)
+    rng = np.random.default_rng(7)
+    sgn = np.where(rng.uniform(size=dom.shape) < 0.5, -1., 1.)
+    st = {'u': sgn * rng.uniform(0.5, 1.5, dom.shape),
+          'v': -sgn * rng.uniform(0.5, 1.5, dom.shape),
+          'p': rng.normal(size=dom.shape)}
+    op = ops.make_advection_inverse_operator(
+        rng.normal(size=dom.shape), alpha=0.7, anneal_half=3.,
+        onlyfinal=onlyfinal, kreg=0.2, ktreg=0.5, divfree=0.4,
+        periodic=periodic)
+    _, grads, _ = tape.eval_loss_grad(op, dom, st, epoch=2)
+
+    def loss(name, x):
+        s = dict(st)
+        s[name] = x
+        return tape.eval_loss_grad(op, dom, s, epoch=2)[0]
+
+    for name in 'uvp':
+        fd = _fd_grad(lambda x: loss(name, x), st[name], 1e-6)
+        np.testing.assert_allclose(grads[name], fd, rtol=1e-6,
+                                   atol=1e-7 * np.abs(fd).max())
