@@ -8,10 +8,14 @@ all on the GPU:
 
 * 'direct' / 'direct_cu' (SuperLU / cuSolver `spsolve` of
   A = J^T J + beta^2 I + betadiag^2 diag, linsolver.py:35-53): A in stencil
-  form (`odil_stencil_normal`), its dense FP64 matrix
-  (`odil_stencil_to_dense`) and a Cholesky factorisation, for systems up to
-  `max_direct` unknowns. Singular normal equations (e.g. a pressure null
-  mode) need beta > 0 here.
+  form (`odil_stencil_normal`); up to `max_dense` unknowns its dense FP64
+  matrix (`odil_stencil_to_dense`) and a Cholesky factorisation, above that
+  a sparse direct solve: the unknowns interleaved per cell make A
+  block-tridiagonal with blocks of nvar (max stencil shift + 1) rows, which
+  a block Cholesky factors in n B^2 work (the reference's N = 128 cavity,
+  49 152 unknowns, B = 774). Singular normal equations (the cavity's
+  constant pressure) are solved with a 1e-14 max(diag) Tikhonov shift, the
+  minimum-norm-like answer (SuperLU's null component is set by round-off).
 * 'lsqr' / 'lsqr_cu' (SciPy / CuPy `lsqr` on J itself with damp = beta,
   atol = btol = linsolver_tol, linsolver.py:57-75): the Paige-Saunders
   LSQR bidiagonalisation with J and J^T applied by the stencil kernels and
@@ -332,14 +336,186 @@ def pcg_multigrid(A, b, tol=1e-6, maxiter=50, nlvl=None, omega=0.6,
     return x, res
 
 
+# Relative Tikhonov shift of a retried direct solve (singular normal
+# equations: the cavity's constant-pressure mode).
+DIRECT_SHIFT = 1e-14
+
+
+def _normal_apply_reg(N, beta, betadiag):
+    """x -> A x with A = J^T J + beta^2 I, its diagonal scaled by
+    1 + betadiag^2 (linsolver.py:36-40), from the stored normal operator."""
+    diag = torch.stack(N.diagonal()) if betadiag else None
+
+    def apply(x):
+        y = torch.stack(N.apply([a.contiguous() for a in x]))
+        if beta:
+            y += beta**2 * x
+        if betadiag:
+            y += betadiag**2 * (diag + beta**2) * x
+        return y
+    return apply
+
+
+def _refine(apply, solve, b, x, passes=2):
+    """FP64 iterative refinement x += M^-1 (b - A x) with M the shifted
+    factorisation: removes the O(shift / lambda) bias the Tikhonov shift of
+    a singular system leaves on the smoothest modes."""
+    for _ in range(passes):
+        x = x + solve(b - apply(x))
+    return x
+
+
+def _direct_dense(matr, b, beta, betadiag):
+    """Dense Cholesky of J^T J (+beta^2 I, diagonal scaled by 1 +
+    betadiag^2, linsolver.py:35-45); if the matrix is singular (Cholesky
+    breaks down) it is factored with DIRECT_SHIFT * max(diag) added and the
+    solution refined against the unshifted matrix."""
+    nvar = matr.nvar_in
+    N = normal_operator(matr, torch.float64, sym=True)
+    M = N.to_dense()
+    if beta:
+        M.diagonal().add_(beta**2)
+    if betadiag:  # diag of the beta-regularised matrix (linsolver.py:39)
+        M.diagonal().mul_(1. + betadiag**2)
+    L, info = torch.linalg.cholesky_ex(M)
+    shifted = int(info.item()) != 0
+    if shifted:
+        M.diagonal().add_(DIRECT_SHIFT * float(M.diagonal().max()))
+        L, info = torch.linalg.cholesky_ex(M)
+        if int(info.item()) != 0:
+            raise ValueError('normal equations not positive semi-definite')
+    shape = (nvar, ) + tuple(matr.nw)
+    solve = lambda r: torch.cholesky_solve(r.reshape(-1, 1), L).reshape(
+        shape)
+    bb = torch.stack(list(b))
+    x = solve(bb)
+    if shifted:
+        x = _refine(_normal_apply_reg(N, beta, betadiag), solve, bb, x)
+    return x
+
+
+def _band_blocks(N, nvar, beta, betadiag):
+    """Block-tridiagonal form of the stored (full) normal operator N with
+    the unknowns interleaved cell by cell (index nvar c + field): a
+    coupling of flat cell shift s lies within nvar (|s| + 1) of the
+    diagonal, so blocks of B = nvar (max|s| + 1) rows couple only to their
+    neighbours. Returns (D [nb][B][B] diagonal blocks, Lo [nb-1][B][B]
+    blocks below the diagonal, B, n)."""
+    nw = N.nw
+    ncell = int(np.prod(nw))
+    strides = [int(np.prod(nw[d + 1:])) for d in range(len(nw))]
+    dev = N.coef.device
+    smax = max(abs(sum(a * b for a, b in zip(sh, strides)))
+               for _, _, sh in N.entries)
+    B = nvar * (smax + 1)
+    n = nvar * ncell
+    nb = (n + B - 1) // B
+    D = torch.zeros((nb, B, B), dtype=torch.float64, device=dev)
+    Lo = torch.zeros((max(nb - 1, 1), B, B), dtype=torch.float64,
+                     device=dev)
+    cells = torch.arange(ncell, device=dev)
+    idx = []
+    rem = cells
+    for d in range(len(nw)):
+        idx.append(rem // strides[d])
+        rem = rem % strides[d]
+    for q, (o, i, sh) in enumerate(N.entries):
+        ok = torch.ones(ncell, dtype=torch.bool, device=dev)
+        for d, sd in enumerate(sh):
+            ok &= (idx[d] + sd >= 0) & (idx[d] + sd < nw[d])
+        c = cells[ok]
+        sf = sum(a * b for a, b in zip(sh, strides))
+        rows = nvar * c + o
+        cols = nvar * (c + sf) + i
+        vals = N.coef[q].reshape(-1)[ok].double()
+        rb, cb = rows // B, cols // B
+        on = rb == cb
+        D.index_put_((rb[on], rows[on] % B, cols[on] % B), vals[on],
+                     accumulate=True)
+        lo = cb == rb - 1
+        Lo.index_put_((rb[lo] - 1, rows[lo] % B, cols[lo] % B), vals[lo],
+                      accumulate=True)
+    diag = D.diagonal(dim1=1, dim2=2)
+    if beta:
+        diag.add_(beta**2)
+    if betadiag:
+        diag.mul_(1. + betadiag**2)
+    # Rows past the last unknown (padding of the last block): identity.
+    pad = nb * B - n
+    if pad:
+        D[-1, B - pad:, B - pad:].diagonal().fill_(1.)
+    return D, Lo, B, n
+
+
+def _direct_banded(matr, b, beta, betadiag):
+    """Sparse direct solve of J^T J x = b: block Cholesky of the
+    block-tridiagonal interleaved form (`_band_blocks`),
+    L_kk L_kk^T = D_k - L_k,k-1 L_k,k-1^T, L_k+1,k = Lo_k L_kk^-T, then the
+    two block substitutions. Work n B^2 and memory 2 n B (B ~ nvar *
+    reach * row length) instead of the dense n^3 and n^2. A singular system
+    is factored with DIRECT_SHIFT * max(diag) on the diagonal and the
+    solution refined against the unshifted operator."""
+    nvar = matr.nvar_in
+    N = normal_operator(matr, torch.float64, sym=False)
+    D, Lo, B, n = _band_blocks(N, nvar, beta, betadiag)
+    nb = D.shape[0]
+    shift = DIRECT_SHIFT * float(D.diagonal(dim1=1, dim2=2).abs().max())
+    for attempt in range(2):
+        Lk, Lsub = [], []
+        ok = True
+        for k in range(nb):
+            A = D[k].clone()
+            if attempt:
+                A.diagonal().add_(shift)
+            if k:
+                A -= Lsub[-1] @ Lsub[-1].T
+            L, info = torch.linalg.cholesky_ex(A)
+            if int(info.item()) != 0:
+                ok = False
+                break
+            Lk.append(L)
+            if k + 1 < nb:
+                # L_{k+1,k} = Lo_k L_kk^-T  <=>  L_kk L_{k+1,k}^T = Lo_k^T
+                Lsub.append(torch.linalg.solve_triangular(
+                    L, Lo[k].T, upper=False).T)
+        if ok:
+            break
+    else:
+        raise ValueError('normal equations not positive semi-definite')
+    shape = (nvar, ) + tuple(matr.nw)
+
+    def solve(r):
+        # fields (nvar, *nw) -> interleaved, padded blocks and back
+        rr = r.reshape(nvar, -1).T.reshape(-1)
+        rr = torch.cat([rr, rr.new_zeros(nb * B - n)]).view(nb, B, 1)
+        y = []
+        for k in range(nb):
+            t = rr[k] if not k else rr[k] - Lsub[k - 1] @ y[-1]
+            y.append(torch.linalg.solve_triangular(Lk[k], t, upper=False))
+        x = [None] * nb
+        for k in reversed(range(nb)):
+            t = y[k] if k + 1 == nb else y[k] - Lsub[k].T @ x[k + 1]
+            x[k] = torch.linalg.solve_triangular(Lk[k].T, t, upper=True)
+        sol = torch.cat(x).reshape(-1)[:n]
+        return sol.view(-1, nvar).T.contiguous().reshape(shape)
+
+    bb = torch.stack(list(b))
+    x = solve(bb)
+    if attempt:
+        x = _refine(_normal_apply_reg(N, beta, betadiag), solve, bb, x)
+    return x
+
+
 def solve(matr, rhs, args=None, dhistory=None, linsolver='direct',
-          max_direct=40000):
+          max_dense=16384):
     """Returns the correction fields dx (list of device tensors, FP64).
 
     matr: `BlockStencil` Jacobian (equations x fields), FP64.
     rhs: list of residual-shaped device tensors (one per equation), usually
          -f as in `optimize_newton` (util.py:261-267).
-    args: optional namespace with `beta`, `betadiag`, `linsolver_maxiter`
+    args: optional namespace with `beta`, `betadiag`, `direct_max_dense`
+          (largest system given to the dense direct path, default
+          `max_dense`), `linsolver_maxiter`
           (default 1000 for 'lsqr', else 50, linsolver.py:29-33),
           `linsolver_tol`, `nlvl`, `omega`, `smooth_pre`, `smooth_post`,
           `ndirect` (linsolver.py:108-170).
@@ -373,23 +549,10 @@ def solve(matr, rhs, args=None, dhistory=None, linsolver='direct',
     b = matr.apply_t(rhs)
     if linsolver in ('direct', 'direct_cu'):
         n = nvar * int(np.prod(matr.nw))
-        if n > max_direct:
-            raise ValueError('direct solve limited to {} unknowns, got {}; '
-                             "use linsolver='multigrid'".format(max_direct,
-                                                                n))
-        N = normal_operator(matr, torch.float64, sym=True)
-        M = N.to_dense()
-        if beta:
-            M.diagonal().add_(beta**2)
-        if betadiag:  # diag of the beta-regularised matrix (linsolver.py:39)
-            M.diagonal().mul_(1. + betadiag**2)
-        L, info = torch.linalg.cholesky_ex(M)
-        if int(info.item()) != 0:
-            raise ValueError('normal equations not positive definite '
-                             '(pass beta > 0 or use multigrid)')
-        sol = torch.cholesky_solve(
-            torch.cat([x.reshape(-1) for x in b]).unsqueeze(1), L)
-        sol = sol.reshape(nvar, *matr.nw)
+        if n <= _arg(args, 'direct_max_dense', max_dense):
+            sol = _direct_dense(matr, b, beta, betadiag)
+        else:
+            sol = _direct_banded(matr, b, beta, betadiag)
         dh['linsolver_niter'] = 1
         return [sol[k].contiguous() for k in range(nvar)]
     A = _regularised_normal(matr, beta, betadiag)
