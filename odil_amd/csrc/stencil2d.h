@@ -161,6 +161,11 @@ __device__ inline void stage_tile(const FieldPtrs<T, NF>& in, T* tile,
 // loss-adjoint tiles of 4..32 x 32..128.)
 constexpr int kResTx = 16, kResTy = 64;
 constexpr int kGradTx32 = 16, kGradTx64 = 8, kGradTy = 32;
+// LDS staging of the matrix-free normal operator (k_normal_apply STG),
+// measured on the 1025^2 cavity with HBM-resident inputs: FP32 state and x
+// staged 28.8 us (nothing staged 30.1); FP64 nothing staged 47.7 us (state
+// and x staged 61.4: three FP64 tiles leave two workgroups per CU).
+constexpr int kNormStage32 = 2, kNormStage64 = 0;
 
 constexpr int kWavesPerBlock = 4;
 inline int residual_nblocks(int n0, int n1) {
@@ -274,6 +279,23 @@ __device__ inline void ring_cell(int q, int& ea, int& eb) {
     const int col = r - row * (2 * RJ);
     ea = RJ + row;
     eb = col < RJ ? col : TY + col;
+  }
+}
+
+// Residual and Jacobian of cell (gi, gj) reading the state from global
+// memory (neighbour reuse through L1/L2); INT blocks skip clamping and wall
+// rules.
+template <class T, class Op>
+__device__ inline void eval_global(const FieldPtrs<T, Op::NF>& in, int gi,
+                                   int gj, int n0, int n1, bool inner,
+                                   const typename Op::Params& P, T* f,
+                                   T (*J)[Op::NF][Op::NS]) {
+  if (inner) {
+    GlobalAcc<T, Op::NF, false> acc{in.f, n0, n1, gi, gj};
+    Op::template eval<true>(acc, gi, gj, n0, n1, P, f, J);
+  } else {
+    GlobalAcc<T, Op::NF> acc{in.f, n0, n1, gi, gj};
+    Op::template eval<false>(acc, gi, gj, n0, n1, P, f, J);
   }
 }
 
@@ -437,9 +459,12 @@ constexpr size_t loss_grad_smem() {
 //   mode 0: y = N x          mode 1: y = b - N x
 //   mode 2: y = x + omega (b - N x) / d   (weighted Jacobi, d = diag N)
 //   mode 3: y = diag(N) = sum_e sum_s J[e][k][s](c - shift_s)^2
-// Same tile scheme as k_loss_grad with f replaced by v = J x; the state
-// (halo RJ + R) and x (halo 2 RJ) are staged in LDS after the products.
-template <class T, class Op, int TX, int TY>
+// Same tile scheme as k_loss_grad with f replaced by v = J x. STG selects
+// what is staged in LDS after the products: 0 nothing (state and x read
+// through the cache), 1 the state (halo RJ + R), 2 the state and x (halo
+// 2 RJ). Staging cuts global loads but costs LDS, hence resident
+// workgroups; the dispatch picks the measured best per precision.
+template <class T, class Op, int TX, int TY, int STG>
 __global__ void __launch_bounds__(256)
     k_normal_apply(FieldPtrs<T, Op::NF> st_in, FieldPtrs<T, Op::NF> x_in,
                    FieldPtrs<T, Op::NF> b_in, FieldPtrs<T, Op::NF> d_in,
@@ -470,9 +495,18 @@ __global__ void __launch_bounds__(256)
   };
   // Coefficients reaching outside the grid are zero, so the clamped values
   // the staging stores there are never weighted.
-  stage_tile<T, NF, TX, TY, H>(st_in, st, x0, y0, n0, n1);
-  if (!diag) stage_tile<T, NF, TX, TY, 2 * RJ>(x_in, xt, x0, y0, n0, n1);
-  __syncthreads();
+  if constexpr (STG >= 1) stage_tile<T, NF, TX, TY, H>(st_in, st, x0, y0, n0,
+                                                        n1);
+  if constexpr (STG >= 2)
+    if (!diag) stage_tile<T, NF, TX, TY, 2 * RJ>(x_in, xt, x0, y0, n0, n1);
+  if constexpr (STG >= 1) __syncthreads();
+  auto xat = [&](int k, int i, int j) -> T {
+    if (!inner) {  // coefficients reaching outside are zero; clamp reads
+      i = min(max(i, 0), n0 - 1);
+      j = min(max(j, 0), n1 - 1);
+    }
+    return x_in.f[k][(size_t)i * n1 + j];
+  };
 
   // Products of the cell at grown-tile position (ea, eb):
   // o[k][s] = sum_e J[e][k][s] v_e with v = J x (or J[e][k][s]^2 summed
@@ -480,8 +514,11 @@ __global__ void __launch_bounds__(256)
   auto cell = [&](int ea, int eb, int gi, int gj, T (*o)[NS]) {
     T f[NE];
     T J[NE][NF][NS];
-    eval_lds<T, Op, S0, S1>(st + (ea + Op::R) * S1 + (eb + Op::R), gi, gj,
-                            n0, n1, inner, P, f, J);
+    if constexpr (STG >= 1)
+      eval_lds<T, Op, S0, S1>(st + (ea + Op::R) * S1 + (eb + Op::R), gi, gj,
+                              n0, n1, inner, P, f, J);
+    else
+      eval_global<T, Op>(st_in, gi, gj, n0, n1, inner, P, f, J);
     if (diag) {
 #pragma unroll
       for (int k = 0; k < NF; ++k)
@@ -495,12 +532,20 @@ __global__ void __launch_bounds__(256)
       return;
     }
     T xv[NF][NS];
-    const T* xc = xt + (ea + RJ) * X1 + (eb + RJ);
+    if constexpr (STG >= 2) {
+      const T* xc = xt + (ea + RJ) * X1 + (eb + RJ);
 #pragma unroll
-    for (int k = 0; k < NF; ++k)
+      for (int k = 0; k < NF; ++k)
 #pragma unroll
-      for (int s = 0; s < NS; ++s)
-        xv[k][s] = xc[k * (X0 * X1) + Op::SX[s] * X1 + Op::SY[s]];
+        for (int s = 0; s < NS; ++s)
+          xv[k][s] = xc[k * (X0 * X1) + Op::SX[s] * X1 + Op::SY[s]];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NF; ++k)
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+          xv[k][s] = xat(k, gi + Op::SX[s], gj + Op::SY[s]);
+    }
     T v[NE];
 #pragma unroll
     for (int e = 0; e < NE; ++e) {
@@ -575,21 +620,24 @@ __global__ void __launch_bounds__(256)
       else if (mode == 1)
         out = b_in.f[k][c] - g;
       else
-        out = xt[k * (X0 * X1) + (a + 2 * RJ) * X1 + (b + 2 * RJ)] +
+        out = (STG >= 2
+                   ? xt[k * (X0 * X1) + (a + 2 * RJ) * X1 + (b + 2 * RJ)]
+                   : x_in.f[k][c]) +
               omega * (b_in.f[k][c] - g) / d_in.f[k][c];
       y.f[k][c] = out;
     }
   }
 }
 
-template <class T, class Op, int TX, int TY>
+template <class T, class Op, int TX, int TY, int STG>
 constexpr size_t normal_apply_smem() {
   constexpr int H = Op::RJ + Op::R;
-  return sizeof(T) * (size_t)(Op::NF * (Op::NS - 1) * (TX + 2 * Op::RJ) *
-                                  (TY + 2 * Op::RJ) +
-                              Op::NF * (TX + 2 * H) * (TY + 2 * H) +
-                              Op::NF * (TX + 4 * Op::RJ) *
-                                  (TY + 4 * Op::RJ));
+  return sizeof(T) *
+         (size_t)(Op::NF * (Op::NS - 1) * (TX + 2 * Op::RJ) *
+                      (TY + 2 * Op::RJ) +
+                  (STG >= 1 ? Op::NF * (TX + 2 * H) * (TY + 2 * H) : 0) +
+                  (STG >= 2 ? Op::NF * (TX + 4 * Op::RJ) * (TY + 4 * Op::RJ)
+                            : 0));
 }
 
 // Deterministic final reduction of per-block partials: out[e] = sum_b
@@ -655,7 +703,7 @@ int launch_loss_grad_t(const void* const* fields, void* const* grad,
   return check_launch("loss_grad");
 }
 
-template <class T, class Op, int TX, int TY>
+template <class T, class Op, int TX, int TY, int STG>
 int launch_normal_apply_t(const void* const* state, const void* const* x,
                           const void* const* b, const void* const* d,
                           void* const* y, int n0, int n1, double omega,
@@ -680,9 +728,9 @@ int launch_normal_apply_t(const void* const* state, const void* const* x,
     Y.f[k] = static_cast<T*>(y[k]);
   }
   dim3 grid(ceil_div(n1, TY), ceil_div(n0, TX));
-  constexpr size_t smem = normal_apply_smem<T, Op, TX, TY>();
+  constexpr size_t smem = normal_apply_smem<T, Op, TX, TY, STG>();
   static_assert(smem <= 160 * 1024, "LDS tile too large");
-  auto kern = k_normal_apply<T, Op, TX, TY>;
+  auto kern = k_normal_apply<T, Op, TX, TY, STG>;
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)smem);
@@ -698,11 +746,11 @@ int dispatch_normal_apply(int dtype, const void* const* state,
                           int n1, double omega, int mode,
                           const typename Op64::Params& P, hipStream_t s) {
   if (dtype == kF64)
-    return launch_normal_apply_t<double, Op64, 8, 32>(state, x, b, d, y, n0,
-                                                      n1, omega, mode, P, s);
+    return launch_normal_apply_t<double, Op64, 8, 32, kNormStage64>(
+        state, x, b, d, y, n0, n1, omega, mode, P, s);
   if (dtype == kF32)
-    return launch_normal_apply_t<float, Op32, 8, 32>(state, x, b, d, y, n0,
-                                                      n1, omega, mode, P, s);
+    return launch_normal_apply_t<float, Op32, 8, 32, kNormStage32>(
+        state, x, b, d, y, n0, n1, omega, mode, P, s);
   set_error("normal_apply: bad dtype %d", dtype);
   return kErrArg;
 }

@@ -894,7 +894,9 @@ class MultigridOp:
         g = torch.Generator(device=self.device).manual_seed(seed)
         nrm = []
         for level in range(self.nlevels - 1):
-            A = self.AA[level]
+            # The finest level's own operator (the matrix-free one when
+            # set): the one the V-cycle smooths with.
+            A = self._op(level)
             b = buf[level]
             zero = b['r']
             for z in zero:

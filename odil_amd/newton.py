@@ -423,11 +423,13 @@ class NewtonMultigrid:
             self.last_linear = 0
             return rms
         main = torch.cuda.current_stream()
+        need_damping = False
         if self._pending is not None:
             ev, mg = self._pending
             main.wait_event(ev)
             self.mg = mg
             self._pending = None
+            need_damping = True
         side = None
         if (self._nstep % self.setup_every == 0 or not self.mg.AA
                 or self._nstep < self.setup_first
@@ -437,14 +439,13 @@ class NewtonMultigrid:
                 side.record()
             else:
                 self._build(fields, self.mg)
+                need_damping = True
                 for mg in self._mgs:
                     if not mg.AA:
                         # First build of the other hierarchy here, on this
                         # stream: it allocates and tunes its launch shapes.
                         mg.update_A(self.N)
                         mg.prepare(self.ndirect)
-                        if self.omega_auto:
-                            mg.estimate_damping()
         self._nstep += 1
         if self.matrix_free:
             if self.state_w is not fields:
@@ -452,6 +453,10 @@ class NewtonMultigrid:
                     a.copy_(b)
             self.fine.refresh()
             self.mg.fine = self.fine
+        if need_damping and self.omega_auto:
+            # On the hierarchy the solve uses, finest level included (the
+            # matrix-free operator of the current state).
+            self.mg.estimate_damping()
         self.timers = {'setup': time.perf_counter() - t0}
         its = self._linear_solve()
         if side is not None:
@@ -480,8 +485,6 @@ class NewtonMultigrid:
                 normal_operator(J, torch.float64, sym=self.sym)
         mg.update_A(self.N)
         mg.prepare(self.ndirect)
-        if self.omega_auto:
-            mg.estimate_damping()
         return ev
 
     def _setup_side(self, fields, after):
