@@ -9,8 +9,8 @@
  *    the calling process.
  *  - All array arguments are device pointers owned by the caller, except
  *    the small host arrays documented as such (grid sizes `nw`, `reach`,
- *    `ent`/`shifts` outputs of *_jac_entries). No entry point allocates
- *    device memory.
+ *    `ent`/`shifts` outputs of *_jac_entries, `dq` and `ent_host` of the
+ *    smoothers). No entry point allocates device memory.
  *  - Fields are C-order arrays of one grid (axis 0 slowest, e.g. t for the
  *    space-time operators, x for the cavity); "fields" arguments are arrays
  *    of per-field pointers (u, v, p for the cavity).
@@ -482,6 +482,30 @@ int odil_chol_block(double* A, int lda, int k0, int nb, double* W,
 int odil_dense_gemv(int dtype_x, int dtype_y, int nvar, int N,
                     const double* M, const void* const* x, void* const* y,
                     void* stream);
+/* Block-diagonal Jacobi, Multigrid.smoother_jacobi(full=True)
+ * (multigrid.py:284-299): factor the per-cell nvar x nvar matrices of the
+ * shift-0 coefficients of every block (`dq` host [nvar][nvar] index into
+ * the nent coefficient arrays, -1 for none) by LU with partial pivoting into lu [nvar^2][N] and piv
+ * int32 [nvar][N]; *singular (device int32, cleared by the caller) becomes 1
+ * for an exactly zero pivot. odil_block_jacobi then does u += omega D^-1 r
+ * per cell. */
+int odil_block_diag_factor(int dtype, int nvar, int N, const int* dq,
+                           int nent, const void* coef, void* lu, int* piv,
+                           int* singular, void* stream);
+int odil_block_jacobi(int dtype, int nvar, int N, const void* lu,
+                      const int* piv, const void* const* r, void* const* u,
+                      double omega, void* stream);
+/* Multigrid.smoother_gauss_seidel (multigrid.py:301-305): one forward
+ * Gauss-Seidel sweep u += omega (L^-1 (f - (A - L) u) - u), L the lower
+ * triangle (diagonal included) of the square stride-1 operator in the flat
+ * field-major C-order numbering; full (non-symmetric) storage, `ent` on the
+ * device and the same table `ent_host` in host memory, x nvar fields of
+ * working storage. */
+int odil_stencil_gauss_seidel(int dtype, int ndim, const int* nw, int nvar,
+                              const int* ent, const int* ent_host, int nent,
+                              const void* coef, void* const* u,
+                              const void* const* f, void* const* x,
+                              double omega, void* stream);
 
 /* ---- vector kernels of the Krylov / Newton drivers ----------------------
  * Scalars live in device memory so a whole solver iteration can be queued

@@ -10,6 +10,8 @@
 // multigrid.py:941-947). Coefficients are one contiguous array
 // coef[nent][nwh]; the entry table is int32 [nent][5] = (out, in, s0, s1, s2)
 // sorted by `out`, with ostart[nvar_out + 1] delimiting each output field.
+#include <algorithm>
+
 #include "common.h"
 
 namespace odil {
@@ -933,6 +935,170 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Block-diagonal Jacobi (smoother_jacobi with full=True, multigrid.py:
+// 289-294). Cell c has the NV x NV matrix D_c[o][i] = shift-0 coefficient of
+// block (o, i) at c (every block's diagonal, the reference's extdiag). It is
+// factored once per operator by LU with partial pivoting (the method of
+// numpy.linalg.solve, LAPACK gesv) into lu[o NV + i][c] (unit lower factor
+// below the diagonal) and piv[k][c] (the row exchanged with row k at step
+// k); an exactly zero pivot flags the block singular, where numpy raises.
+// The sweep solves D_c du = r_c with those factors and adds omega du to u.
+struct BlockIdx {
+  int q[kMaxVar * kMaxVar];  // entry of block (o, i) at shift 0; -1: none
+};
+
+template <class T, int NV>
+__global__ void __launch_bounds__(256)
+    k_block_factor(BlockIdx bi, const T* __restrict__ coef, T* __restrict__ lu,
+                   int* __restrict__ piv, int* __restrict__ singular,
+                   size_t N) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  T a[NV][NV];
+#pragma unroll
+  for (int o = 0; o < NV; ++o)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int q = bi.q[o * kMaxVar + i];
+      a[o][i] = q >= 0 ? coef[(size_t)q * N + c] : T(0);
+    }
+  bool sing = false;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    int m = k;
+    T best = fabs(a[k][k]);
+#pragma unroll
+    for (int r = k + 1; r < NV; ++r)
+      if (fabs(a[r][k]) > best) {
+        best = fabs(a[r][k]);
+        m = r;
+      }
+    piv[(size_t)k * N + c] = m;
+#pragma unroll
+    for (int r = k + 1; r < NV; ++r)
+      if (r == m) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+          const T t = a[k][j];
+          a[k][j] = a[r][j];
+          a[r][j] = t;
+        }
+      }
+    if (a[k][k] == T(0)) {
+      sing = true;
+      continue;
+    }
+#pragma unroll
+    for (int r = k + 1; r < NV; ++r) {
+      a[r][k] /= a[k][k];
+#pragma unroll
+      for (int j = k + 1; j < NV; ++j) a[r][j] -= a[r][k] * a[k][j];
+    }
+  }
+#pragma unroll
+  for (int o = 0; o < NV; ++o)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) lu[(size_t)(o * NV + i) * N + c] = a[o][i];
+  if (sing) *singular = 1;
+}
+
+template <class T, int NV>
+__global__ void __launch_bounds__(256)
+    k_block_jacobi(const T* __restrict__ lu, const int* __restrict__ piv,
+                   CVecPtrs<T> r, VecPtrs<T> u, size_t N, T omega) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  T b[NV];
+#pragma unroll
+  for (int o = 0; o < NV; ++o) b[o] = r.p[o][c];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const int m = piv[(size_t)k * N + c];
+#pragma unroll
+    for (int q = k + 1; q < NV; ++q)
+      if (q == m) {
+        const T t = b[k];
+        b[k] = b[q];
+        b[q] = t;
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int q = k + 1; q < NV; ++q)
+      b[q] -= lu[(size_t)(q * NV + k) * N + c] * b[k];
+#pragma unroll
+  for (int k = NV - 1; k >= 0; --k) {
+#pragma unroll
+    for (int j = k + 1; j < NV; ++j)
+      b[k] -= lu[(size_t)(k * NV + j) * N + c] * b[j];
+    b[k] /= lu[(size_t)(k * NV + k) * N + c];
+  }
+#pragma unroll
+  for (int o = 0; o < NV; ++o) u.p[o][c] += omega * b[o];
+}
+
+// Forward Gauss-Seidel (smoother_gauss_seidel, multigrid.py:301-305):
+// x = L^-1 (f - (A - L) u) with L the lower triangle of A, diagonal
+// included, in the flat numbering (field-major, cells in C order); then
+// u += omega (x - u). Cell c of field o needs x at the cells of field o that
+// precede it (entries (o, o, s) with s < 0 lexicographically) and at every
+// cell of the fields i < o; the other entries read the unchanged u. Cells of
+// one field with equal level a0 i0 + a1 i1 + i2 are independent when
+// a0 s0 + a1 s1 + s2 <= -1 for every such s (the host picks the smallest
+// a0, a1), so each field is a0 (n0 - 1) + a1 (n1 - 1) + n2 waves, one launch
+// each in level order. A wave's threads walk i0 from `lo` (2D grids) or the
+// (i0, i1) plane from row `lo` (3D), each deriving the last index.
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_gs_wave(const int* __restrict__ ent, int e0, int e1, int qd,
+              const T* __restrict__ coef, CVecPtrs<T> u, VecPtrs<T> x,
+              const T* __restrict__ f, Grid3 nw, int o, int w, int a0, int a1,
+              int lo, int cnt) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cnt) return;
+  int i0, i1, i2;
+  if (nw.n2 == 1) {
+    i0 = lo + t;
+    const int rem = w - a0 * i0;
+    if (rem < 0 || rem % a1) return;
+    i1 = rem / a1;
+    i2 = 0;
+    if (i1 >= nw.n1) return;
+  } else {
+    i0 = lo + t / nw.n1;
+    i1 = t % nw.n1;
+    i2 = w - a0 * i0 - a1 * i1;
+    if (i2 < 0 || i2 >= nw.n2) return;
+  }
+  const size_t N = nw.size();
+  const size_t c = ((size_t)i0 * nw.n1 + i1) * nw.n2 + i2;
+  T acc = 0;
+  for (int e = e0; e < e1; ++e) {
+    if (e == qd) continue;
+    const int* it = ent + 5 * (size_t)e;
+    const int in = it[1], s0 = it[2], s1 = it[3], s2 = it[4];
+    const int p0 = i0 + s0, p1 = i1 + s1, p2 = i2 + s2;
+    if (!inside(p0, p1, p2, nw)) continue;
+    const size_t p = ((size_t)p0 * nw.n1 + p1) * nw.n2 + p2;
+    const bool lower =
+        in < o ||
+        (in == o && (s0 < 0 || (s0 == 0 && (s1 < 0 || (s1 == 0 && s2 < 0)))));
+    const T v = lower ? x.p[in][p] : u.p[in][p];
+    acc += coef[(size_t)e * N + c] * v;
+  }
+  x.p[o][c] = (f[c] - acc) / coef[(size_t)qd * N + c];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_relax(CVecPtrs<T> x, VecPtrs<T> u, size_t N, T omega) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  const int o = blockIdx.y;
+  u.p[o][c] += (x.p[o][c] - u.p[o][c]) * omega;
+}
+
 }  // namespace odil
 
 using namespace odil;
@@ -1593,6 +1759,183 @@ int odil_dense_gemv(int dtype_x, int dtype_y, int nvar, int N,
   }
 #undef ODIL_GEMV
   return check_launch("dense_gemv");
+}
+
+// Factor the per-cell nvar x nvar shift-0 blocks of a stride-1 operator
+// (block-diagonal Jacobi). `dq` (host, nvar * nvar) gives the entry of block
+// (o, i) at shift 0 among the nent coefficient arrays of `coef`, -1 where
+// the block has none. lu [nvar^2][N] in dtype,
+// piv int32 [nvar][N]; *singular (device int32, cleared by the caller) is
+// set to 1 when a block has an exactly zero pivot.
+int odil_block_diag_factor(int dtype, int nvar, int N, const int* dq,
+                           int nent, const void* coef, void* lu, int* piv,
+                           int* singular, void* stream) {
+  if (bad_var(nvar, "block_diag_factor")) return kErrArg;
+  if (N < 1 || !dq || !coef || !lu || !piv || !singular) {
+    set_error("block_diag_factor: empty grid or null argument");
+    return kErrArg;
+  }
+  BlockIdx bi;
+  for (int k = 0; k < kMaxVar * kMaxVar; ++k) bi.q[k] = -1;
+  for (int o = 0; o < nvar; ++o)
+    for (int i = 0; i < nvar; ++i) {
+      const int q = dq[o * nvar + i];
+      if (q < -1 || q >= nent) {
+        set_error("block_diag_factor: block (%d, %d) entry %d outside "
+                  "[-1, %d)", o, i, q, nent);
+        return kErrArg;
+      }
+      bi.q[o * kMaxVar + i] = q;
+    }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define ODIL_BF(T, NV)                                                       \
+  hipLaunchKernelGGL((k_block_factor<T, NV>), dim3(nblk(N)), dim3(256), 0, s, \
+                     bi, static_cast<const T*>(coef), static_cast<T*>(lu),   \
+                     piv, singular, (size_t)N)
+#define ODIL_BF_NV(T)           \
+  switch (nvar) {               \
+    case 1: ODIL_BF(T, 1); break; \
+    case 2: ODIL_BF(T, 2); break; \
+    case 3: ODIL_BF(T, 3); break; \
+    default: ODIL_BF(T, 4); break; \
+  }
+  if (dtype == kF64) {
+    ODIL_BF_NV(double);
+  } else if (dtype == kF32) {
+    ODIL_BF_NV(float);
+  } else {
+    set_error("block_diag_factor: bad dtype %d", dtype);
+    return kErrArg;
+  }
+#undef ODIL_BF_NV
+#undef ODIL_BF
+  return check_launch("block_diag_factor");
+}
+
+// u += omega D^-1 r per cell with the factors of odil_block_diag_factor
+// (r = f - A u computed beforehand, e.g. by odil_stencil_apply mode 1).
+int odil_block_jacobi(int dtype, int nvar, int N, const void* lu,
+                      const int* piv, const void* const* r, void* const* u,
+                      double omega, void* stream) {
+  if (bad_var(nvar, "block_jacobi")) return kErrArg;
+  if (N < 1 || !lu || !piv || !r || !u) {
+    set_error("block_jacobi: empty grid or null argument");
+    return kErrArg;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define ODIL_BJ(T, NV)                                                        \
+  hipLaunchKernelGGL((k_block_jacobi<T, NV>), dim3(nblk(N)), dim3(256), 0, s, \
+                     static_cast<const T*>(lu), piv, cvp<T>(r, nvar),         \
+                     vp<T>(u, nvar), (size_t)N, (T)omega)
+#define ODIL_BJ_NV(T)           \
+  switch (nvar) {               \
+    case 1: ODIL_BJ(T, 1); break; \
+    case 2: ODIL_BJ(T, 2); break; \
+    case 3: ODIL_BJ(T, 3); break; \
+    default: ODIL_BJ(T, 4); break; \
+  }
+  if (dtype == kF64) {
+    ODIL_BJ_NV(double);
+  } else if (dtype == kF32) {
+    ODIL_BJ_NV(float);
+  } else {
+    set_error("block_jacobi: bad dtype %d", dtype);
+    return kErrArg;
+  }
+#undef ODIL_BJ_NV
+#undef ODIL_BJ
+  return check_launch("block_jacobi");
+}
+
+// One forward Gauss-Seidel sweep of a square stride-1 operator in full
+// (non-symmetric) storage, u updated in place, x (nvar fields of N values)
+// working storage. `ent` is the device entry table and `ent_host` the same
+// nent x 5 table in host memory (sorted by output field), from which the
+// wave levels are derived.
+int odil_stencil_gauss_seidel(int dtype, int ndim, const int* nw, int nvar,
+                              const int* ent, const int* ent_host, int nent,
+                              const void* coef, void* const* u,
+                              const void* const* f, void* const* x,
+                              double omega, void* stream) {
+  if (bad_dim(ndim, nw, "gauss_seidel") || bad_var(nvar, "gauss_seidel"))
+    return kErrArg;
+  if (!ent || !ent_host || nent < 1 || !coef || !u || !f || !x) {
+    set_error("gauss_seidel: null argument or empty operator");
+    return kErrArg;
+  }
+  int ostart[kMaxVar + 1] = {0};
+  int qd[kMaxVar];
+  for (int o = 0; o < nvar; ++o) qd[o] = -1;
+  int a1 = 1, a0 = 1;
+  for (int e = 0; e < nent; ++e) {
+    const int* it = ent_host + 5 * (size_t)e;
+    const int o = it[0], in = it[1];
+    if (o < 0 || o >= nvar || in < 0 || in >= nvar ||
+        (e > 0 && o < ent_host[5 * (size_t)(e - 1)])) {
+      set_error("gauss_seidel: entry %d: fields (%d, %d) out of range or "
+                "entries not sorted by output field", e, o, in);
+      return kErrArg;
+    }
+    if (ndim == 2 && it[4] != 0) {
+      set_error("gauss_seidel: entry %d has a third shift on a 2D grid", e);
+      return kErrArg;
+    }
+    ostart[o + 1] = e + 1;
+    const int s0 = it[2], s1 = it[3], s2 = it[4];
+    if (o == in && s0 == 0 && s1 == 0 && s2 == 0) qd[o] = e;
+    // Level weights: a1 |s1| >= 1 + s2 for s = (0, s1 < 0, s2).
+    if (o == in && s0 == 0 && s1 < 0)
+      a1 = std::max(a1, ceil_div(1 + s2, -s1));
+  }
+  for (int e = 0; e < nent; ++e) {
+    const int* it = ent_host + 5 * (size_t)e;
+    const int s0 = it[2], s1 = it[3], s2 = it[4];
+    // a0 |s0| >= 1 + a1 s1 + s2 for s = (s0 < 0, s1, s2).
+    if (it[0] == it[1] && s0 < 0) {
+      const int need = 1 + a1 * s1 + s2;
+      if (need > 0) a0 = std::max(a0, ceil_div(need, -s0));
+    }
+  }
+  for (int o = 0; o < nvar; ++o) {
+    if (qd[o] < 0) {
+      set_error("gauss_seidel: field %d has no diagonal entry", o);
+      return kErrArg;
+    }
+    if (ostart[o + 1] < ostart[o]) ostart[o + 1] = ostart[o];
+  }
+  Grid3 g = mk_grid(ndim, nw);
+  const size_t N = g.size();
+  const int span = a1 * (g.n1 - 1) + (g.n2 - 1);
+  const int nwave = a0 * (g.n0 - 1) + span + 1;
+  const int row = g.n2 == 1 ? 1 : g.n1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define ODIL_GS(T)                                                            \
+  for (int o = 0; o < nvar; ++o)                                              \
+    for (int w = 0; w < nwave; ++w) {                                         \
+      const int lo = std::max(0, ceil_div(std::max(w - span, 0), a0));        \
+      const int hi = std::min(g.n0 - 1, w / a0);                              \
+      if (hi < lo) continue;                                                  \
+      const int cnt = (hi - lo + 1) * row;                                    \
+      hipLaunchKernelGGL((k_gs_wave<T>), dim3(nblk(cnt)), dim3(256), 0, s,    \
+                         ent, ostart[o], ostart[o + 1], qd[o],                \
+                         static_cast<const T*>(coef),                         \
+                         cvp<T>((const void* const*)u, nvar), vp<T>(x, nvar), \
+                         static_cast<const T*>(f[o]), g, o, w, a0, a1, lo,    \
+                         cnt);                                                \
+    }                                                                         \
+  hipLaunchKernelGGL((k_relax<T>), dim3(nblk(N), nvar), dim3(256), 0, s,      \
+                     cvp<T>((const void* const*)x, nvar), vp<T>(u, nvar), N,  \
+                     (T)omega)
+  if (dtype == kF64) {
+    ODIL_GS(double);
+  } else if (dtype == kF32) {
+    ODIL_GS(float);
+  } else {
+    set_error("gauss_seidel: bad dtype %d", dtype);
+    return kErrArg;
+  }
+#undef ODIL_GS
+  return check_launch("gauss_seidel");
 }
 
 }  // extern "C"

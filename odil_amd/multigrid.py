@@ -1208,9 +1208,16 @@ class Multigrid:
     smoothing, transfers and the coarsest solve (a dense inverse applied by
     a GEMV kernel) run in the HIP kernels of `MultigridOp`. Like the
     reference, `step` passes `ndirect` to the coarser levels but not `pre`
-    and `post` (multigrid.py:331-332). Node grids of odd size only; the
-    block-diagonal (`full=True`) and Gauss-Seidel smoothers are not
-    provided (the Newton driver uses `full=False`, util.py:471-473).
+    and `post` (multigrid.py:331-332). Node grids of odd size only.
+
+    The three smoothers of the reference: point Jacobi, block-diagonal
+    Jacobi (`full=True`, the per-cell nvar x nvar blocks factored once per
+    `update_A`, an exactly singular block raising numpy's LinAlgError like
+    `numpy.linalg.solve`) and forward Gauss-Seidel in the flat numbering
+    (lexicographic within a field, fields in order), swept in wavefronts of
+    independent cells, one launch per wavefront: a sequential method, about
+    nvar (3 n0 + n1) launches per sweep on a 2D grid, meant for small
+    levels and reference checks rather than the Newton hot path.
     """
 
     def __init__(self, nw, nvar=1, nlevels=None, restriction='full',
@@ -1225,21 +1232,60 @@ class Multigrid:
         self.dim = len(self.nnw[0])
         self.restriction = restriction
         self.AA = None
+        self._buf = {}
+        self._factors = {}
+        self._gs_tab = {}
+        self._levels_given = False
 
-    def update_A(self, A0):
+    def update_A(self, A0, DD=None, compute_lower=False):
+        """Set the operator (multigrid.py:116-157): the finest matrix (SciPy
+        sparse or `BlockStencil`), coarsened by R A T, or a list of one
+        matrix per level, used as given. `DD` is recomputed from the
+        matrices, as in the reference; `compute_lower` is accepted and not
+        needed (Gauss-Seidel reads the lower triangle from the stored
+        operator)."""
+        self._factors = {}
+        self._gs_tab = {}
         if isinstance(A0, list):
-            raise ValueError('Multigrid.update_A: a list of per-level '
-                             'matrices is not supported; pass the finest')
-        if not isinstance(A0, BlockStencil):
-            A0 = BlockStencil.from_csr(A0, self.nnw[0], self.nvar,
-                                       device=self._mg.device)
-        if any(abs(x) > 2 for _, _, sh in A0.entries for x in sh):
+            if len(A0) != self.nlevels:
+                raise ValueError('Expected len(A0)={:}, got {:}'.format(
+                    self.nlevels, len(A0)))
+            AA = [self._stencil(a, level) for level, a in enumerate(A0)]
+            self._mg.AA = AA
+            self._mg._coarse_inv = {}
+            self._mg._tune()
+            self._levels_given = True
+        else:
+            A0 = self._stencil(A0, 0)
+            if self._levels_given:
+                self._mg.AA = None  # rebuild the Galerkin levels
+                self._levels_given = False
+            self._mg.update_A(A0)
+        self.AA = self._mg.AA
+        return self
+
+    def _stencil(self, A, level):
+        if not isinstance(A, BlockStencil):
+            A = BlockStencil.from_csr(A, self.nnw[level], self.nvar,
+                                      device=self._mg.device)
+        if A.nw != tuple(self.nnw[level]) or A.nvar_in != self.nvar:
+            raise ValueError('level {} operator grid/fields do not match '
+                             '{} x {}'.format(level, self.nvar,
+                                              self.nnw[level]))
+        if any(abs(x) > 2 for _, _, sh in A.entries for x in sh):
             raise ValueError('Multigrid.update_A: couplings beyond 2 nodes '
                              '(e.g. a periodic wrap) are not supported by '
                              'the stencil Galerkin product')
-        self._mg.update_A(A0)
-        self.AA = self._mg.AA
-        return self
+        return A.astype(torch.float64)
+
+    def _work(self, level, key):
+        k = (level, key)
+        if k not in self._buf:
+            self._buf[k] = [torch.empty(tuple(self.nnw[level]),
+                                        dtype=torch.float64,
+                                        device=self._mg.device)
+                            for _ in range(self.nvar)]
+        return self._buf[k]
 
     def _fields(self, v, level):
         n = (self.nvar, ) + tuple(self.nnw[level])
@@ -1250,12 +1296,68 @@ class Multigrid:
         return list(v.view(n))
 
     def smoother_jacobi(self, level, u, f, omega=1., full=False):
-        """u += omega (f - A u) / diag(A) in place (multigrid.py:284-299)."""
-        if full:
-            raise ValueError('Multigrid.smoother_jacobi: full=True '
-                             '(block-diagonal) is not provided')
-        self._mg.smoother_jacobi(level, self._fields(u, level),
-                                 self._fields(f, level), omega)
+        """u += omega D^-1 (f - A u) in place (multigrid.py:284-299): D the
+        diagonal of the diagonal blocks, or with full=True the per-cell
+        nvar x nvar matrix of every block's diagonal."""
+        uu, ff = self._fields(u, level), self._fields(f, level)
+        if not full:
+            self._mg.smoother_jacobi(level, uu, ff, omega)
+            return u
+        A = self._mg.AA[level]
+        r = A.residual(uu, ff, self._work(level, 'r'))
+        lu, piv = self._block_factors(level)
+        st = _lib.lib().odil_block_jacobi(
+            c_int(_lib.F64), c_int(self.nvar), c_int(uu[0].numel()), ptr(lu),
+            ptr(piv), ptrs(r), ptrs(uu), c_double(omega), stream())
+        check(st, 'block_jacobi')
+        return u
+
+    def _block_factors(self, level):
+        if level in self._factors:
+            return self._factors[level]
+        A = self._mg.AA[level]
+        nv = self.nvar
+        zero = (0, ) * self.dim
+        at = {(o, i): q for q, (o, i, s) in enumerate(A.entries) if s == zero}
+        dq = []
+        for o in range(nv):
+            for i in range(nv):
+                q = at.get((o, i))
+                if q is None and A.sym:
+                    q = at.get((i, o))  # mirror at shift 0: same cell
+                dq.append(-1 if q is None else q)
+        N = int(np.prod(self.nnw[level]))
+        dev = self._mg.device
+        lu = torch.empty((nv * nv, N), dtype=torch.float64, device=dev)
+        piv = torch.empty((nv, N), dtype=torch.int32, device=dev)
+        flag = torch.zeros(1, dtype=torch.int32, device=dev)
+        st = _lib.lib().odil_block_diag_factor(
+            c_int(_lib.F64), c_int(nv), c_int(N), (ctypes.c_int * len(dq))(*dq),
+            c_int(len(A.entries)), ptr(A.coef), ptr(lu), ptr(piv), ptr(flag), stream())
+        check(st, 'block_diag_factor')
+        if int(flag.item()):
+            raise np.linalg.LinAlgError(
+                'Singular matrix (a diagonal block on level {})'.format(level))
+        self._factors[level] = (lu, piv)
+        return lu, piv
+
+    def smoother_gauss_seidel(self, level, u, f, omega=1.):
+        """u += omega (L^-1 (f - (A - L) u) - u) in place, L the lower
+        triangle of A with its diagonal (multigrid.py:301-305)."""
+        A = self._mg.AA[level]
+        if A.sym:
+            raise ValueError('Gauss-Seidel needs the operator in full '
+                             '(non-symmetric) storage')
+        uu, ff = self._fields(u, level), self._fields(f, level)
+        if level not in self._gs_tab:
+            ent = [v for o, i, s in A.entries for v in (o, i) + _shift3(s)]
+            self._gs_tab[level] = (ctypes.c_int * len(ent))(*ent)
+        st = _lib.lib().odil_stencil_gauss_seidel(
+            c_int(_lib.F64), c_int(self.dim), _grid_arr(A.nw),
+            c_int(self.nvar), ptr(A.t_ent), self._gs_tab[level],
+            c_int(len(A.entries)), ptr(A.coef), ptrs(uu), ptrs(ff),
+            ptrs(self._work(level, 'x')), c_double(omega), stream())
+        check(st, 'gauss_seidel')
         return u
 
     def smoother_direct(self, level, u, f):

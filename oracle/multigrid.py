@@ -317,10 +317,13 @@ def coeffs_from_tape(grads, field_desc, neqn, nvar):
 class Multigrid:
     """The sparse-matrix `Multigrid` (`multigrid.py:72-339`): block-diagonal
     R and T over nvar fields, coarse matrices R A T by sparse products
-    (`update_A`, `:116-157`), point-Jacobi smoother on the diagonal blocks
-    (`:284-299`, full=False), sparse direct coarsest solve (`:307-310`) and
-    the V-cycle whose recursion forwards `ndirect` but not `pre`/`post`
-    (`:312-339`). Vectors are flat, fields concatenated."""
+    (`update_A`, `:116-157`), Jacobi smoother on the diagonal blocks
+    (`:284-299`: full=False point Jacobi, full=True a dense nvar x nvar
+    solve per cell with every block's diagonal), Gauss-Seidel with the
+    lower triangle (`:301-305`, SciPy's sparse triangular solve), sparse
+    direct coarsest solve (`:307-310`) and the V-cycle whose recursion
+    forwards `ndirect` but not `pre`/`post` (`:312-339`). Vectors are flat,
+    fields concatenated."""
 
     def __init__(self, nw, nvar=1, nlevels=None, restriction='full'):
         nw = np.array(nw)
@@ -346,10 +349,32 @@ class Multigrid:
                        self.TT[level - 1]).tocsr())
         self.AA = AA
         self.DD = [A.diagonal() for A in AA]
+        # extdiag (`:133-140`): diagonal of block (i, j), shape (nv, nv, n).
+        nv = self.nvar
+        self.DB = []
+        for A in AA:
+            ns = A.shape[0] // nv
+            self.DB.append(np.array([[
+                A[i * ns:(i + 1) * ns, j * ns:(j + 1) * ns].diagonal(0)
+                for j in range(nv)] for i in range(nv)]))
+        # tril including the diagonal (`:148-149`).
+        self.LL = [scipy.sparse.tril(A).tocsr() for A in AA]
 
-    def smoother_jacobi(self, level, u, f, omega=1.):
+    def smoother_jacobi(self, level, u, f, omega=1., full=False):
         r = f - self.AA[level] @ u
-        u += r / self.DD[level] * omega
+        if full:
+            d = np.moveaxis(self.DB[level], 2, 0)  # (n, nv, nv)
+            rr = r.reshape(self.nvar, -1).T[..., None]  # (n, nv, 1)
+            du = np.linalg.solve(d, rr)[..., 0].T.ravel()
+        else:
+            du = r / self.DD[level]
+        u += du * omega
+        return u
+
+    def smoother_gauss_seidel(self, level, u, f, omega=1.):
+        A, L = self.AA[level], self.LL[level]
+        x = scipy.sparse.linalg.spsolve_triangular(L, f - (A @ u - L @ u))
+        u += (x - u) * omega
         return u
 
     def smoother_direct(self, level, u, f):

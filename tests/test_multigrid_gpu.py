@@ -539,11 +539,104 @@ def test_sparse_multigrid_rejects(dev):
         _poisson_dirichlet17().tocsr())
     u = torch.zeros(n * n, dtype=torch.float64, device=dev)
     with pytest.raises(ValueError):
-        mg.smoother_jacobi(0, u, u.clone(), 0.7, full=True)
-    with pytest.raises(ValueError):
         mg.smoother_jacobi(0, u[:-1], u[:-1], 0.7)
+    with pytest.raises(ValueError):
+        mg.smoother_gauss_seidel(0, u[:-1], u[:-1])
+    with pytest.raises(ValueError):  # a list needs one matrix per level
+        mg.update_A([_poisson_dirichlet17().tocsr()])
     # A periodic wrap (coupling node 0 to node n - 1 along x) is refused.
     per = scipy.sparse.eye(n * n, format='lil')
     per[0, (n - 1) * n] = 1.
     with pytest.raises(ValueError):
         Multigrid((n, n), device=dev).update_A(per.tocsr())
+
+
+@pytest.mark.parametrize('case', ['poisson', 'cavity'])
+@pytest.mark.parametrize('smoother', ['block', 'gauss_seidel'])
+def test_sparse_multigrid_smoothers_match_oracle(dev, case, smoother):
+    """V-cycles of the GPU `Multigrid` with the block-diagonal Jacobi
+    (full=True) or the Gauss-Seidel smoother equal the oracle's (dense
+    per-cell solves, SciPy triangular solve) to round-off."""
+    from odil_amd.multigrid import Multigrid
+    n = 17
+    if case == 'poisson':
+        A = _poisson_dirichlet17().tocsr()
+        f = np.full(n * n, 10.)
+        nvar, nlv, omega, ndirect = 1, None, 0.7, 3
+    else:
+        from tests.test_oracle import _cavity17_normal
+        blocks, rhs = _cavity17_normal(1e-2)
+        A = scipy.sparse.bmat([[blocks[i][j].tocsr() for j in range(3)]
+                               for i in range(3)]).tocsr()
+        f = np.concatenate(rhs)
+        nvar, nlv, omega, ndirect = 3, 4, 0.6, 3
+    if smoother == 'gauss_seidel':
+        omega = 1.
+    S = omg.Multigrid((n, n), nvar=nvar, nlevels=nlv)
+    S.update_A(A)
+    mg = Multigrid((n, n), nvar=nvar, nlevels=nlv, device=dev)
+    mg.update_A(A, compute_lower=True)
+    if smoother == 'block':
+        sm_g = lambda l, a, b: mg.smoother_jacobi(l, a, b, omega, full=True)
+        sm_o = lambda l, a, b: S.smoother_jacobi(l, a, b, omega, full=True)
+    else:
+        sm_g = lambda l, a, b: mg.smoother_gauss_seidel(l, a, b, omega)
+        sm_o = lambda l, a, b: S.smoother_gauss_seidel(l, a, b, omega)
+    ft = torch.tensor(f, device=dev)
+    u = torch.zeros_like(ft)
+    v = np.zeros_like(f)
+    for _ in range(3):
+        u = mg.step(u, ft, sm_g, ndirect=ndirect)
+        v = S.step(v, f, sm_o, ndirect=ndirect)
+        got = u.cpu().numpy()
+        assert np.abs(got - v).max() <= 1e-10 * np.abs(v).max()
+
+
+@pytest.mark.parametrize('nw,nvar,reach', [((9, 11), 2, 2), ((17, 9), 3, 1),
+                                           ((5, 7, 9), 2, 2),
+                                           ((7, 5, 5), 1, 1)])
+def test_smoother_sweeps_random_operators(dev, nw, nvar, reach):
+    """Single sweeps on random non-symmetric block stencil matrices (all
+    shifts up to `reach`, cross-field blocks, 2D and 3D): Gauss-Seidel
+    (omega 1 and 0.8) and block Jacobi equal the oracle's sweeps, which
+    the CPU tests pin to the sequential loop and the exact block solve."""
+    from odil_amd.multigrid import Multigrid
+    from tests.test_oracle import _random_block_csr
+    A = _random_block_csr(nw, nvar, reach, seed=11)
+    rng = np.random.default_rng(12)
+    u0 = rng.standard_normal(A.shape[0])
+    f = rng.standard_normal(A.shape[0])
+    S = omg.Multigrid(nw, nvar=nvar, nlevels=1)
+    S.update_A(A)
+    mg = Multigrid(nw, nvar=nvar, nlevels=1, device=dev)
+    mg.update_A(A)
+    t = lambda a: torch.tensor(a, device=dev)
+    for omega in (1., 0.8):
+        ref = S.smoother_gauss_seidel(0, u0.copy(), f, omega)
+        u = t(u0)
+        out = mg.smoother_gauss_seidel(0, u, t(f), omega)
+        assert out is u
+        got = u.cpu().numpy()
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+        ref = S.smoother_jacobi(0, u0.copy(), f, omega, full=True)
+        u = t(u0)
+        mg.smoother_jacobi(0, u, t(f), omega, full=True)
+        got = u.cpu().numpy()
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_block_jacobi_singular_block_raises(dev):
+    """An exactly singular per-cell block raises numpy's LinAlgError, as
+    numpy.linalg.solve does in the reference's full=True sweep."""
+    from odil_amd.multigrid import Multigrid
+    from tests.test_oracle import _random_block_csr
+    nw = (5, 7)
+    N = 35
+    Z = _random_block_csr(nw, 3, 1, seed=5).tolil()
+    for k in range(3):  # block row of field 1 at cell 4
+        Z[N + 4, k * N + 4] = 0.
+    mg = Multigrid(nw, nvar=3, nlevels=1, device=dev)
+    mg.update_A(Z.tocsr())
+    u = torch.zeros(3 * N, dtype=torch.float64, device=dev)
+    with pytest.raises(np.linalg.LinAlgError):
+        mg.smoother_jacobi(0, u, torch.ones_like(u), 1., full=True)

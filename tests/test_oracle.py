@@ -288,6 +288,94 @@ def test_sparse_multigrid_equals_stencil_multigrid():
             assert np.abs(v - ref).max() <= 1e-10 * np.abs(ref).max()
 
 
+def _random_block_csr(nw, nvar, reach, seed, shift0_only=False):
+    """Random square block stencil matrix on node grid `nw` (fields
+    concatenated, C order): random couplings for every shift within `reach`
+    (about 70 % of the blocks present), made diagonally dominant."""
+    import itertools
+    rng = np.random.default_rng(seed)
+    nw = tuple(nw)
+    N = int(np.prod(nw))
+    idx = np.arange(N).reshape(nw)
+    zero = (0, ) * len(nw)
+    shifts = [zero] if shift0_only else list(
+        itertools.product(range(-reach, reach + 1), repeat=len(nw)))
+    rows, cols, vals = [], [], []
+    for o in range(nvar):
+        for i in range(nvar):
+            for s in shifts:
+                if rng.random() < 0.3 and not (o == i and s == zero):
+                    continue
+                c = rng.uniform(-1, 1, nw)
+                src = tuple(slice(max(0, -d), n - max(0, d))
+                            for d, n in zip(s, nw))
+                dst = tuple(slice(max(0, d), n - max(0, -d))
+                            for d, n in zip(s, nw))
+                rows.append(o * N + idx[src].ravel())
+                cols.append(i * N + idx[dst].ravel())
+                vals.append(c[src].ravel())
+    A = scipy.sparse.coo_matrix(
+        (np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+        shape=(nvar * N, nvar * N)).tocsr()
+    dom = np.asarray(abs(A).sum(axis=1)).ravel() + 1.
+    return (A + scipy.sparse.diags(dom)).tocsr()
+
+
+def test_gauss_seidel_oracle_is_the_sequential_sweep():
+    """The oracle's Gauss-Seidel (spsolve_triangular of the lower triangle,
+    multigrid.py:301-305) equals the textbook forward sweep written as a
+    loop over unknowns in flat order: x_k = (f_k - sum_(j<k) A_kj x_j -
+    sum_(j>k) A_kj u_j) / A_kk, then u + omega (x - u). A 2-field
+    non-symmetric 5x5-stencil matrix, so a transposed triangle, a wrong
+    field order or a dropped coupling changes the result."""
+    nw = (5, 7)
+    A = _random_block_csr(nw, 2, 2, seed=3)
+    rng = np.random.default_rng(4)
+    u0, f = rng.standard_normal(A.shape[0]), rng.standard_normal(A.shape[0])
+    S = omg.Multigrid(nw, nvar=2, nlevels=1)
+    S.update_A(A)
+    for omega in (1., 0.8):
+        got = S.smoother_gauss_seidel(0, u0.copy(), f, omega)
+        D = A.toarray()
+        x = np.zeros_like(u0)
+        for k in range(len(u0)):
+            t = f[k] - D[k, :k] @ x[:k] - D[k, k + 1:] @ u0[k + 1:]
+            x[k] = t / D[k, k]
+        assert np.allclose(got, u0 + omega * (x - u0), rtol=1e-13, atol=0)
+
+
+def test_block_jacobi_oracle_closed_forms():
+    """full=True Jacobi (multigrid.py:289-294): on a matrix whose couplings
+    are all at shift 0 (a block diagonal of per-cell nvar x nvar blocks)
+    one sweep with omega = 1 is the exact solve A^-1 f from any start; with
+    one field it is point Jacobi; a cell whose block is singular raises."""
+    nw = (5, 7)
+    A = _random_block_csr(nw, 3, 0, seed=5, shift0_only=True)
+    rng = np.random.default_rng(6)
+    f = rng.standard_normal(A.shape[0])
+    S = omg.Multigrid(nw, nvar=3, nlevels=1)
+    S.update_A(A)
+    got = S.smoother_jacobi(0, rng.standard_normal(A.shape[0]), f, 1.,
+                            full=True)
+    ref = scipy.sparse.linalg.spsolve(A.tocsc(), f)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-14)
+    A1 = _random_block_csr(nw, 1, 2, seed=7)
+    S1 = omg.Multigrid(nw, nvar=1, nlevels=1)
+    S1.update_A(A1)
+    u0 = rng.standard_normal(A1.shape[0])
+    f1 = rng.standard_normal(A1.shape[0])
+    a = S1.smoother_jacobi(0, u0.copy(), f1, 0.7, full=True)
+    b = S1.smoother_jacobi(0, u0.copy(), f1, 0.7)
+    assert np.allclose(a, b, rtol=1e-14, atol=1e-15)
+    Z = A.tolil()
+    N = int(np.prod(nw))
+    for k in range(3):  # cell 4: block row of field 1 is zero
+        Z[N + 4, k * N + 4] = 0.
+    S.update_A(Z.tocsr())
+    with pytest.raises(np.linalg.LinAlgError):
+        S.smoother_jacobi(0, np.zeros(A.shape[0]), f, 1., full=True)
+
+
 def test_poisson_multigrid_converges():
     """Acceptance of the reference's test_multigrid_poisson
     (test_multigrid.py:251-368): 10 V-cycles, |rhs - A u| <= 1e-6."""
