@@ -158,8 +158,10 @@ __device__ inline void stage_tile(const FieldPtrs<T, NF>& in, T* tile,
 // along axis 1), and the partial-sum counts every operator's *_nblocks
 // reports: one partial per wavefront (4 per 256-thread workgroup).
 // (On the 1025^2 cavity 16 x 32 (FP32) and 8 x 32 (FP64) were the fastest
-// loss-adjoint tiles of 4..32 x 32..128.)
-constexpr int kResTx = 16, kResTy = 64;
+// loss-adjoint tiles of 4..32 x 32..128; residual tiles of 4, 8, 16 and 32
+// rows took 16.7, 15.7, 16.9 and 21.3 us in FP32 and 22.0, 21.2, 23.2 and
+// 31.6 us in FP64, inputs and outputs HBM-resident.)
+constexpr int kResTx = 8, kResTy = 64;
 constexpr int kGradTx32 = 16, kGradTx64 = 8, kGradTy = 32;
 // LDS staging of the matrix-free normal operator (k_normal_apply STG),
 // measured on the 1025^2 cavity with HBM-resident inputs: FP32 state and x
