@@ -194,7 +194,11 @@ def test_vcycle_poisson_golden(dev):
 
 def test_vcycle_cavity_golden(dev):
     """Reference V-cycle iterates on the cavity normal equations (17x17, 4
-    levels, reference Galerkin). u and v match; p up to its null constant."""
+    levels, reference Galerkin). u and v match; p up to its null constant.
+    The reference solves the coarsest level exactly (spsolve); there the
+    27 x 27 system is non-singular (smallest singular value 5.75, condition
+    562), so the exact inverse is used: a Tikhonov shift of 1e-15 max
+    diag(M^T M) would move the answer by ~3e-10 relative."""
     from odil_amd.multigrid import BlockStencil, MultigridOp, normal_operator
     from odil_amd.operators import Cavity
     g = np.load(os.path.join(GOLD, 'mg_vcycle.npz'))
@@ -206,7 +210,7 @@ def test_vcycle_cavity_golden(dev):
         _flat(rhs), g['cavity17_rhs'].ravel(), rtol=0,
         atol=1e-11 * np.abs(g['cavity17_rhs']).max())
     mg = MultigridOp((17, 17), nvar=3, nlevels=4, device=dev,
-                     coarse_reg=1e-15).update_A(normal_operator(J))
+                     coarse_reg=0.).update_A(normal_operator(J))
     sol = [torch.zeros(17, 17, dtype=torch.float64, device=dev)
            for _ in range(3)]
     sol2 = [x.clone() for x in sol]
@@ -219,7 +223,7 @@ def test_vcycle_cavity_golden(dev):
                 d = got[k].cpu().numpy().ravel() - ref[k]
                 if k == 2:
                     d = d - d.mean()
-                assert np.abs(d).max() <= 1e-9 * np.abs(ref[k]).max()
+                assert np.abs(d).max() <= 1e-10 * np.abs(ref[k]).max()
 
 
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
@@ -244,7 +248,13 @@ def test_poisson_convergence(dev, dtype):
 def test_vcycle_cavity_65_matches_oracle(dev, mode):
     """Two V-cycles on the zero-state cavity normal equations, 65x65, five
     levels: GPU equals the oracle (u, v; p up to its constant). Both use the
-    same Tikhonov least-squares coarsest solve."""
+    same Tikhonov least-squares coarsest solve. Tolerance: 1e-10 for the
+    reference hierarchy. The 'exact' hierarchy's 75 x 75 coarsest system is
+    singular (constant pressure) with condition 2.2e4 on the rest; the
+    Tikhonov normal equations square that (4.9e8), so two backward-stable
+    FP64 solves (LAPACK LU in the oracle, the blocked Cholesky here) may
+    differ by up to ~kappa^2 eps = 1e-7 relative in the coarse correction
+    (observed 5e-10): asserted at 1e-9."""
     from odil_amd.multigrid import BlockStencil, MultigridOp, normal_operator
     from odil_amd.operators import Cavity
     n = 65
@@ -276,7 +286,8 @@ def test_vcycle_cavity_65_matches_oracle(dev, mode):
             d = x[k].cpu().numpy().ravel() - sol[k]
             if k == 2:
                 d = d - d.mean()
-            assert np.abs(d).max() <= 1e-9 * np.abs(sol[k]).max(), (it, k)
+            tol = 1e-10 if mode == 'reference' else 1e-9
+            assert np.abs(d).max() <= tol * np.abs(sol[k]).max(), (it, k)
 
 
 # -- symmetric (upper-half) storage ------------------------------------------
@@ -443,7 +454,7 @@ def test_sym_vcycle_matches_full(dev):
         # round-off and is compared modulo the mean.
         u[2] = u[2] - u[2].mean()
         out.append(_flat(u))
-    assert np.abs(out[0] - out[1]).max() <= 1e-9 * np.abs(out[0]).max()
+    assert np.abs(out[0] - out[1]).max() <= 1e-10 * np.abs(out[0]).max()
 
 
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
