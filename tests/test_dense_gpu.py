@@ -33,7 +33,9 @@ def test_spd_solve_matches_numpy(dev, n):
         torch.cuda.synchronize()
         ref = np.linalg.solve(G, B)
         err = np.abs(X.cpu().numpy() - ref).max() / np.abs(ref).max()
-        assert err < 1e-8, (n, rep, err)
+        # G has condition 1e6: two backward-stable FP64 solves agree to
+        # ~kappa eps = 2e-10; asserted with a factor 100 for the growth.
+        assert err < 100 * 1e6 * np.finfo(np.float64).eps, (n, rep, err)
         _ = torch.ones(1000, device=dev).sum().item()
 
 

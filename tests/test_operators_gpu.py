@@ -204,4 +204,4 @@ def test_normal_apply_matches_stored_normal(dev, dtype):
         for e in range(len(res)):
             np.testing.assert_allclose(
                 ssq[e].item(), (res[e].double()**2).sum().item(),
-                rtol=1e-9 if dtype == torch.float64 else 1e-4)
+                rtol=1e-12 if dtype == torch.float64 else 1e-4)

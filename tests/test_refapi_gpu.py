@@ -248,17 +248,19 @@ def test_facade_multigridop_krylov_cavity(dev):
                          fieldnames=['u', 'v', 'p'])
     problem = odil.Problem(Cavity((n, n), 0.01), domain)
     state = odil.State()
-    args = odil.util.default_args(epochs=45, nlvl=5, linsolver_maxiter=10)
+    args = odil.util.default_args(epochs=60, nlvl=5, linsolver_maxiter=10)
     res = []
     odil.util.optimize_multigridop(
         args, problem, state,
         lambda packed, epoch, dhistory=None, opt=None: res.append(
             max(opt.last_residual)), krylov=True)
-    assert res[-1] < 1e-11, res[-5:]
+    # Stopped at RMS residual 1e-12, |x - x*| <= |J^-1| |r| allows 1e-10
+    # of max |x*| (as the cavity Newton tests).
+    assert res[-1] < 1e-12, res[-5:]
     d = np.load(os.path.join(os.path.dirname(__file__), 'golden',
                              'cavity_converged_N65_Re100.npz'))
     u = state.fields['u'].cpu().numpy()
-    assert np.abs(u - d['u']).max() <= 1e-9 * np.abs(d['u']).max()
+    assert np.abs(u - d['u']).max() <= 1e-10 * np.abs(d['u']).max()
 
 
 @pytest.mark.parametrize('cell', [False, True])
@@ -303,6 +305,8 @@ def test_multigrid_domain_loss_grad(dev, cell):
         fd = (oracle_loss(uw + eps * v) - oracle_loss(uw - eps * v)) / (
             2 * eps)
         an = float(wgrads['u'][1].cpu().numpy() @ v)
+        # A gradient check, not parity: the loss is quadratic in w, so
+        # central differences are exact up to round-off eps |L| / h.
         np.testing.assert_allclose(an, fd, rtol=1e-8)
     u = torch.tensor(rng.normal(size=(n, n)), device=dev)
     back = domain.multigrid_to_field(domain.field_to_multigrid(u))
