@@ -208,15 +208,12 @@ def level1_sweep_kernel(args, nm):
     for k in range(1, _nrot(args, nbytes)):
         Ak = copy.copy(A)
         Ak.coef = A.coef.clone()
-        if A.coefb is not None:  # the blocked copy the sweep reads
-            Ak.coefb = A.coefb.clone()
         sets.append((Ak, dup(buf['u']), dup(buf['f']), dup(buf['v'])))
     ms = _event_time(lambda k: sets[k][0].jacobi(sets[k][1], sets[k][2],
                                                  sets[k][3], args.omega),
                      args.kernel_reps, not args.no_graph, len(sets))
     return {'ms': ms, 'GBps': nbytes / (ms * 1e-3) / 1e9, 'bytes': nbytes,
             'nent': len(A.entries), 'grid': list(A.nw),
-            'layout': 'blocked' if A.blocked else 'arrays',
             'lanes': A.lanes, 'per': A.per, 'buffer_sets': len(sets)}
 
 

@@ -382,21 +382,6 @@ int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
                            const void* const* x, void* const* y,
                            const void* const* f, int mode, double omega,
                            void* stream);
-/* Blocked coefficient layout of the symmetric sweep (same products and
- * modes as odil_stencil_apply_sym, x in one allocation, lanes 0/1/2/4/8/16):
- * coefb[(b nent + q) 64 + l] = coef_q[64 b + l], ceil(N / 64) nent 64
- * elements written by odil_stencil_pack_blocked from coef [nent][N]. The 64
- * cells of a wavefront read every coefficient array inside one contiguous
- * block instead of nent streams N elements apart. */
-int odil_stencil_pack_blocked(int dtype, int nent, int N, const void* coef,
-                              void* coefb, void* stream);
-int odil_stencil_apply_symb(int dtype, int ndim, const int* nw, int nvar,
-                            const int* items, const int* ostart,
-                            const int* dent, const int* reach, int max_items,
-                            int nent, int lanes, int per, const void* coefb,
-                            const void* const* x, void* const* y,
-                            const void* const* f, int mode, double omega,
-                            void* stream);
 /* y = A^T x or y += A^T x (mul_transpose_field, multigrid.py:1006-1033).
  * `tent` lists entry indices grouped by input field, `istart` delimits. */
 int odil_stencil_apply_t(int dtype, int ndim, const int* nw, int stride,

@@ -16,6 +16,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, 'libodil_amd.so')
 
 F32, F64 = 0, 1
+# ABI of include/odil_amd.h these bindings call (odil_abi_version).
+ABI_VERSION = 2
 _lib = None
 
 c_int = ctypes.c_int
@@ -36,8 +38,15 @@ def lib():
             raise OdilError(
                 'odil_amd native library not built ({}); run '
                 '`python -m odil_amd.build`'.format(LIB_PATH))
-        _lib = ctypes.CDLL(LIB_PATH)
-        _lib.odil_last_error.restype = ctypes.c_char_p
+        handle = ctypes.CDLL(LIB_PATH)
+        handle.odil_last_error.restype = ctypes.c_char_p
+        found = handle.odil_abi_version()
+        if found != ABI_VERSION:
+            raise OdilError(
+                'odil_amd native library {} has ABI {}, these bindings need '
+                '{}; rebuild with `python -m odil_amd.build`'.format(
+                    LIB_PATH, found, ABI_VERSION))
+        _lib = handle
     return _lib
 
 

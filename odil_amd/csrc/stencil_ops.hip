@@ -291,124 +291,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Blocked coefficient layout of the symmetric sweep:
-//     coefb[(b nent + q) 64 + l] = coef_q[64 b + l]   (zero past the grid).
-// The 64 consecutive cells of a wavefront then find every coefficient
-// array as a 64-element run inside one contiguous block of nent x 64
-// elements, instead of nent runs N elements apart. On MI355X a streaming
-// pass over 72 FP32 arrays of 513^2 cells from HBM takes 16.0 us blocked
-// against 32.8 us with one array after the other (82 MB).
-template <class T>
-__global__ void __launch_bounds__(256)
-    k_pack_blocked(const T* __restrict__ coef, T* __restrict__ coefb,
-                   int nent, int N, size_t total) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= total) return;
-  const int l = (int)(i & 63);
-  const size_t bq = i >> 6;
-  const int q = (int)(bq % (size_t)nent);
-  const size_t c = (bq / (size_t)nent) * 64 + l;
-  coefb[i] = c < (size_t)N ? coef[(size_t)q * N + c] : T(0);
-}
-
-// k_apply_sym on the blocked layout, x contiguous (x_k = x_0 + k N). The
-// coefficient of item (q, cs) for cell c sits at element
-//     (c >> 6) nent 64 + ((l + cs) >> 6) nent 64 + ((l + cs) & 63) + 64 q
-// with l = c & 63 (arithmetic shift: cs < 0 reaches back into earlier
-// blocks). Items are staged as (cs, 64 q, x offset); shifts as k_apply_sym.
-template <class T, int LANES, bool B32>
-__global__ void __launch_bounds__(256)
-    k_apply_symb(const int* __restrict__ items, const int* __restrict__ ostart,
-                 const int* __restrict__ dent, const T* __restrict__ coefb,
-                 int nent, CVecPtrs<T> x, VecPtrs<T> y, CVecPtrs<T> f,
-                 Grid3 nw, Grid3 reach, int mode, T omega, int per) {
-  __shared__ int4 fb[kMaxItems];
-  __shared__ int2 shf[kMaxItems];
-  const unsigned g = xcd_block(blockIdx.x + blockIdx.y * gridDim.x,
-                               gridDim.x * gridDim.y);
-  const int o = g % gridDim.y;
-  const unsigned bx = g / gridDim.y;
-  const int i0e = ostart[o];
-  const int ne = ostart[o + 1] - i0e;
-  const int N = (int)nw.size();
-  const int row = nent * 64;
-  constexpr unsigned S = B32 ? sizeof(T) : 1;
-  for (int t = threadIdx.x; t < ne; t += blockDim.x) {
-    const int* it = items + 10 * (i0e + t);
-    fb[t] = make_int4(it[2], it[0] * 64,
-                      (int)((unsigned)(it[1] * N + it[3]) * S), 0);
-    shf[t] = make_int2(pack_shift(it[4], it[5], it[6]),
-                       pack_shift(it[7], it[8], it[9]));
-  }
-  __syncthreads();
-  const T* __restrict__ xb = x.p[0];
-  auto cat = [&](unsigned idx) -> T {
-    return B32 ? ld_b32(coefb, idx * S) : coefb[idx];
-  };
-  auto xat = [&](unsigned cs, int c, int off) -> T {
-    return B32 ? ld_b32(xb, cs + (unsigned)off) : xb[(unsigned)(c + off)];
-  };
-  const int lane = threadIdx.x % LANES;
-  for (int r = 0; r < per; ++r) {
-    const int c = (int)((((size_t)bx * per + r) * blockDim.x + threadIdx.x) /
-                        LANES);
-    if (c >= N) return;  // whole groups exit together; later tiles too
-    const int l = c & 63;
-    const unsigned cb = (unsigned)(c >> 6) * (unsigned)row;
-    T fo = 0, xo = 0, dd = 1;
-    if (lane == 0) {
-      if (mode >= 1) fo = f.p[o][c];
-      if (mode == 2) {
-        xo = x.p[o][c];
-        dd = coefb[cb + (unsigned)(dent[o] * 64 + l)];
-      }
-    }
-    const unsigned cs = (unsigned)c * S;
-    int i0, i1, i2;
-    unflat(c, nw, i0, i1, i2);
-    const bool inner = i0 >= reach.n0 && i0 < nw.n0 - reach.n0 &&
-                       i1 >= reach.n1 && i1 < nw.n1 - reach.n1 &&
-                       i2 >= reach.n2 && i2 < nw.n2 - reach.n2;
-    T acc = 0;
-    if (__all(inner)) {
-#pragma unroll 16
-      for (int e = lane; e < ne; e += LANES) {
-        const int4 it = fb[e];
-        const int t = l + it.x;
-        const unsigned idx =
-            cb + (unsigned)((t >> 6) * row + (t & 63) + it.y);
-        acc += cat(idx) * xat(cs, c, it.z);
-      }
-    } else {
-#pragma unroll 4
-      for (int e = lane; e < ne; e += LANES) {
-        const int4 it = fb[e];
-        const int2 sh = shf[e];
-        const bool ok = shifted_inside(i0, i1, i2, sh.x, nw) &
-                        shifted_inside(i0, i1, i2, sh.y, nw);
-        const int t = l + it.x;
-        const unsigned idx =
-            ok ? cb + (unsigned)((t >> 6) * row + (t & 63) + it.y) : 0u;
-        const T a = cat(idx);
-        const T b = B32 ? ld_b32(xb, ok ? cs + (unsigned)it.z : 0u)
-                        : xb[ok ? (unsigned)(c + it.z) : 0u];
-        acc += ok ? a * b : T(0);
-      }
-    }
-#pragma unroll
-    for (int off = LANES / 2; off > 0; off >>= 1)
-      acc += __shfl_xor(acc, off, LANES);
-    if (lane != 0) continue;
-    if (mode == 0) {
-      y.p[o][c] = acc;
-    } else if (mode == 1) {
-      y.p[o][c] = fo - acc;
-    } else {
-      y.p[o][c] = xo + omega * (fo - acc) / dd;
-    }
-  }
-}
-
 // Two cells per thread for contiguous x (x_k = x_0 + k N), one thread per
 // cell otherwise as k_apply_sym: cells c and c + 256 (the same lane of two
 // consecutive tiles) with their products interleaved, so a wavefront keeps
@@ -1510,132 +1392,6 @@ int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
 #undef ODIL_APPLY2
 #undef ODIL_APPLY3
   return check_launch("stencil_apply_sym");
-}
-
-// coefb [ceil(N / 64)][nent][64] from coef [nent][N] (blocked layout of
-// odil_stencil_apply_symb; elements past the grid are zero).
-int odil_stencil_pack_blocked(int dtype, int nent, int N, const void* coef,
-                              void* coefb, void* stream) {
-  if (nent < 1 || N < 1 || !coef || !coefb) {
-    set_error("stencil_pack_blocked: empty operator or null array");
-    return kErrArg;
-  }
-  const size_t total = (size_t)ceil_div(N, 64) * nent * 64;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const unsigned nb = (unsigned)((total + 255) / 256);
-  if (dtype == kF64)
-    hipLaunchKernelGGL(k_pack_blocked<double>, dim3(nb), dim3(256), 0, s,
-                       static_cast<const double*>(coef),
-                       static_cast<double*>(coefb), nent, N, total);
-  else if (dtype == kF32)
-    hipLaunchKernelGGL(k_pack_blocked<float>, dim3(nb), dim3(256), 0, s,
-                       static_cast<const float*>(coef),
-                       static_cast<float*>(coefb), nent, N, total);
-  else {
-    set_error("stencil_pack_blocked: bad dtype %d", dtype);
-    return kErrArg;
-  }
-  return check_launch("stencil_pack_blocked");
-}
-
-// odil_stencil_apply_sym on the blocked coefficient layout (coefb from
-// odil_stencil_pack_blocked); x must be one allocation (x_k = x_0 + k N);
-// lanes 0 (size-based default), 1, 2, 4, 8 or 16.
-int odil_stencil_apply_symb(int dtype, int ndim, const int* nw, int nvar,
-                            const int* items, const int* ostart,
-                            const int* dent, const int* reach, int max_items,
-                            int nent, int lanes, int per, const void* coefb,
-                            const void* const* x, void* const* y,
-                            const void* const* f, int mode, double omega,
-                            void* stream) {
-  if (bad_dim(ndim, nw, "stencil_apply_symb") ||
-      bad_var(nvar, "stencil_apply_symb"))
-    return kErrArg;
-  if (!items || !ostart || !coefb || !x || !y || nent < 1) {
-    set_error("stencil_apply_symb: null table, coefficient or field array");
-    return kErrArg;
-  }
-  if (mode == 2 && !dent) {
-    set_error("stencil_apply_symb: Jacobi needs the diagonal entries");
-    return kErrArg;
-  }
-  if ((mode == 1 || mode == 2) && !f) {
-    set_error("stencil_apply_symb: mode %d needs f", mode);
-    return kErrArg;
-  }
-  if (per < 1 || per > 64) {
-    set_error("stencil_apply_symb: tiles per workgroup must be in [1, 64]");
-    return kErrArg;
-  }
-  if (max_items < 0 || max_items > kMaxItems) {
-    set_error("stencil_apply_symb: %d products per field exceed %d",
-              max_items, kMaxItems);
-    return kErrCapacity;
-  }
-  if (!reach || reach[0] < 0 || reach[1] < 0 || reach[2] < 0 ||
-      (ndim == 2 && reach[2] != 0) || reach[0] > kMaxShift ||
-      reach[1] > kMaxShift || reach[2] > kMaxShift) {
-    set_error("stencil_apply_symb: bad reach");
-    return kErrArg;
-  }
-  Grid3 g = mk_grid(ndim, nw);
-  Grid3 rc{reach[0], reach[1], reach[2]};
-  const double padded = (double)ceil_div((int)g.size(), 64) * 64.0;
-  if (padded * (double)(nent > nvar ? nent : nvar) > 2.0e9) {
-    set_error("stencil_apply_symb: grid too large for 32-bit offsets");
-    return kErrCapacity;
-  }
-  const size_t esz = dtype == kF64 ? 8 : 4;
-  for (int k = 1; k < nvar; ++k)
-    if (static_cast<const char*>(x[k]) !=
-        static_cast<const char*>(x[0]) + k * g.size() * esz) {
-      set_error("stencil_apply_symb: x must be one allocation");
-      return kErrArg;
-    }
-  if (lanes == 0) {
-    const size_t groups = g.size() * (size_t)nvar;
-    lanes = 1;
-    while (lanes < 16 && groups * lanes < (1u << 20)) lanes *= 2;
-  }
-  if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8 && lanes != 16) {
-    set_error("stencil_apply_symb: lanes must be 0, 1, 2, 4, 8 or 16");
-    return kErrArg;
-  }
-  if (dtype != kF64 && dtype != kF32) {
-    set_error("stencil_apply_symb: bad dtype %d", dtype);
-    return kErrArg;
-  }
-  const bool b32 = padded * nent * esz < 4294967296.0 &&
-                   (double)nvar * g.size() * esz < 4294967296.0;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  dim3 grid(ceil_div((int)nblk(g.size() * lanes), per), nvar);
-#define ODIL_SYMB2(T, L, B)                                                  \
-  hipLaunchKernelGGL((k_apply_symb<T, L, B>), grid, dim3(256), 0, s, items,  \
-                     ostart, dent, static_cast<const T*>(coefb), nent,       \
-                     cvp<T>(x, nvar), vp<T>(y, nvar), cvp<T>(f, nvar), g, rc, \
-                     mode, (T)omega, per)
-#define ODIL_SYMB(T, L)          \
-  if (b32)                       \
-    ODIL_SYMB2(T, L, true);      \
-  else                           \
-    ODIL_SYMB2(T, L, false)
-#define ODIL_SYMB_L(T)              \
-  switch (lanes) {                  \
-    case 1: ODIL_SYMB(T, 1); break; \
-    case 2: ODIL_SYMB(T, 2); break; \
-    case 4: ODIL_SYMB(T, 4); break; \
-    case 8: ODIL_SYMB(T, 8); break; \
-    default: ODIL_SYMB(T, 16); break; \
-  }
-  if (dtype == kF64) {
-    ODIL_SYMB_L(double);
-  } else {
-    ODIL_SYMB_L(float);
-  }
-#undef ODIL_SYMB_L
-#undef ODIL_SYMB
-#undef ODIL_SYMB2
-  return check_launch("stencil_apply_symb");
 }
 
 // y = A^T x (accumulate != 0: y += A^T x). `tent`/`istart` group entry

@@ -54,10 +54,6 @@ class BlockStencil:
         self.sym = bool(sym)
         self.lanes = 0  # threads per cell of the symmetric sweep, 0 = auto
         self.per = 1    # 256-thread tiles per workgroup of that sweep
-        # Symmetric sweep on the blocked coefficient copy `coefb` (kept in
-        # step with coef by pack_blocked; chosen by tune).
-        self.blocked = False
-        self.coefb = None
         self.nw = tuple(int(n) for n in nw)
         self.ndim = len(self.nw)
         self.stride = int(stride)
@@ -178,40 +174,12 @@ class BlockStencil:
         return [torch.empty(shape, dtype=self.dtype, device=self.device)
                 for _ in range(n)]
 
-    def pack_blocked(self):
-        """Refresh `coefb`, the blocked copy of the coefficients
-        (coefb[(b nent + q) 64 + l] = coef_q[64 b + l]) that the blocked
-        symmetric sweep reads; call after every change of coef."""
-        N = int(np.prod(self.nw))
-        nent = len(self.entries)
-        n = -(-N // 64) * nent * 64
-        if (self.coefb is None or self.coefb.numel() != n
-                or self.coefb.dtype != self.dtype):
-            self.coefb = torch.empty(n, dtype=self.dtype, device=self.device)
-        st = _lib.lib().odil_stencil_pack_blocked(
-            c_int(_lib.dtype_tag(self.dtype)), c_int(nent), c_int(N),
-            ptr(self.coef), ptr(self.coefb), stream())
-        check(st, 'stencil_pack_blocked')
-        return self
-
     def _contig(self, x):
         return all(a.data_ptr() == x[0].data_ptr() +
                    k * x[0].numel() * x[0].element_size()
                    for k, a in enumerate(x))
 
     def _call_apply(self, x, y, f, mode, omega):
-        if self.sym and self.blocked and self._contig(x):
-            st = _lib.lib().odil_stencil_apply_symb(
-                c_int(_lib.dtype_tag(self.dtype)), c_int(self.ndim),
-                _grid_arr(self.nw), c_int(self.nvar_out), ptr(self.t_items),
-                ptr(self.t_iostart), ptr(self.t_dent), self._reach,
-                c_int(self._max_items), c_int(len(self.entries)),
-                c_int(max(self.lanes, 0)), c_int(self.per),
-                ptr(self.coefb), ptrs(x), ptrs(y),
-                ptrs(f) if f is not None else None, c_int(mode),
-                c_double(omega), stream())
-            check(st, 'stencil_apply_symb')
-            return
         if self.sym:
             st = _lib.lib().odil_stencil_apply_sym(
                 c_int(_lib.dtype_tag(self.dtype)), c_int(self.ndim),
@@ -279,23 +247,15 @@ class BlockStencil:
         from odil_amd import autotune
         contig = self._contig(x)
         key = ('apply_sym', self.nw, len(self.entries), self._max_items,
-               str(self.dtype), contig, 'blocked')
+               str(self.dtype), contig)
 
         def run(c):
-            self.lanes, self.per = c[:2]
-            self.blocked = len(c) == 3
+            self.lanes, self.per = c
             self._call_apply(x, y, f, 2, omega)
         cand = [(L, p) for L in (0, 1, 2, 4, 8, 16) for p in (1, 2, 4, 8)]
         if contig:
             cand += [(-1, 1), (-2, 1)]
-            cand += [(L, p, 'blocked') for L in (1, 2, 4, 8, 16)
-                     for p in (1, 2, 4, 8)]
-            self.pack_blocked()
-        c = autotune.pick(key, cand, run)
-        self.lanes, self.per = c[:2]
-        self.blocked = len(c) == 3
-        if not self.blocked:
-            self.coefb = None
+        self.lanes, self.per = autotune.pick(key, cand, run)
         return self
 
     def apply_t(self, x, y=None, accumulate=False):
@@ -1008,9 +968,6 @@ class MultigridOp:
                          self.galerkin_mode, out=self.AA[level])
             for level in list(self._coarse_inv):
                 self._coarse_inv[level].copy_(self._invert(level))
-            for A in self.AA:
-                if A.blocked:
-                    A.pack_blocked()
         return self
 
     def _tune(self):

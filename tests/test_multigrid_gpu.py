@@ -376,26 +376,7 @@ def test_sym_apply(dev, nw, nvar, dtype):
             want = xn[k] + 0.6 * (fn[k] - ref[k]) / d
             assert np.abs(yc[k].double().cpu().numpy() - want).max() <= \
                 tol * sc, lanes
-    # The blocked coefficient layout (64-cell blocks of every array; the
-    # grids here end in partial blocks) gives the same sums in the same
-    # order: bitwise equal to the array layout at each launch shape.
-    A.pack_blocked()
-    for L in (1, 2, 4, 8, 16):
-        for p in (1, 3):
-            A.lanes, A.per, A.blocked = L, p, False
-            want = [a.clone() for a in A.apply(xc)]
-            yw = [torch.empty_like(a) for a in xc]
-            A.jacobi(xc, fc, yw, 0.6)
-            rw = A.residual(xc, fc, [torch.empty_like(a) for a in xc])
-            A.blocked = True
-            for a, b in zip(A.apply(xc), want):
-                assert torch.equal(a, b), (L, p)
-            yb = [torch.empty_like(a) for a in xc]
-            A.jacobi(xc, fc, yb, 0.6)
-            rb = A.residual(xc, fc, [torch.empty_like(a) for a in xc])
-            for a, b in zip(yb + rb, yw + rw):
-                assert torch.equal(a, b), (L, p)
-    A.lanes, A.per, A.blocked = 0, 1, False
+    A.lanes, A.per = 0, 1
     # the mirrored CSR and the dense matrix are symmetric and match
     M = A.tocsr()
     assert abs(M - M.T).max() <= 1e-15 * abs(M).max()
@@ -455,42 +436,6 @@ def test_sym_vcycle_matches_full(dev):
         u[2] = u[2] - u[2].mean()
         out.append(_flat(u))
     assert np.abs(out[0] - out[1]).max() <= 1e-10 * np.abs(out[0]).max()
-
-
-@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
-def test_blocked_layout_vcycle_and_refresh(dev, dtype):
-    """V-cycles whose stored levels sweep the blocked coefficient copy
-    equal those on the coefficient arrays, bit for bit, also after
-    update_A refills the hierarchy in place (the copies are repacked)."""
-    from odil_amd.multigrid import MultigridOp, normal_operator
-    _, J, res = _cavity_jac(dev, 65, 65, 4)
-    f = [a.to(dtype) for a in J.apply_t([-r for r in res])]
-    N = normal_operator(J, dtype, sym=True)
-    M = MultigridOp((65, 65), nvar=3, nlevels=4, restriction='transpose',
-                    coarse_reg=1e-12, galerkin_mode='exact', dtype=dtype)
-    M.update_A(N)
-    M.prepare(3)
-
-    def cycles(blocked):
-        for A in M.AA:
-            A.lanes, A.per, A.blocked = 1, 1, blocked
-            if blocked:
-                A.pack_blocked()
-        u = [torch.zeros_like(a) for a in f]
-        for _ in range(2):
-            M.vcycle(u, f, omega=0.5, ndirect=3)
-        return [a.clone() for a in u]
-    for a, b in zip(cycles(False), cycles(True)):
-        assert torch.equal(a, b)
-    # Refill in place with another operator: blocked levels are repacked.
-    N2 = normal_operator(J, dtype, sym=True)
-    N2.coef.mul_(1.5)
-    M.update_A(N2)
-    got = [torch.zeros_like(a) for a in f]
-    for _ in range(2):
-        M.vcycle(got, f, omega=0.5, ndirect=3)
-    for a, b in zip(got, cycles(False)):
-        assert torch.equal(a, b)
 
 
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
