@@ -15,9 +15,10 @@
   so graph size stays bounded as the batch grows, and the graphs replay
   back to back with no host work in between.
 
-Node-based multigrid needs 2^k + 1 points per axis (multigrid.py:362-370),
-so `batch_sizes` draws sizes log-uniformly between 64 and 1024 and snaps
-them to the nearest 2^k + 1.
+`batch_sizes` draws sizes log-uniformly between 64 and 1024. The batched
+loss/adjoint launch takes any sizes (rectangular too); the solver needs
+node grids of 2^k + 1 points per axis (multigrid.py:362-370), so for it the
+sizes are snapped to the nearest 2^k + 1 (`snap=True`).
 """
 
 import ctypes
@@ -32,11 +33,14 @@ from odil_amd.newton import DeviceScalars, FusedNormal, VectorOps
 from odil_amd.operators import Poisson
 
 
-def batch_sizes(count, seed, lo=64, hi=1024):
-    """`count` grid sizes n = round(lo (hi/lo)^U), U ~ U(0, 1), snapped to
-    the nearest 2^k + 1 in log scale."""
+def batch_sizes(count, seed, lo=64, hi=1024, snap=True):
+    """`count` grid sizes n = round(lo (hi/lo)^U), U ~ U(0, 1); with snap,
+    moved to the nearest 2^k + 1 in log scale (what the multigrid solver
+    needs)."""
     rng = np.random.default_rng(seed)
     raw = lo * (hi / lo)**rng.uniform(0, 1, count)
+    if not snap:
+        return [int(round(n)) for n in raw]
     ks = np.arange(int(np.log2(lo)), int(np.log2(hi)) + 1)
     cand = 2.0**ks + 1
     idx = np.abs(np.log(raw[:, None]) - np.log(cand[None, :])).argmin(1)

@@ -207,12 +207,16 @@ def _progress(msg):
 def case_p(count=256, seed=0, solve=True):
     from odil_amd.batched import (PoissonBatch, PoissonBatchSolver,
                                   batch_sizes, gaussian_mixture)
-    sizes = batch_sizes(count, seed)
+    # The persistent loss/adjoint launch runs the drawn sizes as they are;
+    # the multigrid solve needs 2^k + 1 nodes per axis, so its batch uses
+    # the same draws snapped.
+    sizes = batch_sizes(count, seed, snap=False)
     srcs = [gaussian_mixture(n, seed=1000 + i) for i, n in enumerate(sizes)]
     _progress('P: building batch of {} instances'.format(count))
     b = PoissonBatch(sizes, srcs, dtype=torch.float32)
-    out = {'case': 'P', 'workload': '{} Poisson instances, sizes {}, FP32'
-           .format(count, {n: sizes.count(n) for n in sorted(set(sizes))}),
+    out = {'case': 'P', 'workload': '{} Poisson instances, n x n with n '
+           'log-uniform in [64, 1024] ({} distinct, {}..{}), FP32'.format(
+               count, len(set(sizes)), min(sizes), max(sizes)),
            'cells': b.total}
     _progress('P: autotuning the batched launch')
     shape = b.tune()
@@ -229,6 +233,11 @@ def case_p(count=256, seed=0, solve=True):
                 'loop_ms': ms_l, 'loop_instances_per_s': count / (ms_l * 1e-3),
                 'loop_GBps': nb / (ms_l * 1e6)})
     if solve:
+        ssz = batch_sizes(count, seed)
+        ssrc = [gaussian_mixture(n, seed=1000 + i) for i, n in enumerate(ssz)]
+        b = PoissonBatch(ssz, ssrc, dtype=torch.float32)
+        srcs = ssrc
+        out['solve_sizes'] = {n: ssz.count(n) for n in sorted(set(ssz))}
         for use_graph in (False, True):
             _progress('P: solver setup, graphs={}'.format(use_graph))
             t0 = time.perf_counter()

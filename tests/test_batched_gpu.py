@@ -13,10 +13,15 @@ from oracle import tape
 pytestmark = pytest.mark.gpu
 
 
-def _batch(dev, dtype, count=10, seed=0, lo=64, hi=256):
+def _batch(dev, dtype, count=10, seed=0, lo=64, hi=256, snap=True,
+           rect=False):
     from odil_amd.batched import PoissonBatch, batch_sizes, gaussian_mixture
-    sizes = batch_sizes(count, seed, lo, hi)
+    sizes = batch_sizes(count, seed, lo, hi, snap=snap)
     srcs = [gaussian_mixture(n, seed=1000 + i) for i, n in enumerate(sizes)]
+    if rect:  # n x m instances, m drawn independently (sources cropped)
+        m = batch_sizes(count, seed + 1, lo, hi, snap=False)
+        sizes = [(n, min(n, k)) for n, k in zip(sizes, m)]
+        srcs = [s[:, :k] for s, (_, k) in zip(srcs, sizes)]
     return PoissonBatch(sizes, srcs, dtype=dtype, device=dev), srcs
 
 
@@ -24,13 +29,20 @@ def test_sizes_and_sources():
     from odil_amd.batched import batch_sizes, gaussian_mixture
     s = batch_sizes(256, 0)
     assert set(s) <= {65, 129, 257, 513, 1025} and len(set(s)) == 5
+    r = batch_sizes(256, 0, snap=False)
+    assert min(r) >= 64 and max(r) <= 1024 and len(set(r)) > 150
     np.testing.assert_array_equal(gaussian_mixture(33, 5),
                                   ops.gaussian_mixture_source(33, 5))
 
 
 @pytest.mark.parametrize('dtype', [torch.float32, torch.float64])
-def test_batch_launch_matches_single_grid_and_oracle(dev, dtype):
-    b, srcs = _batch(dev, dtype)
+@pytest.mark.parametrize('sizes', ['snapped', 'uneven', 'rectangular'])
+def test_batch_launch_matches_single_grid_and_oracle(dev, dtype, sizes):
+    """Snapped (2^k + 1), uneven (any n) and rectangular (n x m) instances:
+    every tile shape of the persistent launch equals the single-grid kernel
+    bit for bit and the oracle to FP64 round-off."""
+    b, srcs = _batch(dev, dtype, snap=sizes == 'snapped',
+                     rect=sizes == 'rectangular')
     rng = np.random.default_rng(3)
     b.u.copy_(torch.tensor(rng.normal(size=b.total), dtype=dtype))
     for shape in b.shapes():
@@ -46,9 +58,9 @@ def test_batch_launch_matches_single_grid_and_oracle(dev, dtype):
         assert torch.equal(grad, g1), shape
     # FP64 oracle on two instances
     for i in (0, b.count - 1):
-        n = b.sizes[i][0]
+        n = b.sizes[i]
         u = b.view(b.u, i).double().cpu().numpy()
-        dom = tape.Domain(ndim=2, shape=(n, n), varnames=('x', 'y'),
+        dom = tape.Domain(ndim=2, shape=n, varnames=('x', 'y'),
                           fieldnames=['u'])
         src = b.view(b.src, i).double().cpu().numpy()
         ol, og, _ = tape.eval_loss_grad(ops.make_poisson_operator(src), dom,
