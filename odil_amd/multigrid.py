@@ -1335,8 +1335,9 @@ class Multigrid:
         piv = torch.empty((nv, N), dtype=torch.int32, device=dev)
         flag = torch.zeros(1, dtype=torch.int32, device=dev)
         st = _lib.lib().odil_block_diag_factor(
-            c_int(_lib.F64), c_int(nv), c_int(N), (ctypes.c_int * len(dq))(*dq),
-            c_int(len(A.entries)), ptr(A.coef), ptr(lu), ptr(piv), ptr(flag), stream())
+            c_int(_lib.F64), c_int(nv), c_int(N),
+            (ctypes.c_int * len(dq))(*dq), c_int(len(A.entries)),
+            ptr(A.coef), ptr(lu), ptr(piv), ptr(flag), stream())
         check(st, 'block_diag_factor')
         if int(flag.item()):
             raise np.linalg.LinAlgError(
