@@ -168,6 +168,11 @@ constexpr int kGradTx32 = 16, kGradTx64 = 8, kGradTy = 32;
 // staged 28.8 us (nothing staged 30.1); FP64 nothing staged 47.7 us (state
 // and x staged 61.4: three FP64 tiles leave two workgroups per CU).
 constexpr int kNormStage32 = 2, kNormStage64 = 0;
+// Its tiles (FP32 with that staging: 16 x 16 20.7 us, 8 x 32 22.6, 32 x 16
+// 25.7, 16 x 32 25.3, 4 x 64 26.5, 8 x 64 26.7; FP64 unstaged: 8 x 32 34.9,
+// 16 x 16 35.7, 4 x 64 37.9, 16 x 32 46.7; mode 2 at 1025^2, cache-resident
+// inputs).
+constexpr int kNormTx32 = 16, kNormTy32 = 16, kNormTx64 = 8, kNormTy64 = 32;
 
 constexpr int kWavesPerBlock = 4;
 inline int residual_nblocks(int n0, int n1) {
@@ -748,10 +753,12 @@ int dispatch_normal_apply(int dtype, const void* const* state,
                           int n1, double omega, int mode,
                           const typename Op64::Params& P, hipStream_t s) {
   if (dtype == kF64)
-    return launch_normal_apply_t<double, Op64, 8, 32, kNormStage64>(
+    return launch_normal_apply_t<double, Op64, kNormTx64, kNormTy64,
+                                 kNormStage64>(
         state, x, b, d, y, n0, n1, omega, mode, P, s);
   if (dtype == kF32)
-    return launch_normal_apply_t<float, Op32, 8, 32, kNormStage32>(
+    return launch_normal_apply_t<float, Op32, kNormTx32, kNormTy32,
+                                 kNormStage32>(
         state, x, b, d, y, n0, n1, omega, mode, P, s);
   set_error("normal_apply: bad dtype %d", dtype);
   return kErrArg;
