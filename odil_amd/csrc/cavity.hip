@@ -465,10 +465,10 @@ int cavity_dispatch_grad(const void* const* fields, void* const* grad,
                          const CavityParams& P, hipStream_t s, int* nb) {
   const T scale = T(2.0 / ((double)nx * ny));
   if constexpr (sizeof(T) == 4)
-    return launch_loss_grad_t<T, CavityOp<T>, kGradTx32, kGradTy>(
+    return launch_loss_grad_t<T, CavityOp<T>, kGradTx32, kGradTy32>(
         fields, grad, partial, nx, ny, scale, P, s, nb);
   else
-    return launch_loss_grad_t<T, CavityOp<T>, kGradTx64, kGradTy>(
+    return launch_loss_grad_t<T, CavityOp<T>, kGradTx64, kGradTy64>(
         fields, grad, partial, nx, ny, scale, P, s, nb);
 }
 }  // namespace
@@ -601,12 +601,12 @@ int odil_cavity_extras_loss_grad(int dtype, int nx, int ny, double dx,
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (dtype == kF64)
     return launch_loss_grad_t<double, CavityExtrasOp<double>, kGradTx64,
-                              kGradTy>(fields, grad, partial, nx, ny,
+                              kGradTy64>(fields, grad, partial, nx, ny,
                                        2.0 / ((double)nx * ny), P, s,
                                        nblocks);
   if (dtype == kF32)
     return launch_loss_grad_t<float, CavityExtrasOp<float>, kGradTx32,
-                              kGradTy>(fields, grad, partial, nx, ny,
+                              kGradTy32>(fields, grad, partial, nx, ny,
                                        (float)(2.0 / ((double)nx * ny)), P,
                                        s, nblocks);
   set_error("cavity_extras_loss_grad: bad dtype %d", dtype);

@@ -132,10 +132,10 @@ int odil_poisson_loss_grad(int dtype, int n0, int n1, double dx, double dy,
   hipStream_t s = static_cast<hipStream_t>(stream);
   const double scale = 2.0 / ((double)n0 * n1);
   if (dtype == kF64)
-    return launch_loss_grad_t<double, PoissonOp<double>, kGradTx64, kGradTy>(
+    return launch_loss_grad_t<double, PoissonOp<double>, kGradTx64, kGradTy64>(
         fields, grad, partial, n0, n1, scale, P, s, nblocks);
   if (dtype == kF32)
-    return launch_loss_grad_t<float, PoissonOp<float>, kGradTx32, kGradTy>(
+    return launch_loss_grad_t<float, PoissonOp<float>, kGradTx32, kGradTy32>(
         fields, grad, partial, n0, n1, (float)scale, P, s, nblocks);
   set_error("poisson_loss_grad: bad dtype %d", dtype);
   return kErrArg;
@@ -199,10 +199,10 @@ int odil_laplace_loss_grad(int dtype, int n0, int n1, double dx, double dy,
   hipStream_t s = static_cast<hipStream_t>(stream);
   const double scale = 2.0 / ((double)n0 * n1);
   if (dtype == kF64)
-    return launch_loss_grad_t<double, LaplaceOp<double>, kGradTx64, kGradTy>(
+    return launch_loss_grad_t<double, LaplaceOp<double>, kGradTx64, kGradTy64>(
         fields, grad, partial, n0, n1, scale, P, s, nblocks);
   if (dtype == kF32)
-    return launch_loss_grad_t<float, LaplaceOp<float>, kGradTx32, kGradTy>(
+    return launch_loss_grad_t<float, LaplaceOp<float>, kGradTx32, kGradTy32>(
         fields, grad, partial, n0, n1, (float)scale, P, s, nblocks);
   set_error("laplace_loss_grad: bad dtype %d", dtype);
   return kErrArg;

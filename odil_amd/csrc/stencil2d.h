@@ -162,7 +162,7 @@ __device__ inline void stage_tile(const FieldPtrs<T, NF>& in, T* tile,
 // rows took 16.7, 15.7, 16.9 and 21.3 us in FP32 and 22.0, 21.2, 23.2 and
 // 31.6 us in FP64, inputs and outputs HBM-resident.)
 constexpr int kResTx = 8, kResTy = 64;
-constexpr int kGradTx32 = 16, kGradTx64 = 8, kGradTy = 32;
+constexpr int kGradTx32 = 16, kGradTy32 = 32, kGradTx64 = 8, kGradTy64 = 32;
 // LDS staging of the matrix-free normal operator (k_normal_apply STG),
 // measured on the 1025^2 cavity with HBM-resident inputs: FP32 state and x
 // staged 28.8 us (nothing staged 30.1); FP64 nothing staged 47.7 us (state
@@ -179,7 +179,8 @@ inline int residual_nblocks(int n0, int n1) {
   return ceil_div(n1, kResTy) * ceil_div(n0, kResTx) * kWavesPerBlock;
 }
 inline int grad_nblocks(bool fp32, int n0, int n1) {
-  return ceil_div(n1, kGradTy) * ceil_div(n0, fp32 ? kGradTx32 : kGradTx64) *
+  return ceil_div(n1, fp32 ? kGradTy32 : kGradTy64) *
+         ceil_div(n0, fp32 ? kGradTx32 : kGradTx64) *
          kWavesPerBlock;
 }
 
@@ -773,10 +774,10 @@ int dispatch_jtf(int dtype, const void* const* fields, void* const* out,
                  const typename Op64::Params& P, hipStream_t s,
                  int* nblocks) {
   if (dtype == kF64)
-    return launch_loss_grad_t<double, Op64, kGradTx64, kGradTy>(
+    return launch_loss_grad_t<double, Op64, kGradTx64, kGradTy64>(
         fields, out, partial, n0, n1, scale, P, s, nblocks);
   if (dtype == kF32)
-    return launch_loss_grad_t<float, Op32, kGradTx32, kGradTy>(
+    return launch_loss_grad_t<float, Op32, kGradTx32, kGradTy32>(
         fields, out, partial, n0, n1, (float)scale, P, s, nblocks);
   set_error("jtf: bad dtype %d", dtype);
   return kErrArg;

@@ -82,10 +82,10 @@ int odil_wave_loss_grad(int dtype, int nt, int nx, double dt, double dx,
   hipStream_t s = static_cast<hipStream_t>(stream);
   const double scale = 2.0 / ((double)nt * nx);
   if (dtype == kF64)
-    return launch_loss_grad_t<double, WaveOp<double>, kGradTx64, kGradTy>(
+    return launch_loss_grad_t<double, WaveOp<double>, kGradTx64, kGradTy64>(
         fields, grad, partial, nt, nx, scale, P, s, nblocks);
   if (dtype == kF32)
-    return launch_loss_grad_t<float, WaveOp<float>, kGradTx32, kGradTy>(
+    return launch_loss_grad_t<float, WaveOp<float>, kGradTx32, kGradTy32>(
         fields, grad, partial, nt, nx, (float)scale, P, s, nblocks);
   set_error("wave_loss_grad: bad dtype %d", dtype);
   return kErrArg;

@@ -201,10 +201,10 @@ int winv_run(int nt, int nx, const WaveParams& P, const int* sensor_of,
   const T scale = T(2.0 / ncell);
   int st;
   if (sizeof(T) == 4)
-    st = launch_loss_grad_t<T, WaveOp<T>, kGradTx32, kGradTy>(
+    st = launch_loss_grad_t<T, WaveOp<T>, kGradTx32, kGradTy32>(
         fields, grads, pw, nt, nx, scale, P, s, nullptr);
   else
-    st = launch_loss_grad_t<T, WaveOp<T>, kGradTx64, kGradTy>(
+    st = launch_loss_grad_t<T, WaveOp<T>, kGradTx64, kGradTy64>(
         fields, grads, pw, nt, nx, scale, P, s, nullptr);
   if (st) return st;
   hipLaunchKernelGGL(k_winv_data<T>, dim3(nbd), dim3(256), 0, s,
