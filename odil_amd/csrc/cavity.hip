@@ -341,6 +341,262 @@ struct CavityOp {
                                      T (*J)[NF][NS]) {
     compute<INT>(q, i, j, nx, ny, P, f, J, true);
   }
+
+  // ---- Interior tiles -------------------------------------------------
+  // Where a cell and its distance-2 neighbours are away from the walls,
+  // compute<true>'s live Jacobian is
+  //   fu <- u, fv <- v: row coefficients of the frozen velocities (a, b) at
+  //     the row's cell, e0 = 2 lxx + 2 lyy + |a| idx + |b| idy,
+  //     e1 = -lxx - max(a, 0) idx (x-1), e2 = -lxx + min(a, 0) idx (x+1),
+  //     e3 = -lyy - max(b, 0) idy (y-1), e4 = -lyy + min(b, 0) idy (y+1);
+  //   fu <- p: -/+ h2x at x-1 / x+1;  fv <- p: -/+ h2y at y-1 / y+1;
+  //   fp <- u: -/+ h2x at x-1 / x+1;  fp <- v: -/+ h2y at y-1 / y+1;
+  //   fp <- p (Rhie-Chow): sum over the four faces of K_f (p_c - p_nbr),
+  //     K = (ig_c + ig_nbr) idx^2 / 2 across x-faces (idy^2 / 2 across
+  //     y-faces), ig = 1 / g of the cell's frozen velocities.
+  // J^T w at a cell is then a 5-point column formula in w and the row
+  // coefficients of the cell's neighbours, with no Jacobian array and no
+  // product buffer. The interior tile kernels below use it for the loss
+  // adjoint (w = f) and the normal operator (w = J x, itself a 5-point
+  // row formula). Results equal the generic path's up to rounding.
+  static constexpr bool INTERIOR_TILES = true;
+
+  struct Coefs {
+    T idx, idy, lxx, lyy, h2x, h2y, e0c, ckx, cky, gdiag;
+    bool rc;
+    __device__ explicit Coefs(const Params& P)
+        : idx(T(P.idx)), idy(T(P.idy)), lxx(T(P.lxx)), lyy(T(P.lyy)),
+          h2x(T(P.h2x)), h2y(T(P.h2y)),
+          e0c(T(2) * T(P.lxx) + T(2) * T(P.lyy)),
+          ckx(T(0.5) * T(P.idx) * T(P.idx)),
+          cky(T(0.5) * T(P.idy) * T(P.idy)), gdiag(T(P.gdiag)),
+          rc(P.rhiechow != 0) {}
+    __device__ inline T ig(T a, T b) const {
+      return recip(fabs(a) * idx + fabs(b) * idy + gdiag);
+    }
+  };
+
+  // y = J x at an interior cell. x: field 0 at the cell, fields XN apart,
+  // rows X1 apart; uv: u at the cell (v UN further), rows U1 apart; ig:
+  // ig at the cell, rows G1 apart.
+  template <int X1, int XN, int U1, int UN, int G1>
+  __device__ static inline void jx_row(const T* x, const T* uv, const T* ig,
+                                       const Coefs& K, T* y) {
+    const T a = uv[0], b = uv[UN];
+    const T e0 = K.e0c + fabs(a) * K.idx + fabs(b) * K.idy;
+    const T e1 = -K.lxx - fmax(a, T(0)) * K.idx;
+    const T e2 = -K.lxx + fmin(a, T(0)) * K.idx;
+    const T e3 = -K.lyy - fmax(b, T(0)) * K.idy;
+    const T e4 = -K.lyy + fmin(b, T(0)) * K.idy;
+    const T* xu = x;
+    const T* xv = x + XN;
+    const T* xp = x + 2 * XN;
+    y[0] = e0 * xu[0] + e1 * xu[-X1] + e2 * xu[X1] + e3 * xu[-1] +
+           e4 * xu[1] + K.h2x * (xp[X1] - xp[-X1]);
+    y[1] = e0 * xv[0] + e1 * xv[-X1] + e2 * xv[X1] + e3 * xv[-1] +
+           e4 * xv[1] + K.h2y * (xp[1] - xp[-1]);
+    T yp = K.h2x * (xu[X1] - xu[-X1]) + K.h2y * (xv[1] - xv[-1]);
+    if (K.rc) {
+      const T g = ig[0];
+      const T kxm = (ig[-G1] + g) * K.ckx, kxp = (ig[G1] + g) * K.ckx;
+      const T kym = (ig[-1] + g) * K.cky, kyp = (ig[1] + g) * K.cky;
+      const T pc = xp[0];
+      yp += kxm * (pc - xp[-X1]) + kxp * (pc - xp[X1]) +
+            kym * (pc - xp[-1]) + kyp * (pc - xp[1]);
+    }
+    y[2] = yp;
+  }
+
+  // g = J^T w at an interior cell; w, uv and ig addressed as in jx_row.
+  // Needs w and ig on the cell and its 4 neighbours, u at x-1, x, x+1 and
+  // v at y-1, y, y+1.
+  template <int W1, int WN, int U1, int UN, int G1>
+  __device__ static inline void jt_column(const T* w, const T* uv,
+                                          const T* ig, const Coefs& K,
+                                          T* g) {
+    const T uc = uv[0], uxp = uv[U1], uxm = uv[-U1];
+    const T vc = uv[UN], vyp = uv[UN + 1], vym = uv[UN - 1];
+    const T e0 = K.e0c + fabs(uc) * K.idx + fabs(vc) * K.idy;
+    const T e1n = -K.lxx - fmax(uxp, T(0)) * K.idx;  // row x+1, this column
+    const T e2n = -K.lxx + fmin(uxm, T(0)) * K.idx;  // row x-1
+    const T e3n = -K.lyy - fmax(vyp, T(0)) * K.idy;  // row y+1
+    const T e4n = -K.lyy + fmin(vym, T(0)) * K.idy;  // row y-1
+    const T* wu = w;
+    const T* wv = w + WN;
+    const T* wp = w + 2 * WN;
+    g[0] = e0 * wu[0] + e1n * wu[W1] + e2n * wu[-W1] + e3n * wu[1] +
+           e4n * wu[-1] + K.h2x * (wp[-W1] - wp[W1]);
+    g[1] = e0 * wv[0] + e1n * wv[W1] + e2n * wv[-W1] + e3n * wv[1] +
+           e4n * wv[-1] + K.h2y * (wp[-1] - wp[1]);
+    T gp = K.h2x * (wu[-W1] - wu[W1]) + K.h2y * (wv[-1] - wv[1]);
+    if (K.rc) {
+      const T c = ig[0];
+      const T kxm = (ig[-G1] + c) * K.ckx, kxp = (ig[G1] + c) * K.ckx;
+      const T kym = (ig[-1] + c) * K.cky, kyp = (ig[1] + c) * K.cky;
+      const T pc = wp[0];
+      gp += kxm * (pc - wp[-W1]) + kxp * (pc - wp[W1]) +
+            kym * (pc - wp[-1]) + kyp * (pc - wp[1]);
+    }
+    g[2] = gp;
+  }
+
+  template <int TX, int TY>
+  static constexpr size_t interior_adjoint_smem() {
+    return sizeof(T) *
+           (size_t)(3 * (TX + 6) * (TY + 6) + 4 * (TX + 2) * (TY + 2));
+  }
+
+  // Loss adjoint grad = scale J^T f of one interior TX x TY tile at
+  // (x0, y0) (the tile grown by 3 lies inside the grid): the state is
+  // staged with halo 3, f and ig are evaluated on the tile grown by 1, then
+  // every cell applies jt_column. Adds the tile's f_e^2 to ssq[e].
+  template <int TX, int TY>
+  __device__ static inline void interior_adjoint_tile(
+      const FieldPtrs<T, 3>& in, const OutPtrs<T, 3>& grad, int n0, int n1,
+      int x0, int y0, T scale, const Params& P, T* smem, double* ssq) {
+    constexpr int H = 3, S0 = TX + 2 * H, S1 = TY + 2 * H, NS1 = S0 * S1;
+    constexpr int E0 = TX + 2, E1 = TY + 2, NE1 = E0 * E1;
+    constexpr int NPT = TX * TY / 256;
+    static_assert((TX * TY) % 256 == 0, "tile must be a multiple of 256");
+    T* st = smem;           // state [3][S0][S1]
+    T* F = st + 3 * NS1;    // residuals [3][E0][E1]
+    T* IG = F + 3 * NE1;    // 1 / g [E0][E1]
+    stage_tile<T, 3, TX, TY, H>(in, st, x0, y0, n0, n1);
+    __syncthreads();
+    const Coefs K(P);
+    for (int q = threadIdx.x; q < NE1; q += 256) {
+      const int ea = q / E1, eb = q - ea * E1;
+      const T* cell = st + (ea + H - 1) * S1 + (eb + H - 1);
+      LdsAcc<T, S0, S1> acc{cell};
+      T f[3];
+      residual<true>(acc, x0 - 1 + ea, y0 - 1 + eb, n0, n1, P, f);
+      if (K.rc) IG[q] = K.ig(cell[0], cell[NS1]);
+      F[q] = f[0];
+      F[NE1 + q] = f[1];
+      F[2 * NE1 + q] = f[2];
+      if (ea >= 1 && ea <= TX && eb >= 1 && eb <= TY) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) ssq[e] += (double)f[e] * (double)f[e];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NPT; ++p) {
+      const int q = threadIdx.x + 256 * p;
+      const int a = q / TY, b = q - a * TY;
+      const int e = (a + 1) * E1 + (b + 1);
+      T g[3];
+      jt_column<E1, NE1, S1, NS1, E1>(F + e, st + (a + H) * S1 + (b + H),
+                                      IG + e, K, g);
+      const size_t c = (size_t)(x0 + a) * n1 + (y0 + b);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        if (grad.f[k]) grad.f[k][c] = g[k] * scale;
+    }
+  }
+
+  template <int TX, int TY>
+  static constexpr size_t interior_normal_smem() {
+    return sizeof(T) *
+           (size_t)(6 * (TX + 4) * (TY + 4) + 3 * (TX + 2) * (TY + 2));
+  }
+
+  // Normal operator N = J^T J of one interior tile (modes 0, 1, 2 of
+  // k_normal_apply). u, v and x are staged with halo 2, and ig is computed
+  // from the staged u, v on the way into LDS; y = J x on the tile grown by
+  // 1; then jt_column and the mode's epilogue (b and d are loaded before
+  // the first barrier, so their latency hides behind the staging).
+  template <int TX, int TY>
+  __device__ static inline void interior_normal_tile(
+      const FieldPtrs<T, 3>& st_in, const FieldPtrs<T, 3>& x_in,
+      const FieldPtrs<T, 3>& b_in, const FieldPtrs<T, 3>& d_in,
+      const OutPtrs<T, 3>& y, int n0, int n1, T omega, int mode,
+      const Params& P, int x0, int y0, T* smem) {
+    constexpr int A0 = TX + 4, A1 = TY + 4, NA = A0 * A1;
+    constexpr int E0 = TX + 2, E1 = TY + 2, NE1 = E0 * E1;
+    constexpr int NPT = TX * TY / 256;
+    constexpr int NL = (NA + 255) / 256;
+    static_assert((TX * TY) % 256 == 0, "tile must be a multiple of 256");
+    T* UV = smem;        // u, v [2][A0][A1]
+    T* X = UV + 2 * NA;  // x [3][A0][A1]
+    T* IG = X + 3 * NA;  // 1 / g [A0][A1]
+    T* Y = IG + NA;      // J x [3][E0][E1]
+    const Coefs K(P);
+    T bv[NPT][3], dv[NPT][3];
+    if (mode != 0) {
+#pragma unroll
+      for (int p = 0; p < NPT; ++p) {
+        const int q = threadIdx.x + 256 * p;
+        const int a = q / TY, b = q - a * TY;
+        const size_t c = (size_t)(x0 + a) * n1 + (y0 + b);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          bv[p][k] = b_in.f[k][c];
+          dv[p][k] = mode == 2 ? d_in.f[k][c] : T(1);
+        }
+      }
+    }
+    {
+      // The tile grown by 2 lies inside the grid: no clamping.
+      T vu[NL], vv[NL], vx[3][NL];
+      const size_t base = (size_t)(x0 - 2) * n1 + (y0 - 2);
+#pragma unroll
+      for (int r = 0; r < NL; ++r) {
+        const int q = threadIdx.x + 256 * r;
+        if (r + 1 < NL || q < NA) {
+          const int a = q / A1, b = q - a * A1;
+          const size_t o = base + (size_t)a * n1 + b;
+          vu[r] = st_in.f[0][o];
+          vv[r] = st_in.f[1][o];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) vx[k][r] = x_in.f[k][o];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < NL; ++r) {
+        const int q = threadIdx.x + 256 * r;
+        if (r + 1 < NL || q < NA) {
+          UV[q] = vu[r];
+          UV[NA + q] = vv[r];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) X[k * NA + q] = vx[k][r];
+          if (K.rc) IG[q] = K.ig(vu[r], vv[r]);
+        }
+      }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < NE1; q += 256) {
+      const int ea = q / E1, eb = q - ea * E1;
+      const int s = (ea + 1) * A1 + (eb + 1);
+      T v[3];
+      jx_row<A1, NA, A1, NA, A1>(X + s, UV + s, IG + s, K, v);
+      Y[q] = v[0];
+      Y[NE1 + q] = v[1];
+      Y[2 * NE1 + q] = v[2];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NPT; ++p) {
+      const int q = threadIdx.x + 256 * p;
+      const int a = q / TY, b = q - a * TY;
+      const int e = (a + 1) * E1 + (b + 1);
+      const int s = (a + 2) * A1 + (b + 2);
+      T g[3];
+      jt_column<E1, NE1, A1, NA, A1>(Y + e, UV + s, IG + s, K, g);
+      const size_t c = (size_t)(x0 + a) * n1 + (y0 + b);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        T out;
+        if (mode == 0)
+          out = g[k];
+        else if (mode == 1)
+          out = bv[p][k] - g[k];
+        else
+          out = X[k * NA + s] + omega * (bv[p][k] - g[k]) / dv[p][k];
+        y.f[k][c] = out;
+      }
+    }
+  }
 };
 
 // Extra loss terms of the cavity's inverse problems (ns_steady.py:396-411),

@@ -47,10 +47,49 @@ __device__ inline void xcd_block2(unsigned& bx, unsigned& by) {
   by = g / gridDim.x;
 }
 
+// Lane exchange inside rows of 16 lanes by a DPP control (quad_perm,
+// row_ror, ...), on 32- or 64-bit values (a 64-bit value moves as two
+// 32-bit halves).
+template <int CTRL>
+__device__ inline float dpp_move(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                         0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+  const int hi =
+      __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(
+      double, ((long long)hi << 32) | (unsigned long long)(unsigned)lo);
+}
+__device__ inline float read_lane(float v, int l) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ inline double read_lane(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(
+      double, ((long long)hi << 32) | (unsigned long long)(unsigned)lo);
+}
+
+// Sum over the 64 lanes of a wavefront, returned in every lane. Every lane
+// must be active. Inside each row of 16 lanes: swap neighbours
+// (quad_perm [1,0,3,2]), swap pairs ([2,3,0,1]), rotate by 4 and by 8
+// (DPP moves, no LDS round trip); then the four row sums are read as
+// uniform values and added in a fixed order.
 template <class T>
 __device__ inline T wave_sum(T v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  v += dpp_move<0xB1>(v);
+  v += dpp_move<0x4E>(v);
+  v += dpp_move<0x124>(v);
+  v += dpp_move<0x128>(v);
+  return (read_lane(v, 0) + read_lane(v, 16)) +
+         (read_lane(v, 32) + read_lane(v, 48));
 }
 
 // Block-level sum: wavefront reduction, then across waves through LDS.

@@ -19,6 +19,7 @@
 // whose live dependence is linear).
 #pragma once
 
+#include <algorithm>
 #include <type_traits>
 
 #include "common.h"
@@ -472,12 +473,14 @@ constexpr size_t loss_grad_smem() {
 // through the cache), 1 the state (halo RJ + R), 2 the state and x (halo
 // 2 RJ). Staging cuts global loads but costs LDS, hence resident
 // workgroups; the dispatch picks the measured best per precision.
+// normal_apply_tile computes one TX x TY tile at (x0, y0) with the whole
+// workgroup; `ot` is its LDS buffer (normal_apply_smem bytes).
 template <class T, class Op, int TX, int TY, int STG>
-__global__ void __launch_bounds__(256)
-    k_normal_apply(FieldPtrs<T, Op::NF> st_in, FieldPtrs<T, Op::NF> x_in,
-                   FieldPtrs<T, Op::NF> b_in, FieldPtrs<T, Op::NF> d_in,
-                   OutPtrs<T, Op::NF> y, int n0, int n1, T omega, int mode,
-                   typename Op::Params P) {
+__device__ inline void normal_apply_tile(
+    const FieldPtrs<T, Op::NF>& st_in, const FieldPtrs<T, Op::NF>& x_in,
+    const FieldPtrs<T, Op::NF>& b_in, const FieldPtrs<T, Op::NF>& d_in,
+    const OutPtrs<T, Op::NF>& y, int n0, int n1, T omega, int mode,
+    const typename Op::Params& P, int x0, int y0, T* ot) {
   constexpr int NF = Op::NF, NE = Op::NE, NS = Op::NS, RJ = Op::RJ;
   constexpr int E0 = TX + 2 * RJ, E1 = TY + 2 * RJ;
   constexpr int NT = 256;
@@ -485,15 +488,9 @@ __global__ void __launch_bounds__(256)
   static_assert((TX * TY) % NT == 0, "tile must be a multiple of 256");
   constexpr int H = RJ + Op::R, S0 = TX + 2 * H, S1 = TY + 2 * H;
   constexpr int X0 = TX + 4 * RJ, X1 = TY + 4 * RJ;
-  extern __shared__ unsigned char smem_raw[];
-  T* ot = reinterpret_cast<T*>(smem_raw);  // [NF][NS-1][E0][E1]
-  T* st = ot + NF * (NS - 1) * E0 * E1;    // state [NF][S0][S1]
-  T* xt = st + NF * S0 * S1;               // x [NF][X0][X1]
-
-  unsigned bx, by;
-  xcd_block2(bx, by);
-  const int x0 = by * TX;
-  const int y0 = bx * TY;
+  // ot: products [NF][NS-1][E0][E1], then state [NF][S0][S1], x [NF][X0][X1]
+  T* st = ot + NF * (NS - 1) * E0 * E1;
+  T* xt = st + NF * S0 * S1;
   const int tid = threadIdx.x;
   const bool diag = (mode == 3);
   const bool inner = interior_box(x0 - RJ, x0 + TX + RJ, y0 - RJ,
@@ -638,6 +635,20 @@ __global__ void __launch_bounds__(256)
 }
 
 template <class T, class Op, int TX, int TY, int STG>
+__global__ void __launch_bounds__(256)
+    k_normal_apply(FieldPtrs<T, Op::NF> st_in, FieldPtrs<T, Op::NF> x_in,
+                   FieldPtrs<T, Op::NF> b_in, FieldPtrs<T, Op::NF> d_in,
+                   OutPtrs<T, Op::NF> y, int n0, int n1, T omega, int mode,
+                   typename Op::Params P) {
+  extern __shared__ unsigned char smem_raw[];
+  unsigned bx, by;
+  xcd_block2(bx, by);
+  normal_apply_tile<T, Op, TX, TY, STG>(st_in, x_in, b_in, d_in, y, n0, n1,
+                                        omega, mode, P, by * TX, bx * TY,
+                                        reinterpret_cast<T*>(smem_raw));
+}
+
+template <class T, class Op, int TX, int TY, int STG>
 constexpr size_t normal_apply_smem() {
   constexpr int H = Op::RJ + Op::R;
   return sizeof(T) *
@@ -646,6 +657,130 @@ constexpr size_t normal_apply_smem() {
                   (STG >= 1 ? Op::NF * (TX + 2 * H) * (TY + 2 * H) : 0) +
                   (STG >= 2 ? Op::NF * (TX + 4 * Op::RJ) * (TY + 4 * Op::RJ)
                             : 0));
+}
+
+// Operators may supply faster code for interior tiles (the tile grown by
+// RJ keeps distance R from every wall, so no wall rule applies in it):
+//   static constexpr bool INTERIOR_TILES = true;
+//   interior_adjoint_smem<TX, TY>(), interior_adjoint_tile<TX, TY>(...)
+//   interior_normal_smem<TX, TY>(),  interior_normal_tile<TX, TY>(...)
+// The loss-adjoint and normal-operator launches then split the tile grid
+// into those interior tiles and the frame of the others (generic path).
+template <class Op, class = void>
+struct has_interior_tiles : std::false_type {};
+template <class Op>
+struct has_interior_tiles<Op, std::void_t<decltype(Op::INTERIOR_TILES)>>
+    : std::bool_constant<Op::INTERIOR_TILES> {};
+
+// Interior tiles are rows [tyl, tyh) x columns [txl, txh) of the TX x TY
+// tile grid; the nframe others are numbered top rows, bottom rows, then
+// the left and right ends of the middle rows. A split launch runs fpad
+// (nframe rounded up to a multiple of the XCD count) frame blocks first,
+// then one block per interior tile; both groups are dealt to the XCDs in
+// contiguous bands (xcd_block).
+struct TileSplit {
+  int ntx, nty, tyl, tyh, txl, txh, nframe, fpad, ninner;
+  __device__ inline void frame_tile(int f, int& ty, int& tx) const {
+    const int top = tyl * ntx;
+    if (f < top) {
+      ty = f / ntx;
+      tx = f - ty * ntx;
+      return;
+    }
+    f -= top;
+    const int bottom = (nty - tyh) * ntx;
+    if (f < bottom) {
+      ty = tyh + f / ntx;
+      tx = f - (ty - tyh) * ntx;
+      return;
+    }
+    f -= bottom;
+    const int side = txl + (ntx - txh);
+    ty = tyl + f / side;
+    const int r = f - (ty - tyl) * side;
+    tx = r < txl ? r : txh + (r - txl);
+  }
+  __device__ inline void inner_tile(int i, int& ty, int& tx) const {
+    const int w = txh - txl;
+    ty = tyl + i / w;
+    tx = txl + (i - (ty - tyl) * w);
+  }
+};
+
+// Tile x0 = ty TX is interior when x0 - RJ >= R and x0 + TX + RJ <= n0 - R
+// (likewise along axis 1).
+inline TileSplit tile_split(int n0, int n1, int TX, int TY, int R, int RJ) {
+  TileSplit s;
+  s.ntx = ceil_div(n1, TY);
+  s.nty = ceil_div(n0, TX);
+  s.tyl = std::min(s.nty, ceil_div(R + RJ, TX));
+  s.tyh = std::max(s.tyl,
+                   std::min(s.nty, n0 - R - RJ >= 0 ? (n0 - R - RJ) / TX : 0));
+  s.txl = std::min(s.ntx, ceil_div(R + RJ, TY));
+  s.txh = std::max(s.txl,
+                   std::min(s.ntx, n1 - R - RJ >= 0 ? (n1 - R - RJ) / TY : 0));
+  s.ninner = (s.tyh - s.tyl) * (s.txh - s.txl);
+  s.nframe = s.ntx * s.nty - s.ninner;
+  s.fpad = ceil_div(s.nframe, kNumXcd) * kNumXcd;
+  return s;
+}
+
+// k_loss_grad over a split grid: frame tiles run loss_grad_tile, interior
+// tiles the operator's interior_adjoint_tile. Partials keep k_loss_grad's
+// layout (slot (ty ntx + tx) waves + wave).
+template <class T, class Op, int TX, int TY>
+__global__ void __launch_bounds__(256)
+    k_loss_grad_split(FieldPtrs<T, Op::NF> in, OutPtrs<T, Op::NF> grad,
+                      double* __restrict__ partial, int n0, int n1, T scale,
+                      typename Op::Params P, TileSplit sp) {
+  extern __shared__ unsigned char smem_raw[];
+  T* smem = reinterpret_cast<T*>(smem_raw);
+  const int b = blockIdx.x;
+  double ssq[Op::NE];
+  for (int e = 0; e < Op::NE; ++e) ssq[e] = 0;
+  int ty, tx;
+  if (b < sp.fpad) {
+    const int f = (int)xcd_block(b, sp.fpad);
+    if (f >= sp.nframe) return;
+    sp.frame_tile(f, ty, tx);
+    loss_grad_tile<T, Op, TX, TY>(in, grad, n0, n1, ty * TX, tx * TY, scale,
+                                  P, smem, ssq);
+  } else {
+    sp.inner_tile((int)xcd_block(b - sp.fpad, sp.ninner), ty, tx);
+    Op::template interior_adjoint_tile<TX, TY>(in, grad, n0, n1, ty * TX,
+                                               tx * TY, scale, P, smem, ssq);
+  }
+  if (partial)
+    wave_partials<Op::NE>(
+        ssq, partial, (size_t)sp.ntx * sp.nty * kWavesPerBlock,
+        (size_t)(ty * sp.ntx + tx) * kWavesPerBlock + (threadIdx.x >> 6));
+}
+
+// k_normal_apply (modes 0, 1, 2) over a split grid.
+template <class T, class Op, int TX, int TY, int STG>
+__global__ void __launch_bounds__(256)
+    k_normal_apply_split(FieldPtrs<T, Op::NF> st_in,
+                         FieldPtrs<T, Op::NF> x_in, FieldPtrs<T, Op::NF> b_in,
+                         FieldPtrs<T, Op::NF> d_in, OutPtrs<T, Op::NF> y,
+                         int n0, int n1, T omega, int mode,
+                         typename Op::Params P, TileSplit sp) {
+  extern __shared__ unsigned char smem_raw[];
+  T* smem = reinterpret_cast<T*>(smem_raw);
+  const int b = blockIdx.x;
+  int ty, tx;
+  if (b < sp.fpad) {
+    const int f = (int)xcd_block(b, sp.fpad);
+    if (f >= sp.nframe) return;
+    sp.frame_tile(f, ty, tx);
+    normal_apply_tile<T, Op, TX, TY, STG>(st_in, x_in, b_in, d_in, y, n0, n1,
+                                          omega, mode, P, ty * TX, tx * TY,
+                                          smem);
+  } else {
+    sp.inner_tile((int)xcd_block(b - sp.fpad, sp.ninner), ty, tx);
+    Op::template interior_normal_tile<TX, TY>(st_in, x_in, b_in, d_in, y,
+                                              n0, n1, omega, mode, P,
+                                              ty * TX, tx * TY, smem);
+  }
 }
 
 // Deterministic final reduction of per-block partials: out[e] = sum_b
@@ -703,6 +838,21 @@ int launch_loss_grad_t(const void* const* fields, void* const* grad,
   if (nblocks) *nblocks = grid.x * grid.y * kWavesPerBlock;
   constexpr size_t smem = loss_grad_smem<T, Op, TX, TY>();
   static_assert(smem <= 160 * 1024, "LDS tile too large");
+  if constexpr (has_interior_tiles<Op>::value) {
+    const TileSplit sp = tile_split(n0, n1, TX, TY, Op::R, Op::RJ);
+    if (sp.ninner > 0) {
+      constexpr size_t smem2 =
+          std::max(smem, Op::template interior_adjoint_smem<TX, TY>());
+      static_assert(smem2 <= 160 * 1024, "LDS tile too large");
+      auto kern = k_loss_grad_split<T, Op, TX, TY>;
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)smem2);
+      hipLaunchKernelGGL(kern, dim3(sp.fpad + sp.ninner), dim3(256), smem2,
+                         stream, in, out, partial, n0, n1, scale, P, sp);
+      return check_launch("loss_grad");
+    }
+  }
   auto kern = k_loss_grad<T, Op, TX, TY>;
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
@@ -738,6 +888,22 @@ int launch_normal_apply_t(const void* const* state, const void* const* x,
   dim3 grid(ceil_div(n1, TY), ceil_div(n0, TX));
   constexpr size_t smem = normal_apply_smem<T, Op, TX, TY, STG>();
   static_assert(smem <= 160 * 1024, "LDS tile too large");
+  if constexpr (has_interior_tiles<Op>::value) {
+    const TileSplit sp = tile_split(n0, n1, TX, TY, Op::R, Op::RJ);
+    if (mode != 3 && sp.ninner > 0) {
+      constexpr size_t smem2 =
+          std::max(smem, Op::template interior_normal_smem<TX, TY>());
+      static_assert(smem2 <= 160 * 1024, "LDS tile too large");
+      auto kern = k_normal_apply_split<T, Op, TX, TY, STG>;
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)smem2);
+      hipLaunchKernelGGL(kern, dim3(sp.fpad + sp.ninner), dim3(256), smem2,
+                         stream, S, X, B, D, Y, n0, n1, (T)omega, mode, P,
+                         sp);
+      return check_launch("normal_apply");
+    }
+  }
   auto kern = k_normal_apply<T, Op, TX, TY, STG>;
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -73,7 +73,13 @@ def test_residual_and_jacobian(dev, n0, n1, re, bc, rc, dtype):
                 assert (e, key) in seen, (e, key)
 
 
-@pytest.mark.parametrize('n0,n1,re,bc,rc', CASES + [(40, 70, 100., 0, 1)])
+# Grids from 40 x 70 up have interior tiles (the adjoint's transposed
+# 5-point path) besides the frame tiles of the generic path.
+LARGE = [(40, 70, 100., 0, 1), (70, 100, 100., 0, 0), (66, 99, 1000., 1, 1),
+         (75, 110, 300., 2, 1)]
+
+
+@pytest.mark.parametrize('n0,n1,re,bc,rc', CASES + LARGE)
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
 def test_loss_grad(dev, n0, n1, re, bc, rc, dtype):
     from odil_amd.operators import Cavity

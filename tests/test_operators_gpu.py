@@ -148,6 +148,10 @@ def _ops_for_normal(dev, dtype):
           for _ in range(3)]
     out.append(('cavity', Cavity((n0, n1), 0.01), st))
     out.append(('cavity_bc2', Cavity((n0, n1), 0.03, bc=2), st))
+    out.append(('cavity_rc0', Cavity((n0, n1), 0.02, rhiechow=0), st))
+    st2 = [torch.tensor(rng.uniform(-1, 1, (70, 100)), dtype=dtype,
+                        device=dev) for _ in range(3)]
+    out.append(('cavity_bc1_70x100', Cavity((70, 100), 0.005, bc=1), st2))
     dom, op_ref, bnd, u = _wave_case(33, 45, 3)
     out.append(('wave', Wave((33, 45), *bnd, device=dev),
                 [torch.tensor(u, dtype=dtype, device=dev)]))
