@@ -382,6 +382,26 @@ int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
                            const void* const* x, void* const* y,
                            const void* const* f, int mode, double omega,
                            void* stream);
+/* The same modes (0: y = A x, 1: y = f - A x, 2: one weighted-Jacobi
+ * sweep, smoother_jacobi of multigrid.py:543-552) for a symmetric operator
+ * in the storage of odil_stencil_apply_sym, on a 2D grid nw[2] with shifts
+ * of at most 2 per axis (`reach`, host int[2 or 3]) and nvar in {1, 3, 4}
+ * fields of x in one allocation. `ent` int32 [nent][5] = (o, i, s0, s1, s2)
+ * is the stored entry table (device), `dent` the diagonal entry of each
+ * field (device, mode 2). One workgroup per tile computes every output
+ * field with the coefficient arrays streamed through LDS; `tile` in 1..6
+ * picks the tile shape (odil_stencil_tile_lds gives its LDS bytes; shapes
+ * above 64 KiB and more than 256 entries are refused with status 3). Same
+ * reference call as odil_stencil_apply_sym. */
+int odil_stencil_apply_sym_tiled(int dtype, const int* nw, int nvar,
+                                 const int* ent, int nent, const int* dent,
+                                 const int* reach, int tile, const void* coef,
+                                 const void* const* x, void* const* y,
+                                 const void* const* f, int mode, double omega,
+                                 void* stream);
+/* LDS bytes per workgroup of tile shape `tile` for nvar fields (-1 for an
+ * unknown shape). Helper of odil_stencil_apply_sym_tiled. */
+int odil_stencil_tile_lds(int dtype, int nvar, int tile);
 /* y = A^T x or y += A^T x (mul_transpose_field, multigrid.py:1006-1033).
  * `tent` lists entry indices grouped by input field, `istart` delimits. */
 int odil_stencil_apply_t(int dtype, int ndim, const int* nw, int stride,

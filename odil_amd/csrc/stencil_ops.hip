@@ -494,6 +494,183 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// LDS-tiled form of k_apply_sym for 2D grids (|shift| <= 2 per axis, x
+// fields in one allocation). One workgroup owns a tile of tx x ty nodes and
+// computes every output field there. The x fields on the tile grown by two
+// nodes per side are staged in LDS once; the stored coefficient arrays
+// stream through LDS K at a time on the same grown tile, in row segments
+// (coalesced, each array read once per tile plus its ring instead of once
+// per product), the next chunk's loads in flight while the current chunk's
+// products run. Stored entry q = (o, i, s) adds the direct product
+//   A_q(c) x_i(c + s)              to y_o(c)
+// and, unless it is a diagonal centre (o == i, s == 0), the mirrored one
+//   A_q(c - s) x_o(c - s)          to y_i(c)
+// (A_(i,o,-s)(c) = A_(o,i,s)(c - s)). Points outside the grid are zero in
+// LDS, so a product reaching outside vanishes as in k_apply_sym. The entry
+// loop has no branches: the x field of a product is an LDS offset and its
+// output field a uniform 0/1 weight on every accumulator, so the LDS reads
+// of a whole chunk issue together. The diagonal d_o(c) of the Jacobi mode is
+// taken from LDS while its chunk is resident. Tiles are dealt to XCDs in
+// bands (xcd_block) so that rings are read from the XCD's L2.
+constexpr int kTileReach = 2;
+constexpr int kTileMaxEnt = 256;  // stored entries (LDS rows)
+
+template <class T, int NV, int TX, int TY, int K>
+__global__ void __launch_bounds__(256)
+    k_apply_sym_tile(const int* __restrict__ ent, int nent,
+                     const int* __restrict__ dent, const T* __restrict__ coef,
+                     const T* __restrict__ xg, VecPtrs<T> y, CVecPtrs<T> f,
+                     int n0, int n1, int tx, int ty, int ntc, int mode,
+                     T omega) {
+  constexpr int R = kTileReach;
+  constexpr int W = TY + 2 * R;              // LDS row stride
+  constexpr int G = (TX + 2 * R) * W;        // grown-tile capacity
+  constexpr int GP = G + 1;                  // per array, with a spare slot
+  constexpr int CPT = (TX * TY + 255) / 256; // nodes per thread
+  constexpr int SPT = (G + 255) / 256;       // staged points per thread
+  // The spare slot of each array takes the stores of dead staging points,
+  // so the store loops have no branches (a branch there makes the compiler
+  // wait for every load in flight).
+  __shared__ T xs[NV * GP];
+  __shared__ T cs[K][GP];
+  __shared__ int4 erow[kTileMaxEnt];
+  const unsigned g = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = (int)(g / (unsigned)ntc) * tx;
+  const int c0 = (int)(g % (unsigned)ntc) * ty;
+  const int rows = min(tx, n0 - r0), cols = min(ty, n1 - c0);
+  const int gw = cols + 2 * R;          // grown width
+  const int gn = (rows + 2 * R) * gw;   // grown points
+  const unsigned N = (unsigned)n0 * (unsigned)n1;
+  // Staging map: point m of this thread -> byte offset of its node in a
+  // field (0 and masked when outside the grid) and its LDS slot (G: the
+  // spare slot).
+  unsigned soff[SPT];
+  int sl[SPT];
+  bool sok[SPT];
+#pragma unroll
+  for (int m = 0; m < SPT; ++m) {
+    const int p = (int)threadIdx.x + 256 * m;
+    const int gr = p / gw, gc = p - gr * gw;
+    const int a = r0 - R + gr, b = c0 - R + gc;
+    const bool live = p < gn;
+    sok[m] = live && a >= 0 && a < n0 && b >= 0 && b < n1;
+    soff[m] = sok[m] ? ((unsigned)a * (unsigned)n1 + (unsigned)b) *
+                           (unsigned)sizeof(T)
+                     : 0u;
+    sl[m] = live ? gr * W + gc : G;
+  }
+  // Nodes of this thread: LDS centre slot and flat index.
+  int cl[CPT];
+  unsigned gi[CPT];
+  bool cv[CPT];
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    const int k = (int)threadIdx.x + 256 * j;
+    cv[j] = k < rows * cols;
+    const int r = cv[j] ? k / cols : 0, cc = cv[j] ? k - r * cols : 0;
+    cl[j] = (r + R) * W + cc + R;
+    gi[j] = (unsigned)(r0 + r) * (unsigned)n1 + (unsigned)(c0 + cc);
+  }
+  // x tile (all fields), loads first, then the LDS stores.
+  {
+    T v[NV][SPT];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const T* a = xg + (size_t)k * N;
+#pragma unroll
+      for (int m = 0; m < SPT; ++m) v[k][m] = ld_b32(a, soff[m]);
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int m = 0; m < SPT; ++m)
+        xs[k * GP + sl[m]] = sok[m] ? v[k][m] : T(0);
+  }
+  // Coefficient chunk prefetch; the in-grid mask is applied at the LDS
+  // store, so nothing waits for these loads before the products.
+  T pf[K][SPT];
+  auto load_chunk = [&](int q0) {
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) {
+      const int q = min(q0 + kk, nent - 1);
+      const T* a = coef + (size_t)q * N;
+#pragma unroll
+      for (int m = 0; m < SPT; ++m) pf[kk][m] = ld_b32(a, soff[m]);
+    }
+  };
+  // Entry rows, precomputed once per workgroup in LDS: x offsets of the
+  // direct and mirrored products, the shift offset, and the output fields
+  // (+1, 0: no product) of both. Read back as uniform values.
+  for (int q = threadIdx.x; q < nent; q += 256) {
+    const int* e = ent + 5 * q;
+    const int o = e[0], i = e[1], so = e[2] * W + e[3];
+    const int om = o == i && so == 0 ? -1 : i;
+    erow[q] = make_int4(i * GP + so, o * GP - so, so,
+                        (o + 1) | ((om + 1) << 8));
+  }
+  T acc[CPT][NV], dd[CPT][NV];
+#pragma unroll
+  for (int j = 0; j < CPT; ++j)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      acc[j][k] = 0;
+      dd[j][k] = 1;
+    }
+  // Diagonal entries (Jacobi mode), read once.
+  int dk[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) dk[k] = mode == 2 ? dent[k] : -1;
+  load_chunk(0);
+  for (int q0 = 0; q0 < nent; q0 += K) {
+    __syncthreads();  // the previous chunk's products are done
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk)
+#pragma unroll
+      for (int m = 0; m < SPT; ++m)
+        cs[kk][sl[m]] = sok[m] ? pf[kk][m] : T(0);
+    __syncthreads();
+    if (q0 + K < nent) load_chunk(q0 + K);
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) {
+      const int q = q0 + kk;
+      const int4 r = erow[min(q, nent - 1)];
+      const int xd = __builtin_amdgcn_readfirstlane(r.x);
+      const int xm = __builtin_amdgcn_readfirstlane(r.y);
+      const int sh = __builtin_amdgcn_readfirstlane(r.z);
+      const int fl = q < nent ? __builtin_amdgcn_readfirstlane(r.w) : 0;
+      const int od = (fl & 255) - 1, om = (fl >> 8) - 1;
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        const T a = cs[kk][cl[j]];
+        const T pd = a * xs[xd + cl[j]];
+        const T pm = cs[kk][cl[j] - sh] * xs[xm + cl[j]];
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          const T wd = od == k ? T(1) : T(0);
+          const T wm = om == k ? T(1) : T(0);
+          acc[j][k] += wd * pd + wm * pm;
+          if (mode == 2) dd[j][k] = q == dk[k] ? a : dd[j][k];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < CPT; ++j) {
+    if (!cv[j]) continue;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      T out;
+      if (mode == 0)
+        out = acc[j][k];
+      else if (mode == 1)
+        out = f.p[k][gi[j]] - acc[j][k];
+      else
+        out = xs[k * GP + cl[j]] + omega * (f.p[k][gi[j]] - acc[j][k]) / dd[j][k];
+      y.p[k][gi[j]] = out;
+    }
+  }
+}
+
 // Transpose: y_k[p] = sum_{e: in(e)=k} sum_{I: stride*I + s_e = p}
 //                      coef_e[I] x_{out(e)}[I].
 // `tent` lists entry indices grouped by input field, delimited by istart.
@@ -1395,6 +1572,164 @@ int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
 #undef ODIL_APPLY2
 #undef ODIL_APPLY3
   return check_launch("stencil_apply_sym");
+}
+
+}  // extern "C"
+
+namespace {
+
+// Tile shapes of k_apply_sym_tile (rows, columns along the contiguous axis,
+// coefficient arrays per LDS chunk), selected by `tile` = 1..kNumTiles.
+struct TileShape {
+  int tx, ty, k;
+};
+constexpr int kNumTiles = 6;
+constexpr TileShape kTiles[kNumTiles] = {{8, 64, 4},  {16, 32, 4},
+                                         {8, 32, 4},  {8, 64, 8},
+                                         {16, 64, 4}, {4, 64, 8}};
+
+template <class T, int NV, int TX, int TY, int K>
+static void launch_tile(const int* ent, int nent, const int* dent,
+                        const void* coef, const void* const* x,
+                        void* const* y, const void* const* f, int n0, int n1,
+                        int mode, double omega, hipStream_t s) {
+  // Shapes above 64 KiB of LDS are not built (the host refuses them).
+  if constexpr (((size_t)(TX + 2 * kTileReach) * (TY + 2 * kTileReach) + 1) *
+                        (NV + K) * sizeof(T) +
+                    kTileMaxEnt * 16 <=
+                65536) {
+    // Balanced tiles: as many as TX x TY tiles need, sized evenly.
+    const int ntr = ceil_div(n0, TX), ntc = ceil_div(n1, TY);
+    const int tx = ceil_div(n0, ntr), ty = ceil_div(n1, ntc);
+    hipLaunchKernelGGL((k_apply_sym_tile<T, NV, TX, TY, K>),
+                       dim3((unsigned)(ntr * ntc)), dim3(256), 0, s, ent,
+                       nent, dent, static_cast<const T*>(coef),
+                       static_cast<const T*>(x[0]), vp<T>(y, NV),
+                       cvp<T>(f, NV), n0, n1, tx, ty, ntc, mode, (T)omega);
+  }
+}
+
+template <class T, int NV>
+static void launch_tile_nv(int tile, const int* ent, int nent,
+                           const int* dent, const void* coef,
+                           const void* const* x, void* const* y,
+                           const void* const* f, int n0, int n1, int mode,
+                           double omega, hipStream_t s) {
+#define ODIL_TILE(C, TX, TY, K)                                              \
+  case C:                                                                    \
+    launch_tile<T, NV, TX, TY, K>(ent, nent, dent, coef, x, y, f, n0, n1,    \
+                                  mode, omega, s);                           \
+    break
+  switch (tile) {
+    ODIL_TILE(1, 8, 64, 4);
+    ODIL_TILE(2, 16, 32, 4);
+    ODIL_TILE(3, 8, 32, 4);
+    ODIL_TILE(4, 8, 64, 8);
+    ODIL_TILE(5, 16, 64, 4);
+    default:
+      ODIL_TILE(6, 4, 64, 8);
+  }
+#undef ODIL_TILE
+}
+
+// LDS bytes of tile shape `tile` for field count nvar and element size esz.
+static size_t tile_lds(int tile, int nvar, size_t esz) {
+  const TileShape& t = kTiles[tile - 1];
+  return ((size_t)(t.tx + 2 * kTileReach) * (t.ty + 2 * kTileReach) + 1) *
+             (size_t)(nvar + t.k) * esz +
+         kTileMaxEnt * 16;
+}
+
+}  // namespace
+
+extern "C" {
+
+int odil_stencil_tile_lds(int dtype, int nvar, int tile) {
+  if (tile < 1 || tile > kNumTiles || nvar < 1 || nvar > kMaxVar)
+    return -1;
+  return (int)tile_lds(tile, nvar, dtype == kF64 ? 8 : 4);
+}
+
+int odil_stencil_apply_sym_tiled(int dtype, const int* nw, int nvar,
+                                 const int* ent, int nent, const int* dent,
+                                 const int* reach, int tile, const void* coef,
+                                 const void* const* x, void* const* y,
+                                 const void* const* f, int mode, double omega,
+                                 void* stream) {
+  if (bad_dim(2, nw, "stencil_apply_sym_tiled")) return kErrArg;
+  if (nvar != 1 && nvar != 3 && nvar != 4) {
+    set_error("stencil_apply_sym_tiled: %d fields (1, 3 or 4 supported)",
+              nvar);
+    return kErrArg;
+  }
+  if (!ent || !coef || !x || !y || nent < 1) {
+    set_error("stencil_apply_sym_tiled: null table, coefficient or field "
+              "array, or no entries");
+    return kErrArg;
+  }
+  if (mode < 0 || mode > 2 || (mode == 2 && !dent) || (mode >= 1 && !f)) {
+    set_error("stencil_apply_sym_tiled: mode %d with missing operands", mode);
+    return kErrArg;
+  }
+  if (!reach || reach[0] < 0 || reach[1] < 0 || reach[0] > kTileReach ||
+      reach[1] > kTileReach) {
+    set_error("stencil_apply_sym_tiled: shifts beyond %d per axis",
+              kTileReach);
+    return kErrArg;
+  }
+  if (tile < 1 || tile > kNumTiles) {
+    set_error("stencil_apply_sym_tiled: tile must be in [1, %d]", kNumTiles);
+    return kErrArg;
+  }
+  if (nent > kTileMaxEnt) {
+    set_error("stencil_apply_sym_tiled: %d stored entries (at most %d)",
+              nent, kTileMaxEnt);
+    return kErrCapacity;
+  }
+  if (dtype != kF64 && dtype != kF32) {
+    set_error("stencil_apply_sym_tiled: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  const size_t esz = dtype == kF64 ? 8 : 4;
+  if (tile_lds(tile, nvar, esz) > 65536) {
+    set_error("stencil_apply_sym_tiled: tile %d needs %zu bytes of LDS "
+              "(64 KiB per workgroup)", tile, tile_lds(tile, nvar, esz));
+    return kErrCapacity;
+  }
+  const size_t n = (size_t)nw[0] * (size_t)nw[1];
+  // 32-bit byte offsets inside one field.
+  if (n * esz >= (1ull << 32)) {
+    set_error("stencil_apply_sym_tiled: field above 4 GiB");
+    return kErrCapacity;
+  }
+  for (int k = 1; k < nvar; ++k)
+    if (static_cast<const char*>(x[k]) !=
+        static_cast<const char*>(x[0]) + k * n * esz) {
+      set_error("stencil_apply_sym_tiled: x fields must be one allocation");
+      return kErrArg;
+    }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+#define ODIL_TILE_NV(T)                                                     \
+  switch (nvar) {                                                           \
+    case 1:                                                                 \
+      launch_tile_nv<T, 1>(tile, ent, nent, dent, coef, x, y, f, nw[0],     \
+                           nw[1], mode, omega, s);                          \
+      break;                                                                \
+    case 3:                                                                 \
+      launch_tile_nv<T, 3>(tile, ent, nent, dent, coef, x, y, f, nw[0],     \
+                           nw[1], mode, omega, s);                          \
+      break;                                                                \
+    default:                                                                \
+      launch_tile_nv<T, 4>(tile, ent, nent, dent, coef, x, y, f, nw[0],     \
+                           nw[1], mode, omega, s);                          \
+  }
+  if (dtype == kF64) {
+    ODIL_TILE_NV(double);
+  } else {
+    ODIL_TILE_NV(float);
+  }
+#undef ODIL_TILE_NV
+  return check_launch("stencil_apply_sym_tiled");
 }
 
 // y = A^T x (accumulate != 0: y += A^T x). `tent`/`istart` group entry

@@ -18,6 +18,10 @@ from odil_amd import _lib
 from odil_amd._lib import c_double, c_int, check, ptr, ptrs, stream
 from odil_amd.dense import SpdSolver
 
+# `lanes` value selecting the LDS-tiled symmetric sweep (`per` is then the
+# tile shape, odil_stencil_apply_sym_tiled).
+TILED = -3
+
 
 def _grid_arr(nw):
     return (ctypes.c_int * 3)(*(list(nw) + [1] * (3 - len(nw))))
@@ -180,6 +184,16 @@ class BlockStencil:
                    for k, a in enumerate(x))
 
     def _call_apply(self, x, y, f, mode, omega):
+        if self.sym and self.lanes == TILED:
+            st = _lib.lib().odil_stencil_apply_sym_tiled(
+                c_int(_lib.dtype_tag(self.dtype)), _grid_arr(self.nw),
+                c_int(self.nvar_out), ptr(self.t_ent), c_int(len(self.entries)),
+                ptr(self.t_dent), self._reach, c_int(self.per),
+                ptr(self.coef), ptrs(x), ptrs(y),
+                ptrs(f) if f is not None else None, c_int(mode),
+                c_double(omega), stream())
+            check(st, 'stencil_apply_sym_tiled')
+            return
         if self.sym:
             st = _lib.lib().odil_stencil_apply_sym(
                 c_int(_lib.dtype_tag(self.dtype)), c_int(self.ndim),
@@ -201,6 +215,23 @@ class BlockStencil:
             ptrs(f) if f is not None else None, c_int(mode),
             c_double(omega), stream())
         check(st, 'stencil_apply')
+
+    def tile_shapes(self):
+        """Tile shapes of the LDS-tiled symmetric sweep that apply to this
+        operator (2D, shifts of at most 2, 1, 3 or 4 fields, at most 256
+        stored entries and 64 KiB of LDS per workgroup); empty otherwise.
+        The autotuner times them beside the per-cell launch shapes."""
+        if not self.sym or self.ndim != 2 or self.nvar_out not in (1, 3, 4):
+            return []
+        if len(self.entries) > 256:
+            return []
+        if max(self._reach[0], self._reach[1]) > 2:
+            return []
+        lib = _lib.lib()
+        dt = _lib.dtype_tag(self.dtype)
+        return [t for t in range(1, 7)
+                if 0 < lib.odil_stencil_tile_lds(dt, self.nvar_out, t)
+                <= 65536]
 
     def apply(self, x, y=None):
         """y = A x (mul_field)."""
@@ -255,6 +286,7 @@ class BlockStencil:
         cand = [(L, p) for L in (0, 1, 2, 4, 8, 16) for p in (1, 2, 4, 8)]
         if contig:
             cand += [(-1, 1), (-2, 1)]
+            cand += [(TILED, t) for t in self.tile_shapes()]
         self.lanes, self.per = autotune.pick(key, cand, run)
         return self
 

@@ -335,7 +335,8 @@ def _rand_sym(dev, dtype, nw, nvar, seed):
 
 @pytest.mark.parametrize('nw,nvar', [((13, 10), 3), ((40, 37), 2),
                                      ((7, 6, 9), 2), ((33, 67), 3),
-                                     ((9, 8, 23), 2)])
+                                     ((9, 8, 23), 2), ((97, 130), 3),
+                                     ((70, 131), 1), ((21, 45), 4)])
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
 def test_sym_apply(dev, nw, nvar, dtype):
     A, ent, coef = _rand_sym(dev, dtype, nw, nvar, seed=len(nw) + nvar)
@@ -359,15 +360,22 @@ def test_sym_apply(dev, nw, nvar, dtype):
         want = xn[k] + 0.6 * (fn[k] - ref[k]) / d
         assert np.abs(y[k].double().cpu().numpy() - want).max() <= tol * sc
     # every launch shape on contiguous x (one allocation), incl. the vector
-    # path (-1: 16 bytes of consecutive cells per thread) and two
-    # interleaved cells per thread (-2)
+    # path (-1: 16 bytes of consecutive cells per thread), two interleaved
+    # cells per thread (-2) and the LDS-tiled sweep (2D, 1, 3 or 4 fields)
+    from odil_amd.multigrid import TILED
     xc = list(torch.tensor(np.stack(xn), dtype=dtype, device=dev).unbind(0))
     fc = list(torch.tensor(np.stack(fn), dtype=dtype, device=dev).unbind(0))
+    tiles = A.tile_shapes()
+    assert tiles or len(nw) == 3 or nvar == 2
     for lanes, per in [(-1, 1), (-2, 1)] + [(L, p) for L in (1, 2, 4, 8, 16)
-                                             for p in (1, 3)]:
+                                             for p in (1, 3)] + \
+            [(TILED, t) for t in tiles]:
         A.lanes, A.per = lanes, per
         for a, r in zip(A.apply(xc), ref):
             assert np.abs(a.double().cpu().numpy() - r).max() <= tol * sc
+        for a, r, b in zip(A.residual(xc, fc), ref, fn):
+            assert np.abs(a.double().cpu().numpy() - (b - r)).max() <= \
+                tol * sc, (lanes, per)
         yc = [torch.empty_like(a) for a in xc]
         A.jacobi(xc, fc, yc, 0.6)
         for k in range(nvar):
