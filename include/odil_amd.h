@@ -372,9 +372,13 @@ int odil_stencil_apply(int dtype, int ndim, const int* nw, int stride,
  * allocation only for -1 and -2), per the 256-thread tiles one
  * workgroup sweeps (1..64; the autotuner chooses lanes and per for each
  * operator shape). Fields of x in one allocation (x_k = x_0 + k N) take
- * faster paths. No reference counterpart for the storage: it applies the
- * same matrices as the reference's full stencils (mul_transpose_op products
- * and Galerkin levels are symmetric). */
+ * faster paths. `reach` may leave out the contiguous axis (with one more
+ * line of margin on the axis before it) when the coefficients that couple
+ * to outside points are zero (odil_stencil_zero_outside): reads past the
+ * end of a grid line then meet zero coefficients. No reference counterpart
+ * for the storage: it applies the same matrices as the reference's full
+ * stencils (mul_transpose_op products and Galerkin levels are
+ * symmetric). */
 int odil_stencil_apply_sym(int dtype, int ndim, const int* nw, int nvar,
                            const int* items, const int* ostart,
                            const int* dent, const int* reach, int max_items,
@@ -402,6 +406,16 @@ int odil_stencil_apply_sym_tiled(int dtype, const int* nw, int nvar,
 /* LDS bytes per workgroup of tile shape `tile` for nvar fields (-1 for an
  * unknown shape). Helper of odil_stencil_apply_sym_tiled. */
 int odil_stencil_tile_lds(int dtype, int nvar, int tile);
+/* Zero the coefficients of a stride-1 stencil operator that couple a node
+ * to a point outside the grid (entry table `ent` int32 [nent][5] =
+ * (o, i, s0, s1, s2), device; `reach` host int[3] bounds |s| per axis).
+ * Under the zero fill of the reference's shifts (multigrid.py:941-947)
+ * these coefficients never enter a product; with them zero the symmetric
+ * sweeps need no per-product test along the contiguous axis away from the
+ * other faces. No reference counterpart (a storage invariant). */
+int odil_stencil_zero_outside(int dtype, int ndim, const int* nw,
+                              const int* ent, int nent, const int* reach,
+                              void* coef, void* stream);
 /* y = A^T x or y += A^T x (mul_transpose_field, multigrid.py:1006-1033).
  * `tent` lists entry indices grouped by input field, `istart` delimits. */
 int odil_stencil_apply_t(int dtype, int ndim, const int* nw, int stride,

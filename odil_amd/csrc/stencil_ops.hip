@@ -671,6 +671,33 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Zero every coefficient that couples a node to a point outside the grid
+// (entry q = (o, i, s) at node c with c + s outside). Such coefficients never
+// enter a product (zero fill, multigrid.py:941-947); once they are zero, a
+// flat read that runs past the end of a grid line lands on a real node whose
+// coefficient for that shift is zero, so the sweeps can skip the per-product
+// test along the contiguous axis. One thread per node; only nodes within
+// `reach` of a face have work.
+template <class T>
+__global__ void __launch_bounds__(256)
+    k_zero_outside(T* __restrict__ coef, const int* __restrict__ ent,
+                   int nent, Grid3 nw, Grid3 reach) {
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = nw.size();
+  if (c >= N) return;
+  int i0, i1, i2;
+  unflat(c, nw, i0, i1, i2);
+  const bool frame = i0 < reach.n0 || i0 >= nw.n0 - reach.n0 ||
+                     i1 < reach.n1 || i1 >= nw.n1 - reach.n1 ||
+                     i2 < reach.n2 || i2 >= nw.n2 - reach.n2;
+  if (!frame) return;
+  for (int q = 0; q < nent; ++q) {
+    const int* e = ent + 5 * q;
+    if (!inside(i0 + e[2], i1 + e[3], i2 + e[4], nw))
+      coef[(size_t)q * N + c] = T(0);
+  }
+}
+
 // Transpose: y_k[p] = sum_{e: in(e)=k} sum_{I: stride*I + s_e = p}
 //                      coef_e[I] x_{out(e)}[I].
 // `tent` lists entry indices grouped by input field, delimited by istart.
@@ -1768,6 +1795,33 @@ int odil_stencil_apply_t(int dtype, int ndim, const int* nw, int stride,
     return kErrArg;
   }
   return check_launch("stencil_apply_t");
+}
+
+int odil_stencil_zero_outside(int dtype, int ndim, const int* nw,
+                              const int* ent, int nent, const int* reach,
+                              void* coef, void* stream) {
+  if (bad_dim(ndim, nw, "stencil_zero_outside")) return kErrArg;
+  if (!ent || !coef || nent < 0 || !reach) {
+    set_error("stencil_zero_outside: null table or coefficients");
+    return kErrArg;
+  }
+  if (nent == 0) return kOk;
+  Grid3 g = mk_grid(ndim, nw);
+  Grid3 rc{reach[0], reach[1], ndim == 3 ? reach[2] : 0};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    hipLaunchKernelGGL(k_zero_outside<double>, dim3(nblk(g.size())),
+                       dim3(256), 0, s, static_cast<double*>(coef), ent, nent,
+                       g, rc);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL(k_zero_outside<float>, dim3(nblk(g.size())),
+                       dim3(256), 0, s, static_cast<float*>(coef), ent, nent,
+                       g, rc);
+  else {
+    set_error("stencil_zero_outside: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  return check_launch("stencil_zero_outside");
 }
 
 // Normal operator sum_e J_e^T J_e from Jacobian coefficients `jc`

@@ -91,6 +91,29 @@ class BlockStencil:
                 raise ValueError('symmetric storage holds upper entries '
                                  'only, got {}'.format(bad[:3]))
         self._tables()
+        if self.sym:
+            self.zero_outside()
+
+    def zero_outside(self):
+        """Zero the coefficients that couple a node to a point outside the
+        grid (they never enter a product: zero fill, multigrid.py:941-947).
+        Symmetric operators keep this invariant, which lets their sweeps skip
+        the per-product bounds test along the contiguous axis; the
+        constructor, normal_operator, galerkin and update_A apply it, and
+        code that writes `coef` in place must call it again."""
+        st = _lib.lib().odil_stencil_zero_outside(
+            c_int(_lib.dtype_tag(self.dtype)), c_int(self.ndim),
+            _grid_arr(self.nw), ptr(self.t_ent), c_int(len(self.entries)),
+            self._reach_all(), ptr(self.coef), stream())
+        check(st, 'stencil_zero_outside')
+        return self
+
+    def _reach_all(self):
+        r = [0, 0, 0]
+        for _, _, s in self.entries:
+            for d, v in enumerate(_shift3(s)):
+                r[d] = max(r[d], abs(v))
+        return (ctypes.c_int * 3)(*r)
 
     @property
     def dtype(self):
@@ -155,6 +178,17 @@ class BlockStencil:
         self.t_items = t(np.array(rows, dtype=np.int32).reshape(-1, 10))
         self.t_iostart = t(np.array(ostart, dtype=np.int32))
         self._reach = (ctypes.c_int * 3)(*reach)
+        # Interior test of the sweeps once the coefficients that couple to
+        # outside points are zero (zero_outside): nothing along the
+        # contiguous axis, one more line of margin on the axis before it, so
+        # that a flat read past the end of a line stays inside the arrays
+        # and lands on a zero coefficient.
+        fast = list(reach)
+        la = self.ndim - 1
+        if reach[la] < self.nw[la]:
+            fast[la] = 0
+            fast[la - 1] += 1
+        self._reach_fast = (ctypes.c_int * 3)(*fast)
         self._max_items = max(b - a for a, b in zip(ostart[:-1], ostart[1:]))
 
     # -- products --------------------------------------------------------
@@ -198,7 +232,7 @@ class BlockStencil:
             st = _lib.lib().odil_stencil_apply_sym(
                 c_int(_lib.dtype_tag(self.dtype)), c_int(self.ndim),
                 _grid_arr(self.nw), c_int(self.nvar_out), ptr(self.t_items),
-                ptr(self.t_iostart), ptr(self.t_dent), self._reach,
+                ptr(self.t_iostart), ptr(self.t_dent), self._reach_fast,
                 c_int(self._max_items), c_int(len(self.entries)),
                 c_int(self.lanes), c_int(self.per if self.lanes >= 0 else 1),
                 ptr(self.coef), ptrs(x), ptrs(y),
@@ -674,6 +708,8 @@ def normal_operator(J, dtype=None, out=None, sym=False):
         c_int(len(keys)), ptr(J.coef), ptr(coef), stream())
     check(st, 'stencil_normal')
     if out is not None:
+        if out.sym:
+            out.zero_outside()
         return out
     return BlockStencil(coef, keys, J.nw, nvar_in=J.nvar_in,
                         nvar_out=J.nvar_in, sym=sym)
@@ -785,6 +821,8 @@ def galerkin(A, restriction='full', rw_dev=None, mode='reference',
         c_int(_rmode(restriction)), stream())
     check(st, 'stencil_galerkin')
     if out is not None:
+        if out.sym:
+            out.zero_outside()
         return out
     return BlockStencil(coef, keys, nwh, nvar_in=A.nvar_in,
                         nvar_out=A.nvar_out, sym=A.sym)
@@ -995,6 +1033,8 @@ class MultigridOp:
             self._tune()
         else:
             self.AA[0].coef.copy_(A0.coef)
+            if self.AA[0].sym:
+                self.AA[0].zero_outside()
             for level in range(1, self.nlevels):
                 galerkin(self.AA[level - 1], self.restriction, self.rw,
                          self.galerkin_mode, out=self.AA[level])

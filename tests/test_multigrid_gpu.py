@@ -314,7 +314,7 @@ def _sym_apply_np(entries, coef, x):
     return y
 
 
-def _rand_sym(dev, dtype, nw, nvar, seed):
+def _rand_sym(dev, dtype, nw, nvar, seed, zero_outside=True):
     from odil_amd.multigrid import BlockStencil, is_upper
     rng = np.random.default_rng(seed)
     dim = len(nw)
@@ -326,8 +326,10 @@ def _rand_sym(dev, dtype, nw, nvar, seed):
     for q, (o, i, s) in enumerate(ent):
         if o == i and not any(s):
             coef[q] += 8.
-        # zero the coefficients whose column leaves the grid
-        coef[q] *= _shifted(np.ones(nw), s)
+        # zero the coefficients whose column leaves the grid (with
+        # zero_outside=False they stay random: they never enter a product)
+        if zero_outside:
+            coef[q] *= _shifted(np.ones(nw), s)
     A = BlockStencil(torch.tensor(coef, dtype=dtype, device=dev), ent, nw,
                      sym=True)
     return A, ent, coef
@@ -391,6 +393,52 @@ def test_sym_apply(dev, nw, nvar, dtype):
     if dtype == torch.float64:
         D = A.to_dense().cpu().numpy()
         np.testing.assert_array_equal(D, M.toarray())
+
+
+@pytest.mark.parametrize('nw,nvar', [((41, 37), 3), ((66, 130), 1),
+                                     ((9, 8, 23), 2), ((5, 64), 3)])
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_sym_outside_coefficients_zeroed(dev, nw, nvar, dtype):
+    """Coefficients that couple a node to a point outside the grid are
+    random in the input: the symmetric storage zeroes them (they never
+    enter a product) and every launch shape, including the interior paths
+    that read past the ends of grid lines, still equals the zero-fill
+    product."""
+    from odil_amd.multigrid import TILED
+    A, ent, coef = _rand_sym(dev, dtype, nw, nvar, seed=11 + nvar,
+                             zero_outside=False)
+    stored = A.coef.double().cpu().numpy()
+    for q, (o, i, s) in enumerate(A.entries):
+        outside = _shifted(np.ones(nw), s) == 0
+        assert np.all(stored[q][outside] == 0)
+        assert np.array_equal(stored[q][~outside],
+                              A.coef[q].double().cpu().numpy()[~outside])
+    rng = np.random.default_rng(3)
+    xn = [rng.uniform(-1, 1, nw) for _ in range(nvar)]
+    fn = [rng.uniform(-1, 1, nw) for _ in range(nvar)]
+    # reference from the input coefficients: the zero fill drops the
+    # outside couplings whatever their values
+    ref = _sym_apply_np(A.entries, coef[[ent.index(e) for e in A.entries]],
+                        xn)
+    tol = 1e-13 if dtype == torch.float64 else 2e-6
+    sc = max(np.abs(r).max() for r in ref)
+    xc = list(torch.tensor(np.stack(xn), dtype=dtype, device=dev).unbind(0))
+    fc = list(torch.tensor(np.stack(fn), dtype=dtype, device=dev).unbind(0))
+    shapes = [(-1, 1), (-2, 1)] + [(L, 1) for L in (1, 2, 4, 8, 16)] + \
+        [(TILED, t) for t in A.tile_shapes()]
+    for lanes, per in shapes:
+        A.lanes, A.per = lanes, per
+        for a, r in zip(A.apply(xc), ref):
+            assert np.abs(a.double().cpu().numpy() - r).max() <= tol * sc, \
+                (lanes, per)
+        yc = [torch.empty_like(a) for a in xc]
+        A.jacobi(xc, fc, yc, 0.6)
+        for k in range(nvar):
+            q = A.entries.index((k, k, (0, ) * len(nw)))
+            d = A.coef[q].double().cpu().numpy()
+            want = xn[k] + 0.6 * (fn[k] - ref[k]) / d
+            assert np.abs(yc[k].double().cpu().numpy() - want).max() <= \
+                tol * sc, (lanes, per)
 
 
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
