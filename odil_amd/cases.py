@@ -9,8 +9,11 @@ I  inverse wave 512 x 512, 64 sensors: loss + u/c adjoint throughput,
    Adam (lr 1e-3) then device L-BFGS, in FP32 and FP64.
 P  256 Poisson instances of uneven size (65^2 .. 1025^2, FP32): one
    persistent residual + adjoint launch (autotuned tiles) against a
-   per-instance launch loop, and the batched FCG + V-cycle solve replayed
-   from HIP graphs against eager per-instance solves.
+   per-instance launch loop, the batched FCG + V-cycle solve replayed
+   from HIP graphs against eager per-instance solves, and the number of
+   such solves (Newton steps of the linear least-squares problem) that
+   brings every instance to relative RMS residual 1e-6, timed back to
+   back.
 All inputs are synthetic and seeded; timings use HIP events on the launch
 stream.
 """
@@ -204,7 +207,8 @@ def _progress(msg):
     print('# ' + msg, file=sys.stderr, flush=True)
 
 
-def case_p(count=256, seed=0, solve=True):
+def case_p(count=256, seed=0, solve=True, p_tol=1e-6, p_max_solves=20,
+           p_maxiter=20):
     from odil_amd.batched import (PoissonBatch, PoissonBatchSolver,
                                   batch_sizes, gaussian_mixture)
     # The persistent loss/adjoint launch runs the drawn sizes as they are;
@@ -241,7 +245,7 @@ def case_p(count=256, seed=0, solve=True):
         for use_graph in (False, True):
             _progress('P: solver setup, graphs={}'.format(use_graph))
             t0 = time.perf_counter()
-            sol = PoissonBatchSolver(b, precision='mixed', maxiter=10,
+            sol = PoissonBatchSolver(b, precision='mixed', maxiter=p_maxiter,
                                      use_graph=use_graph)
             _progress('P: first solve (graphs={}: eager + capture)'.format(
                 use_graph))
@@ -260,9 +264,34 @@ def case_p(count=256, seed=0, solve=True):
             out[key + '_s'] = t3 - t2
             out[key + '_instances_per_s'] = count / (t3 - t2)
             if use_graph:
-                rms = sol.residual_rms()
-                out['solve_rms_max_rel'] = float(max(
-                    r / np.sqrt(np.mean(s**2)) for r, s in zip(rms, srcs)))
+                rel = lambda: float(max(
+                    r / np.sqrt(np.mean(s**2))
+                    for r, s in zip(sol.residual_rms(), srcs)))
+                out['solve_rms_max_rel'] = rel()
+                # Each solve is one Newton step of the linear least-squares
+                # problem (FP32 Krylov and V-cycle, FP64 residual): repeated,
+                # it refines every instance. Count the solves that bring the
+                # largest relative RMS residual below the tolerance, then time
+                # that many back to back from the zero state.
+                _progress('P: solves to relative residual {:g}'.format(
+                    p_tol))
+                hist = [out['solve_rms_max_rel']]
+                while hist[-1] > p_tol and len(hist) < p_max_solves:
+                    sol.solve()
+                    hist.append(rel())
+                k = len(hist)
+                sol.reset()
+                torch.cuda.synchronize()
+                t4 = time.perf_counter()
+                for _ in range(k):
+                    sol.solve()
+                torch.cuda.synchronize()
+                t5 = time.perf_counter()
+                out['tol'] = {'max_rel_rms': p_tol, 'solves': k,
+                              'reached': hist[-1] <= p_tol,
+                              'rel_rms_per_solve': hist,
+                              'final_max_rel': rel(), 'seconds': t5 - t4,
+                              'instances_per_s': count / (t5 - t4)}
             del sol
             torch.cuda.empty_cache()
     return out
