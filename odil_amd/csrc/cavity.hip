@@ -896,7 +896,7 @@ int odil_cavity_extras_jac(int dtype, int nx, int ny, double dx, double dy,
 
 // Per-block partial counts of the extras kernels (the cavity's tiles).
 int odil_cavity_extras_nblocks(int kind, int dtype, int nx, int ny) {
-  if (kind == 0) return residual_nblocks(nx, ny);
+  if (kind == 0) return residual_nblocks(nx, ny, CavityExtrasOp<float>::R);
   return grad_nblocks(dtype == kF32, nx, ny);
 }
 
@@ -919,7 +919,7 @@ int odil_cavity_extras_jac_entries(int* ent, int* shifts) {
 // Number of per-block partials written by the residual (kind 0) and
 // loss-grad (kind 1) kernels of a 2D operator for the given dtype.
 int odil_cavity_nblocks(int kind, int dtype, int nx, int ny) {
-  if (kind == 0) return residual_nblocks(nx, ny);
+  if (kind == 0) return residual_nblocks(nx, ny, CavityOp<float>::R);
   return grad_nblocks(dtype == kF32, nx, ny);
 }
 

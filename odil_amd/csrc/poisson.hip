@@ -142,7 +142,7 @@ int odil_poisson_loss_grad(int dtype, int n0, int n1, double dx, double dy,
 }
 
 int odil_poisson_nblocks(int kind, int dtype, int n0, int n1) {
-  if (kind == 0) return residual_nblocks(n0, n1);
+  if (kind == 0) return residual_nblocks(n0, n1, PoissonOp<float>::R);
   return grad_nblocks(dtype == kF32, n0, n1);
 }
 
@@ -209,7 +209,7 @@ int odil_laplace_loss_grad(int dtype, int n0, int n1, double dx, double dy,
 }
 
 int odil_laplace_nblocks(int kind, int dtype, int n0, int n1) {
-  if (kind == 0) return residual_nblocks(n0, n1);
+  if (kind == 0) return residual_nblocks(n0, n1, LaplaceOp<float>::R);
   return grad_nblocks(dtype == kF32, n0, n1);
 }
 

@@ -176,8 +176,50 @@ constexpr int kNormStage32 = 2, kNormStage64 = 0;
 constexpr int kNormTx32 = 16, kNormTy32 = 16, kNormTx64 = 8, kNormTy64 = 32;
 
 constexpr int kWavesPerBlock = 4;
-inline int residual_nblocks(int n0, int n1) {
-  return ceil_div(n1, kResTy) * ceil_div(n0, kResTx) * kWavesPerBlock;
+
+// Residual launch geometry: TX x TY tiles over the interior cells (at least
+// R from every face; every computed cell takes the operator's interior
+// path), the last tile of a row or column partial, and one-cell-per-thread
+// blocks over the frame of width R (the operator's generic path). With no
+// interior every cell is a frame cell.
+struct ResSplit {
+  int ntx, nty, nint, nframe, nfb;
+  bool has_inner;
+  __device__ inline void frame_cell(int f, int n0, int n1, int R, int& i,
+                                    int& j) const {
+    if (!has_inner) {
+      i = f / n1;
+      j = f - i * n1;
+      return;
+    }
+    const int band = R * n1;
+    if (f < 2 * band) {  // the first and the last R rows
+      const int r = f / n1;
+      i = r < R ? r : n0 - 2 * R + r;
+      j = f - r * n1;
+      return;
+    }
+    f -= 2 * band;  // the first and the last R columns of the middle rows
+    const int r = f / (2 * R), c = f - r * (2 * R);
+    i = R + r;
+    j = c < R ? c : n1 - 2 * R + c;
+  }
+};
+
+inline ResSplit residual_split(int n0, int n1, int R) {
+  ResSplit s;
+  s.has_inner = n0 > 2 * R && n1 > 2 * R;
+  s.ntx = s.has_inner ? ceil_div(n1 - 2 * R, kResTy) : 0;
+  s.nty = s.has_inner ? ceil_div(n0 - 2 * R, kResTx) : 0;
+  s.nint = s.ntx * s.nty;
+  s.nframe = s.has_inner ? n0 * n1 - (n0 - 2 * R) * (n1 - 2 * R) : n0 * n1;
+  s.nfb = ceil_div(s.nframe, 256);
+  return s;
+}
+
+inline int residual_nblocks(int n0, int n1, int R) {
+  const ResSplit s = residual_split(n0, n1, R);
+  return (s.nint + s.nfb) * kWavesPerBlock;
 }
 inline int grad_nblocks(bool fp32, int n0, int n1) {
   return ceil_div(n1, fp32 ? kGradTy32 : kGradTy64) *
@@ -191,53 +233,67 @@ constexpr size_t residual_smem() {
          (kResTy + 2 * Op::R);
 }
 
-// Residual f_e at every cell, and per-wavefront partial sums of f_e^2. The
-// workgroup's TX x TY tile of the fields, grown by the operator reach R, is
-// staged in LDS with one coalesced read per value; every cell then reads its
-// stencil from LDS. A wavefront covers one row of 64 cells.
+// Residual f_e at every cell, and per-wavefront partial sums of f_e^2.
+// Interior blocks (the first nint) stage their TX x TY tile of the fields,
+// grown by the operator reach R, in LDS with one coalesced read per value
+// (clamped where a partial tile's halo leaves the grid) and evaluate the
+// interior path of the operator from LDS, one wavefront per 64-cell row.
+// Frame blocks evaluate one frame cell per thread from global memory on the
+// operator's generic path (wall rules). Partial slot of block b, wave w:
+// b * 4 + w (fixed order, bitwise reproducible sums).
 template <class T, class Op>
 __global__ void __launch_bounds__(256)
     k_residual(FieldPtrs<T, Op::NF> in, OutPtrs<T, Op::NE> out,
                double* __restrict__ partial, int n0, int n1,
-               typename Op::Params P) {
+               typename Op::Params P, ResSplit sp) {
   constexpr int R = Op::R;
   constexpr int E0 = kResTx + 2 * R, E1 = kResTy + 2 * R;
   constexpr int NPT = kResTx * kResTy / 256;
   static_assert(kResTy == 64, "one wavefront per tile row");
   extern __shared__ unsigned char smem_raw[];
   T* tile = reinterpret_cast<T*>(smem_raw);
-  unsigned bx, by;
-  xcd_block2(bx, by);
-  const int x0 = by * kResTx, y0 = bx * kResTy;
-  stage_tile<T, Op::NF, kResTx, kResTy, R>(in, tile, x0, y0, n0, n1);
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
   double ssq[Op::NE];
   for (int e = 0; e < Op::NE; ++e) ssq[e] = 0;
-  const bool inner = interior_box(x0, x0 + kResTx, y0, y0 + kResTy, n0, n1,
-                                  R);
-  __syncthreads();
-  const int b = threadIdx.x & 63;
-  const int j = y0 + b;
+  if ((int)b < sp.nint) {
+    const int x0 = R + (int)(b / (unsigned)sp.ntx) * kResTx;
+    const int y0 = R + (int)(b % (unsigned)sp.ntx) * kResTy;
+    stage_tile<T, Op::NF, kResTx, kResTy, R>(in, tile, x0, y0, n0, n1);
+    __syncthreads();
+    const int bb = threadIdx.x & 63;
+    const int j = y0 + bb;
 #pragma unroll
-  for (int r = 0; r < NPT; ++r) {
-    const int a = (threadIdx.x >> 6) + 4 * r;
-    const int i = x0 + a;
-    if (i >= n0 || j >= n1) continue;
-    LdsAcc<T, E0, E1> acc{tile + (a + R) * E1 + (b + R)};
-    T f[Op::NE];
-    if (inner)
+    for (int r = 0; r < NPT; ++r) {
+      const int a = (threadIdx.x >> 6) + 4 * r;
+      const int i = x0 + a;
+      if (i >= n0 - R || j >= n1 - R) continue;
+      LdsAcc<T, E0, E1> acc{tile + (a + R) * E1 + (bb + R)};
+      T f[Op::NE];
       Op::template residual<true>(acc, i, j, n0, n1, P, f);
-    else
+      const size_t c = (size_t)i * n1 + j;
+      for (int e = 0; e < Op::NE; ++e) {
+        if (out.f[e]) out.f[e][c] = f[e];
+        ssq[e] += (double)f[e] * (double)f[e];
+      }
+    }
+  } else {
+    const int fc = ((int)b - sp.nint) * 256 + (int)threadIdx.x;
+    if (fc < sp.nframe) {
+      int i, j;
+      sp.frame_cell(fc, n0, n1, R, i, j);
+      GlobalAcc<T, Op::NF> acc{in.f, n0, n1, i, j};
+      T f[Op::NE];
       Op::template residual<false>(acc, i, j, n0, n1, P, f);
-    const size_t c = (size_t)i * n1 + j;
-    for (int e = 0; e < Op::NE; ++e) {
-      if (out.f[e]) out.f[e][c] = f[e];
-      ssq[e] += (double)f[e] * (double)f[e];
+      const size_t c = (size_t)i * n1 + j;
+      for (int e = 0; e < Op::NE; ++e) {
+        if (out.f[e]) out.f[e][c] = f[e];
+        ssq[e] += (double)f[e] * (double)f[e];
+      }
     }
   }
   if (partial)
-    wave_partials<Op::NE>(
-        ssq, partial, (size_t)gridDim.x * gridDim.y * kWavesPerBlock,
-        (size_t)(by * gridDim.x + bx) * kWavesPerBlock + (threadIdx.x >> 6));
+    wave_partials<Op::NE>(ssq, partial, (size_t)gridDim.x * kWavesPerBlock,
+                          (size_t)b * kWavesPerBlock + (threadIdx.x >> 6));
 }
 
 // Jacobian coefficient fields: coef[q] for q < NJ holds entry
@@ -799,12 +855,13 @@ int launch_residual(const void* const* fields, void* const* res,
   for (int k = 0; k < Op::NF; ++k) in.f[k] = static_cast<const T*>(fields[k]);
   for (int e = 0; e < Op::NE; ++e)
     out.f[e] = res ? static_cast<T*>(res[e]) : nullptr;
-  dim3 grid(ceil_div(n1, kResTy), ceil_div(n0, kResTx));
-  if (nblocks) *nblocks = grid.x * grid.y * kWavesPerBlock;
+  const ResSplit sp = residual_split(n0, n1, Op::R);
+  const int nb = sp.nint + sp.nfb;
+  if (nblocks) *nblocks = nb * kWavesPerBlock;
   constexpr size_t smem = residual_smem<T, Op>();
   static_assert(smem <= 64 * 1024, "residual tile too large");
-  hipLaunchKernelGGL((k_residual<T, Op>), grid, dim3(256), smem, stream, in,
-                     out, partial, n0, n1, P);
+  hipLaunchKernelGGL((k_residual<T, Op>), dim3(nb), dim3(256), smem, stream,
+                     in, out, partial, n0, n1, P, sp);
   return check_launch("residual");
 }
 

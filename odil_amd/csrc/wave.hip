@@ -92,7 +92,7 @@ int odil_wave_loss_grad(int dtype, int nt, int nx, double dt, double dx,
 }
 
 int odil_wave_nblocks(int kind, int dtype, int nt, int nx) {
-  if (kind == 0) return residual_nblocks(nt, nx);
+  if (kind == 0) return residual_nblocks(nt, nx, WaveOp<float>::R);
   return grad_nblocks(dtype == kF32, nt, nx);
 }
 
