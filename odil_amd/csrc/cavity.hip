@@ -453,7 +453,8 @@ struct CavityOp {
   template <int TX, int TY>
   __device__ static inline void interior_adjoint_tile(
       const FieldPtrs<T, 3>& in, const OutPtrs<T, 3>& grad, int n0, int n1,
-      int x0, int y0, T scale, const Params& P, T* smem, double* ssq) {
+      int x0, int y0, T scale, const Params& P, T* smem, double* ssq,
+      int v0, int v1) {
     constexpr int H = 3, S0 = TX + 2 * H, S1 = TY + 2 * H, NS1 = S0 * S1;
     constexpr int E0 = TX + 2, E1 = TY + 2, NE1 = E0 * E1;
     constexpr int NPT = TX * TY / 256;
@@ -474,7 +475,8 @@ struct CavityOp {
       F[q] = f[0];
       F[NE1 + q] = f[1];
       F[2 * NE1 + q] = f[2];
-      if (ea >= 1 && ea <= TX && eb >= 1 && eb <= TY) {
+      if (ea >= 1 && ea <= TX && eb >= 1 && eb <= TY &&
+          x0 - 1 + ea < v0 && y0 - 1 + eb < v1) {
 #pragma unroll
         for (int e = 0; e < 3; ++e) ssq[e] += (double)f[e] * (double)f[e];
       }
@@ -484,6 +486,7 @@ struct CavityOp {
     for (int p = 0; p < NPT; ++p) {
       const int q = threadIdx.x + 256 * p;
       const int a = q / TY, b = q - a * TY;
+      if (x0 + a >= v0 || y0 + b >= v1) continue;
       const int e = (a + 1) * E1 + (b + 1);
       T g[3];
       jt_column<E1, NE1, S1, NS1, E1>(F + e, st + (a + H) * S1 + (b + H),
@@ -920,7 +923,7 @@ int odil_cavity_extras_jac_entries(int* ent, int* shifts) {
 // loss-grad (kind 1) kernels of a 2D operator for the given dtype.
 int odil_cavity_nblocks(int kind, int dtype, int nx, int ny) {
   if (kind == 0) return residual_nblocks(nx, ny, CavityOp<float>::R);
-  return grad_nblocks(dtype == kF32, nx, ny);
+  return loss_grad_nblocks<CavityOp<float>>(dtype == kF32, nx, ny);
 }
 
 // Matrix-free normal operator J^T J with J recomputed from `state`
@@ -1035,7 +1038,7 @@ int odil_cavity_loss_grad_noalloc(int dtype, int nx, int ny, double dx,
     set_error("cavity_loss_grad_noalloc: bad dtype %d", dtype);
     return kErrArg;
   }
-  const int need = grad_nblocks(dtype == kF32, nx, ny);
+  const int need = loss_grad_nblocks<CavityOp<float>>(dtype == kF32, nx, ny);
   if (need > kNoAllocParts) {
     set_error("cavity_loss_grad_noalloc: %d x %d needs %d partials per "
               "equation, more than the %d of the library's scratch",

@@ -177,48 +177,58 @@ constexpr int kNormTx32 = 16, kNormTy32 = 16, kNormTx64 = 8, kNormTy64 = 32;
 
 constexpr int kWavesPerBlock = 4;
 
-// Residual launch geometry: TX x TY tiles over the interior cells (at least
-// R from every face; every computed cell takes the operator's interior
-// path), the last tile of a row or column partial, and one-cell-per-thread
-// blocks over the frame of width R (the operator's generic path). With no
-// interior every cell is a frame cell.
-struct ResSplit {
+// Thin-frame launch geometry: TX x TY tiles over the interior cells (at
+// least H from every face), the last tile of a row or column partial, and
+// frame blocks over the cells within H of a face, `per` frame cells per
+// block. With no interior every cell is a frame cell. Used by the residual
+// (H = R, every tile on the operator's interior path) and by the operators
+// with interior tiles (H = R + RJ).
+struct ThinSplit {
   int ntx, nty, nint, nframe, nfb;
   bool has_inner;
-  __device__ inline void frame_cell(int f, int n0, int n1, int R, int& i,
+  // Frame cell f -> (i, j): the first and last H rows, then the first and
+  // last H columns of the middle rows.
+  __device__ inline void frame_cell(int f, int n0, int n1, int H, int& i,
                                     int& j) const {
     if (!has_inner) {
       i = f / n1;
       j = f - i * n1;
       return;
     }
-    const int band = R * n1;
-    if (f < 2 * band) {  // the first and the last R rows
+    const int band = H * n1;
+    if (f < 2 * band) {
       const int r = f / n1;
-      i = r < R ? r : n0 - 2 * R + r;
+      i = r < H ? r : n0 - 2 * H + r;
       j = f - r * n1;
       return;
     }
-    f -= 2 * band;  // the first and the last R columns of the middle rows
-    const int r = f / (2 * R), c = f - r * (2 * R);
-    i = R + r;
-    j = c < R ? c : n1 - 2 * R + c;
+    f -= 2 * band;
+    const int r = f / (2 * H), c = f - r * (2 * H);
+    i = H + r;
+    j = c < H ? c : n1 - 2 * H + c;
   }
 };
 
-inline ResSplit residual_split(int n0, int n1, int R) {
-  ResSplit s;
-  s.has_inner = n0 > 2 * R && n1 > 2 * R;
-  s.ntx = s.has_inner ? ceil_div(n1 - 2 * R, kResTy) : 0;
-  s.nty = s.has_inner ? ceil_div(n0 - 2 * R, kResTx) : 0;
+inline ThinSplit thin_split(int n0, int n1, int H, int TX, int TY, int per) {
+  ThinSplit s;
+  s.has_inner = n0 > 2 * H && n1 > 2 * H;
+  s.ntx = s.has_inner ? ceil_div(n1 - 2 * H, TY) : 0;
+  s.nty = s.has_inner ? ceil_div(n0 - 2 * H, TX) : 0;
   s.nint = s.ntx * s.nty;
-  s.nframe = s.has_inner ? n0 * n1 - (n0 - 2 * R) * (n1 - 2 * R) : n0 * n1;
-  s.nfb = ceil_div(s.nframe, 256);
+  s.nframe = s.has_inner ? n0 * n1 - (n0 - 2 * H) * (n1 - 2 * H) : n0 * n1;
+  // Frame blocks come first in dispatch order, padded to a multiple of the
+  // XCD count: they are dealt round-robin over the XCDs, and the interior
+  // tiles after them keep xcd_block's alignment.
+  s.nfb = ceil_div(ceil_div(s.nframe, per), (int)kNumXcd) * (int)kNumXcd;
   return s;
 }
 
+inline ThinSplit residual_split(int n0, int n1, int R) {
+  return thin_split(n0, n1, R, kResTx, kResTy, 256);
+}
+
 inline int residual_nblocks(int n0, int n1, int R) {
-  const ResSplit s = residual_split(n0, n1, R);
+  const ThinSplit s = residual_split(n0, n1, R);
   return (s.nint + s.nfb) * kWavesPerBlock;
 }
 inline int grad_nblocks(bool fp32, int n0, int n1) {
@@ -234,30 +244,33 @@ constexpr size_t residual_smem() {
 }
 
 // Residual f_e at every cell, and per-wavefront partial sums of f_e^2.
-// Interior blocks (the first nint) stage their TX x TY tile of the fields,
+// Frame blocks (the first nfb in dispatch order, spread over the XCDs)
+// evaluate one frame cell per
+// thread from global memory on the operator's generic path (wall rules).
+// Interior blocks stage their TX x TY tile of the fields,
 // grown by the operator reach R, in LDS with one coalesced read per value
 // (clamped where a partial tile's halo leaves the grid) and evaluate the
 // interior path of the operator from LDS, one wavefront per 64-cell row.
-// Frame blocks evaluate one frame cell per thread from global memory on the
-// operator's generic path (wall rules). Partial slot of block b, wave w:
+// Partial slot of block b, wave w:
 // b * 4 + w (fixed order, bitwise reproducible sums).
 template <class T, class Op>
 __global__ void __launch_bounds__(256)
     k_residual(FieldPtrs<T, Op::NF> in, OutPtrs<T, Op::NE> out,
                double* __restrict__ partial, int n0, int n1,
-               typename Op::Params P, ResSplit sp) {
+               typename Op::Params P, ThinSplit sp) {
   constexpr int R = Op::R;
   constexpr int E0 = kResTx + 2 * R, E1 = kResTy + 2 * R;
   constexpr int NPT = kResTx * kResTy / 256;
   static_assert(kResTy == 64, "one wavefront per tile row");
   extern __shared__ unsigned char smem_raw[];
   T* tile = reinterpret_cast<T*>(smem_raw);
-  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const unsigned b = blockIdx.x;
   double ssq[Op::NE];
   for (int e = 0; e < Op::NE; ++e) ssq[e] = 0;
-  if ((int)b < sp.nint) {
-    const int x0 = R + (int)(b / (unsigned)sp.ntx) * kResTx;
-    const int y0 = R + (int)(b % (unsigned)sp.ntx) * kResTy;
+  if ((int)b >= sp.nfb) {
+    const unsigned t = xcd_block(b - (unsigned)sp.nfb, (unsigned)sp.nint);
+    const int x0 = R + (int)(t / (unsigned)sp.ntx) * kResTx;
+    const int y0 = R + (int)(t % (unsigned)sp.ntx) * kResTy;
     stage_tile<T, Op::NF, kResTx, kResTy, R>(in, tile, x0, y0, n0, n1);
     __syncthreads();
     const int bb = threadIdx.x & 63;
@@ -277,7 +290,7 @@ __global__ void __launch_bounds__(256)
       }
     }
   } else {
-    const int fc = ((int)b - sp.nint) * 256 + (int)threadIdx.x;
+    const int fc = (int)b * 256 + (int)threadIdx.x;
     if (fc < sp.nframe) {
       int i, j;
       sp.frame_cell(fc, n0, n1, R, i, j);
@@ -781,35 +794,128 @@ inline TileSplit tile_split(int n0, int n1, int TX, int TY, int R, int RJ) {
   return s;
 }
 
-// k_loss_grad over a split grid: frame tiles run loss_grad_tile, interior
-// tiles the operator's interior_adjoint_tile. Partials keep k_loss_grad's
-// layout (slot (ty ntx + tx) waves + wave).
+// Loss adjoint of frame cells (within H = R + RJ of a face) on the
+// operator's generic path, for frame block fb of a thin split: each thread
+// evaluates f and J at one neighbour m = c - shift_s of one frame cell c
+// (NS threads per cell, 256 / NS cells per block) and keeps
+// out_{k,s} = sum_e J_e(m)[k][s] f_e(m) in LDS; one thread per cell then
+// sums grad_k(c) = scale sum_s out_{k,s} in the order of loss_grad_tile
+// (centre first). The thread of shift 0 adds f_e(c)^2 to ssq[e].
+template <class T, class Op>
+__device__ inline void frame_adjoint(const FieldPtrs<T, Op::NF>& in,
+                                     const OutPtrs<T, Op::NF>& grad, int n0,
+                                     int n1, T scale,
+                                     const typename Op::Params& P,
+                                     const ThinSplit& sp, int fb, T* o,
+                                     double* ssq) {
+  constexpr int NF = Op::NF, NE = Op::NE, NS = Op::NS;
+  constexpr int H = Op::R + Op::RJ, PER = 256 / NS;
+  const int t = threadIdx.x;
+  const int cl = t / NS, sh = t - cl * NS;
+  const int f = fb * PER + cl;
+  if (cl < PER && f < sp.nframe) {
+    int i, j;
+    sp.frame_cell(f, n0, n1, H, i, j);
+    // The shift is selected by unrolled compares so that J stays in
+    // registers (a run-time index would move it to scratch memory).
+    int mi = i, mj = j;
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+      if (q == sh) {
+        mi = i - Op::SX[q];
+        mj = j - Op::SY[q];
+      }
+    T out[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) out[k] = 0;
+    if (mi >= 0 && mi < n0 && mj >= 0 && mj < n1) {
+      GlobalAcc<T, NF> acc{in.f, n0, n1, mi, mj};
+      T fe[NE];
+      T J[NE][NF][NS];
+      Op::template eval<false>(acc, mi, mj, n0, n1, P, fe, J);
+#pragma unroll
+      for (int q = 0; q < NS; ++q)
+        if (q == sh)
+#pragma unroll
+          for (int k = 0; k < NF; ++k) {
+            T v = 0;
+#pragma unroll
+            for (int e = 0; e < NE; ++e) v += J[e][k][q] * fe[e];
+            out[k] = v;
+          }
+      if (sh == 0)
+        for (int e = 0; e < NE; ++e) ssq[e] += (double)fe[e] * (double)fe[e];
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) o[(cl * NS + sh) * NF + k] = out[k];
+  }
+  __syncthreads();
+  const int fc = fb * PER + t;
+  if (t < PER && fc < sp.nframe) {
+    int i, j;
+    sp.frame_cell(fc, n0, n1, H, i, j);
+    const size_t c = (size_t)i * n1 + j;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      T g = o[(t * NS) * NF + k];
+#pragma unroll
+      for (int q = 1; q < NS; ++q) g += o[(t * NS + q) * NF + k];
+      if (grad.f[k]) {
+        if constexpr (op_accumulates<Op>::value)
+          grad.f[k][c] += g * scale;
+        else
+          grad.f[k][c] = g * scale;
+      }
+    }
+  }
+}
+
+// k_loss_grad over a thin split: the first nfb blocks in dispatch order run
+// frame_adjoint (dispatched first and spread over the XCDs, so that their
+// dependent global loads overlap the tiles), the
+// others the operator's interior_adjoint_tile on tiles of interior cells
+// (H = R + RJ from every face; the last tile of a row or column partial).
+// Partial slot of block b, wave w: b * 4 + w.
 template <class T, class Op, int TX, int TY>
 __global__ void __launch_bounds__(256)
     k_loss_grad_split(FieldPtrs<T, Op::NF> in, OutPtrs<T, Op::NF> grad,
                       double* __restrict__ partial, int n0, int n1, T scale,
-                      typename Op::Params P, TileSplit sp) {
+                      typename Op::Params P, ThinSplit sp) {
   extern __shared__ unsigned char smem_raw[];
   T* smem = reinterpret_cast<T*>(smem_raw);
-  const int b = blockIdx.x;
+  constexpr int H = Op::R + Op::RJ;
+  const unsigned b = blockIdx.x;
   double ssq[Op::NE];
   for (int e = 0; e < Op::NE; ++e) ssq[e] = 0;
-  int ty, tx;
-  if (b < sp.fpad) {
-    const int f = (int)xcd_block(b, sp.fpad);
-    if (f >= sp.nframe) return;
-    sp.frame_tile(f, ty, tx);
-    loss_grad_tile<T, Op, TX, TY>(in, grad, n0, n1, ty * TX, tx * TY, scale,
-                                  P, smem, ssq);
+  if ((int)b >= sp.nfb) {
+    const unsigned t = xcd_block(b - (unsigned)sp.nfb, (unsigned)sp.nint);
+    const int x0 = H + (int)(t / (unsigned)sp.ntx) * TX;
+    const int y0 = H + (int)(t % (unsigned)sp.ntx) * TY;
+    Op::template interior_adjoint_tile<TX, TY>(in, grad, n0, n1, x0, y0,
+                                               scale, P, smem, ssq, n0 - H,
+                                               n1 - H);
   } else {
-    sp.inner_tile((int)xcd_block(b - sp.fpad, sp.ninner), ty, tx);
-    Op::template interior_adjoint_tile<TX, TY>(in, grad, n0, n1, ty * TX,
-                                               tx * TY, scale, P, smem, ssq);
+    frame_adjoint<T, Op>(in, grad, n0, n1, scale, P, sp, (int)b, smem, ssq);
   }
   if (partial)
-    wave_partials<Op::NE>(
-        ssq, partial, (size_t)sp.ntx * sp.nty * kWavesPerBlock,
-        (size_t)(ty * sp.ntx + tx) * kWavesPerBlock + (threadIdx.x >> 6));
+    wave_partials<Op::NE>(ssq, partial, (size_t)gridDim.x * kWavesPerBlock,
+                          (size_t)b * kWavesPerBlock + (threadIdx.x >> 6));
+}
+
+// Partial count of the loss-adjoint launch of operator Op (TX x TY tiles).
+template <class Op, int TX, int TY>
+inline int loss_grad_nblocks_t(int n0, int n1) {
+  if constexpr (has_interior_tiles<Op>::value) {
+    const ThinSplit sp =
+        thin_split(n0, n1, Op::R + Op::RJ, TX, TY, 256 / Op::NS);
+    return (sp.nint + sp.nfb) * kWavesPerBlock;
+  }
+  return ceil_div(n1, TY) * ceil_div(n0, TX) * kWavesPerBlock;
+}
+template <class Op>
+inline int loss_grad_nblocks(bool fp32, int n0, int n1) {
+  return fp32 ? loss_grad_nblocks_t<Op, kGradTx32, kGradTy32>(n0, n1)
+              : loss_grad_nblocks_t<Op, kGradTx64, kGradTy64>(n0, n1);
 }
 
 // k_normal_apply (modes 0, 1, 2) over a split grid.
@@ -855,7 +961,7 @@ int launch_residual(const void* const* fields, void* const* res,
   for (int k = 0; k < Op::NF; ++k) in.f[k] = static_cast<const T*>(fields[k]);
   for (int e = 0; e < Op::NE; ++e)
     out.f[e] = res ? static_cast<T*>(res[e]) : nullptr;
-  const ResSplit sp = residual_split(n0, n1, Op::R);
+  const ThinSplit sp = residual_split(n0, n1, Op::R);
   const int nb = sp.nint + sp.nfb;
   if (nblocks) *nblocks = nb * kWavesPerBlock;
   constexpr size_t smem = residual_smem<T, Op>();
@@ -892,24 +998,26 @@ int launch_loss_grad_t(const void* const* fields, void* const* grad,
     out.f[k] = static_cast<T*>(grad[k]);
   }
   dim3 grid(ceil_div(n1, TY), ceil_div(n0, TX));
-  if (nblocks) *nblocks = grid.x * grid.y * kWavesPerBlock;
   constexpr size_t smem = loss_grad_smem<T, Op, TX, TY>();
   static_assert(smem <= 160 * 1024, "LDS tile too large");
   if constexpr (has_interior_tiles<Op>::value) {
-    const TileSplit sp = tile_split(n0, n1, TX, TY, Op::R, Op::RJ);
-    if (sp.ninner > 0) {
-      constexpr size_t smem2 =
-          std::max(smem, Op::template interior_adjoint_smem<TX, TY>());
-      static_assert(smem2 <= 160 * 1024, "LDS tile too large");
-      auto kern = k_loss_grad_split<T, Op, TX, TY>;
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)smem2);
-      hipLaunchKernelGGL(kern, dim3(sp.fpad + sp.ninner), dim3(256), smem2,
-                         stream, in, out, partial, n0, n1, scale, P, sp);
-      return check_launch("loss_grad");
-    }
+    const ThinSplit sp =
+        thin_split(n0, n1, Op::R + Op::RJ, TX, TY, 256 / Op::NS);
+    const int nb = sp.nint + sp.nfb;
+    if (nblocks) *nblocks = nb * kWavesPerBlock;
+    constexpr size_t smem2 = std::max(
+        Op::template interior_adjoint_smem<TX, TY>(),
+        sizeof(T) * (size_t)(256 / Op::NS) * Op::NS * Op::NF);
+    static_assert(smem2 <= 160 * 1024, "LDS tile too large");
+    auto kern = k_loss_grad_split<T, Op, TX, TY>;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)smem2);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), smem2, stream, in, out,
+                       partial, n0, n1, scale, P, sp);
+    return check_launch("loss_grad");
   }
+  if (nblocks) *nblocks = grid.x * grid.y * kWavesPerBlock;
   auto kern = k_loss_grad<T, Op, TX, TY>;
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
