@@ -514,7 +514,7 @@ struct CavityOp {
       const FieldPtrs<T, 3>& st_in, const FieldPtrs<T, 3>& x_in,
       const FieldPtrs<T, 3>& b_in, const FieldPtrs<T, 3>& d_in,
       const OutPtrs<T, 3>& y, int n0, int n1, T omega, int mode,
-      const Params& P, int x0, int y0, T* smem) {
+      const Params& P, int x0, int y0, T* smem, int v0, int v1) {
     constexpr int A0 = TX + 4, A1 = TY + 4, NA = A0 * A1;
     constexpr int E0 = TX + 2, E1 = TY + 2, NE1 = E0 * E1;
     constexpr int NPT = TX * TY / 256;
@@ -531,7 +531,8 @@ struct CavityOp {
       for (int p = 0; p < NPT; ++p) {
         const int q = threadIdx.x + 256 * p;
         const int a = q / TY, b = q - a * TY;
-        const size_t c = (size_t)(x0 + a) * n1 + (y0 + b);
+        const bool ok = x0 + a < v0 && y0 + b < v1;
+        const size_t c = ok ? (size_t)(x0 + a) * n1 + (y0 + b) : 0;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
           bv[p][k] = b_in.f[k][c];
@@ -540,15 +541,19 @@ struct CavityOp {
       }
     }
     {
-      // The tile grown by 2 lies inside the grid: no clamping.
+      // The tile grown by 2 starts inside the grid (x0, y0 >= 3); a partial
+      // tile's far side is clamped (its values only reach the products of
+      // cells outside [.., v0) x [.., v1), which are not stored).
+      const bool inside = x0 + TX + 2 <= n0 && y0 + TY + 2 <= n1;
       T vu[NL], vv[NL], vx[3][NL];
-      const size_t base = (size_t)(x0 - 2) * n1 + (y0 - 2);
 #pragma unroll
       for (int r = 0; r < NL; ++r) {
         const int q = threadIdx.x + 256 * r;
         if (r + 1 < NL || q < NA) {
           const int a = q / A1, b = q - a * A1;
-          const size_t o = base + (size_t)a * n1 + b;
+          const int gi = inside ? x0 - 2 + a : min(x0 - 2 + a, n0 - 1);
+          const int gj = inside ? y0 - 2 + b : min(y0 - 2 + b, n1 - 1);
+          const size_t o = (size_t)gi * n1 + gj;
           vu[r] = st_in.f[0][o];
           vv[r] = st_in.f[1][o];
 #pragma unroll
@@ -582,6 +587,7 @@ struct CavityOp {
     for (int p = 0; p < NPT; ++p) {
       const int q = threadIdx.x + 256 * p;
       const int a = q / TY, b = q - a * TY;
+      if (x0 + a >= v0 || y0 + b >= v1) continue;
       const int e = (a + 1) * E1 + (b + 1);
       const int s = (a + 2) * A1 + (b + 2);
       T g[3];

@@ -728,71 +728,21 @@ constexpr size_t normal_apply_smem() {
                             : 0));
 }
 
-// Operators may supply faster code for interior tiles (the tile grown by
-// RJ keeps distance R from every wall, so no wall rule applies in it):
+// Operators may supply faster code for tiles of interior cells (cells at
+// least R + RJ from every wall: no wall rule applies to them or to the
+// Jacobian rows they gather from):
 //   static constexpr bool INTERIOR_TILES = true;
 //   interior_adjoint_smem<TX, TY>(), interior_adjoint_tile<TX, TY>(...)
 //   interior_normal_smem<TX, TY>(),  interior_normal_tile<TX, TY>(...)
-// The loss-adjoint and normal-operator launches then split the tile grid
-// into those interior tiles and the frame of the others (generic path).
+// taking the bounds (v0, v1) of the interior cells a partial tile may
+// store. The loss-adjoint and normal-operator launches then run those tiles
+// over the interior and per-cell frame blocks over the frame of width
+// R + RJ (generic path; a thin split).
 template <class Op, class = void>
 struct has_interior_tiles : std::false_type {};
 template <class Op>
 struct has_interior_tiles<Op, std::void_t<decltype(Op::INTERIOR_TILES)>>
     : std::bool_constant<Op::INTERIOR_TILES> {};
-
-// Interior tiles are rows [tyl, tyh) x columns [txl, txh) of the TX x TY
-// tile grid; the nframe others are numbered top rows, bottom rows, then
-// the left and right ends of the middle rows. A split launch runs fpad
-// (nframe rounded up to a multiple of the XCD count) frame blocks first,
-// then one block per interior tile; both groups are dealt to the XCDs in
-// contiguous bands (xcd_block).
-struct TileSplit {
-  int ntx, nty, tyl, tyh, txl, txh, nframe, fpad, ninner;
-  __device__ inline void frame_tile(int f, int& ty, int& tx) const {
-    const int top = tyl * ntx;
-    if (f < top) {
-      ty = f / ntx;
-      tx = f - ty * ntx;
-      return;
-    }
-    f -= top;
-    const int bottom = (nty - tyh) * ntx;
-    if (f < bottom) {
-      ty = tyh + f / ntx;
-      tx = f - (ty - tyh) * ntx;
-      return;
-    }
-    f -= bottom;
-    const int side = txl + (ntx - txh);
-    ty = tyl + f / side;
-    const int r = f - (ty - tyl) * side;
-    tx = r < txl ? r : txh + (r - txl);
-  }
-  __device__ inline void inner_tile(int i, int& ty, int& tx) const {
-    const int w = txh - txl;
-    ty = tyl + i / w;
-    tx = txl + (i - (ty - tyl) * w);
-  }
-};
-
-// Tile x0 = ty TX is interior when x0 - RJ >= R and x0 + TX + RJ <= n0 - R
-// (likewise along axis 1).
-inline TileSplit tile_split(int n0, int n1, int TX, int TY, int R, int RJ) {
-  TileSplit s;
-  s.ntx = ceil_div(n1, TY);
-  s.nty = ceil_div(n0, TX);
-  s.tyl = std::min(s.nty, ceil_div(R + RJ, TX));
-  s.tyh = std::max(s.tyl,
-                   std::min(s.nty, n0 - R - RJ >= 0 ? (n0 - R - RJ) / TX : 0));
-  s.txl = std::min(s.ntx, ceil_div(R + RJ, TY));
-  s.txh = std::max(s.txl,
-                   std::min(s.ntx, n1 - R - RJ >= 0 ? (n1 - R - RJ) / TY : 0));
-  s.ninner = (s.tyh - s.tyl) * (s.txh - s.txl);
-  s.nframe = s.ntx * s.nty - s.ninner;
-  s.fpad = ceil_div(s.nframe, kNumXcd) * kNumXcd;
-  return s;
-}
 
 // Loss adjoint of frame cells (within H = R + RJ of a face) on the
 // operator's generic path, for frame block fb of a thin split: each thread
@@ -918,30 +868,120 @@ inline int loss_grad_nblocks(bool fp32, int n0, int n1) {
               : loss_grad_nblocks_t<Op, kGradTx64, kGradTy64>(n0, n1);
 }
 
-// k_normal_apply (modes 0, 1, 2) over a split grid.
+// Normal operator N = J^T J (modes 0, 1, 2 of normal_apply_tile) at frame
+// cells (within H = R + RJ of a face) on the operator's generic path, for
+// frame block fb of a thin split: each thread evaluates J at one neighbour
+// m = c - shift_s of one frame cell c (NS threads per cell), v = (J x)(m)
+// from clamped global reads of x (coefficients reaching outside are zero),
+// and keeps out_{k,s} = sum_e J_e(m)[k][s] v_e in LDS; one thread per cell
+// sums g_k = sum_s out_{k,s} (centre first, as normal_apply_tile) and
+// stores g, b - g or the Jacobi update.
+template <class T, class Op>
+__device__ inline void frame_normal(
+    const FieldPtrs<T, Op::NF>& st_in, const FieldPtrs<T, Op::NF>& x_in,
+    const FieldPtrs<T, Op::NF>& b_in, const FieldPtrs<T, Op::NF>& d_in,
+    const OutPtrs<T, Op::NF>& y, int n0, int n1, T omega, int mode,
+    const typename Op::Params& P, const ThinSplit& sp, int fb, T* o) {
+  constexpr int NF = Op::NF, NE = Op::NE, NS = Op::NS;
+  constexpr int H = Op::R + Op::RJ, PER = 256 / NS;
+  const int t = threadIdx.x;
+  const int cl = t / NS, sh = t - cl * NS;
+  const int f = fb * PER + cl;
+  if (cl < PER && f < sp.nframe) {
+    int i, j;
+    sp.frame_cell(f, n0, n1, H, i, j);
+    // The shift is selected by unrolled compares so that J stays in
+    // registers.
+    int mi = i, mj = j;
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+      if (q == sh) {
+        mi = i - Op::SX[q];
+        mj = j - Op::SY[q];
+      }
+    T out[NF];
+#pragma unroll
+    for (int k = 0; k < NF; ++k) out[k] = 0;
+    if (mi >= 0 && mi < n0 && mj >= 0 && mj < n1) {
+      GlobalAcc<T, NF> acc{st_in.f, n0, n1, mi, mj};
+      T fe[NE];
+      T J[NE][NF][NS];
+      Op::template eval<false>(acc, mi, mj, n0, n1, P, fe, J);
+      T v[NE];
+#pragma unroll
+      for (int e = 0; e < NE; ++e) v[e] = 0;
+#pragma unroll
+      for (int k = 0; k < NF; ++k)
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+          const int xi = min(max(mi + Op::SX[q], 0), n0 - 1);
+          const int xj = min(max(mj + Op::SY[q], 0), n1 - 1);
+          const T xv = x_in.f[k][(size_t)xi * n1 + xj];
+#pragma unroll
+          for (int e = 0; e < NE; ++e) v[e] += J[e][k][q] * xv;
+        }
+#pragma unroll
+      for (int q = 0; q < NS; ++q)
+        if (q == sh)
+#pragma unroll
+          for (int k = 0; k < NF; ++k) {
+            T a = 0;
+#pragma unroll
+            for (int e = 0; e < NE; ++e) a += J[e][k][q] * v[e];
+            out[k] = a;
+          }
+    }
+#pragma unroll
+    for (int k = 0; k < NF; ++k) o[(cl * NS + sh) * NF + k] = out[k];
+  }
+  __syncthreads();
+  const int fc = fb * PER + t;
+  if (t < PER && fc < sp.nframe) {
+    int i, j;
+    sp.frame_cell(fc, n0, n1, H, i, j);
+    const size_t c = (size_t)i * n1 + j;
+#pragma unroll
+    for (int k = 0; k < NF; ++k) {
+      T g = o[(t * NS) * NF + k];
+#pragma unroll
+      for (int q = 1; q < NS; ++q) g += o[(t * NS + q) * NF + k];
+      T r;
+      if (mode == 0)
+        r = g;
+      else if (mode == 1)
+        r = b_in.f[k][c] - g;
+      else
+        r = x_in.f[k][c] + omega * (b_in.f[k][c] - g) / d_in.f[k][c];
+      y.f[k][c] = r;
+    }
+  }
+}
+
+// k_normal_apply (modes 0, 1, 2) over a thin split: the first nfb blocks in
+// dispatch order run frame_normal, the others the operator's
+// interior_normal_tile on tiles of interior cells (the last tile of a row
+// or column partial).
 template <class T, class Op, int TX, int TY, int STG>
 __global__ void __launch_bounds__(256)
     k_normal_apply_split(FieldPtrs<T, Op::NF> st_in,
                          FieldPtrs<T, Op::NF> x_in, FieldPtrs<T, Op::NF> b_in,
                          FieldPtrs<T, Op::NF> d_in, OutPtrs<T, Op::NF> y,
                          int n0, int n1, T omega, int mode,
-                         typename Op::Params P, TileSplit sp) {
+                         typename Op::Params P, ThinSplit sp) {
   extern __shared__ unsigned char smem_raw[];
   T* smem = reinterpret_cast<T*>(smem_raw);
-  const int b = blockIdx.x;
-  int ty, tx;
-  if (b < sp.fpad) {
-    const int f = (int)xcd_block(b, sp.fpad);
-    if (f >= sp.nframe) return;
-    sp.frame_tile(f, ty, tx);
-    normal_apply_tile<T, Op, TX, TY, STG>(st_in, x_in, b_in, d_in, y, n0, n1,
-                                          omega, mode, P, ty * TX, tx * TY,
-                                          smem);
-  } else {
-    sp.inner_tile((int)xcd_block(b - sp.fpad, sp.ninner), ty, tx);
+  constexpr int H = Op::R + Op::RJ;
+  const unsigned b = blockIdx.x;
+  if ((int)b >= sp.nfb) {
+    const unsigned t = xcd_block(b - (unsigned)sp.nfb, (unsigned)sp.nint);
+    const int x0 = H + (int)(t / (unsigned)sp.ntx) * TX;
+    const int y0 = H + (int)(t % (unsigned)sp.ntx) * TY;
     Op::template interior_normal_tile<TX, TY>(st_in, x_in, b_in, d_in, y,
-                                              n0, n1, omega, mode, P,
-                                              ty * TX, tx * TY, smem);
+                                              n0, n1, omega, mode, P, x0, y0,
+                                              smem, n0 - H, n1 - H);
+  } else {
+    frame_normal<T, Op>(st_in, x_in, b_in, d_in, y, n0, n1, omega, mode, P,
+                        sp, (int)b, smem);
   }
 }
 
@@ -1054,16 +1094,18 @@ int launch_normal_apply_t(const void* const* state, const void* const* x,
   constexpr size_t smem = normal_apply_smem<T, Op, TX, TY, STG>();
   static_assert(smem <= 160 * 1024, "LDS tile too large");
   if constexpr (has_interior_tiles<Op>::value) {
-    const TileSplit sp = tile_split(n0, n1, TX, TY, Op::R, Op::RJ);
-    if (mode != 3 && sp.ninner > 0) {
-      constexpr size_t smem2 =
-          std::max(smem, Op::template interior_normal_smem<TX, TY>());
+    if (mode != 3) {
+      const ThinSplit sp =
+          thin_split(n0, n1, Op::R + Op::RJ, TX, TY, 256 / Op::NS);
+      constexpr size_t smem2 = std::max(
+          Op::template interior_normal_smem<TX, TY>(),
+          sizeof(T) * (size_t)(256 / Op::NS) * Op::NS * Op::NF);
       static_assert(smem2 <= 160 * 1024, "LDS tile too large");
       auto kern = k_normal_apply_split<T, Op, TX, TY, STG>;
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)smem2);
-      hipLaunchKernelGGL(kern, dim3(sp.fpad + sp.ninner), dim3(256), smem2,
+      hipLaunchKernelGGL(kern, dim3(sp.nfb + sp.nint), dim3(256), smem2,
                          stream, S, X, B, D, Y, n0, n1, (T)omega, mode, P,
                          sp);
       return check_launch("normal_apply");
