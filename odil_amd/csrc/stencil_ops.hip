@@ -11,6 +11,7 @@
 // coef[nent][nwh]; the entry table is int32 [nent][5] = (out, in, s0, s1, s2)
 // sorted by `out`, with ostart[nvar_out + 1] delimiting each output field.
 #include <algorithm>
+#include <type_traits>
 
 #include "common.h"
 
@@ -245,39 +246,77 @@ __global__ void __launch_bounds__(256)
                        i2 >= reach.n2 && i2 < nw.n2 - reach.n2;
     T acc = 0;
     // One loop per wavefront: one that holds any cell near a face runs the
-    // masked loop for all its cells instead of both loops in turn.
+    // masked loop for all its cells instead of both loops in turn. Items go
+    // in batches of U per lane, every load of a batch issued before its
+    // products: full batches first, then one predicated batch (items past
+    // the end reread the last item with a zero coefficient). A remainder
+    // loop would wait for each item's loads in turn, one memory round trip
+    // per item.
+    constexpr int U = LANES <= 2 ? 16 : (LANES == 4 ? 12 : (LANES == 8 ? 8 : 4));
+    constexpr int UM = LANES <= 4 ? 8 : U;
     if (__all(inner)) {
       if (CONTIG) {
         const T* __restrict__ xb = x.p[0];
-#pragma unroll 16
-        for (int e = lane; e < ne; e += LANES) {
-          const int2 it = item(e);
-          acc += at(coef, it.x) * at(xb, it.y);
-        }
+        auto batch = [&](int e0, auto tail) {
+          T pa[U], pb[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int2 it = item(decltype(tail)::value ? min(e0 + u * LANES, ne - 1)
+                                                       : e0 + u * LANES);
+            pa[u] = at(coef, it.x);
+            pb[u] = at(xb, it.y);
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            acc += (!decltype(tail)::value || e0 + u * LANES < ne ? pa[u] : T(0)) * pb[u];
+        };
+        int e0 = lane;
+        for (; e0 + (U - 1) * LANES < ne; e0 += U * LANES)
+          batch(e0, std::false_type{});
+        if (e0 < ne) batch(e0, std::true_type{});
       } else {
-#pragma unroll 4
-        for (int e = lane; e < ne; e += LANES) {
-          const int2 it = item(e);
-          acc += at(coef, it.x) * at(pick(x, var_of(e)), it.y);
-        }
+        auto batch = [&](int e0, auto tail) {
+          T pa[UM], pb[UM];
+#pragma unroll
+          for (int u = 0; u < UM; ++u) {
+            const int e = decltype(tail)::value ? min(e0 + u * LANES, ne - 1)
+                                                : e0 + u * LANES;
+            const int2 it = item(e);
+            pa[u] = at(coef, it.x);
+            pb[u] = at(pick(x, var_of(e)), it.y);
+          }
+#pragma unroll
+          for (int u = 0; u < UM; ++u)
+            acc += (!decltype(tail)::value || e0 + u * LANES < ne ? pa[u] : T(0)) * pb[u];
+        };
+        int e0 = lane;
+        for (; e0 + (UM - 1) * LANES < ne; e0 += UM * LANES)
+          batch(e0, std::false_type{});
+        if (e0 < ne) batch(e0, std::true_type{});
       }
     } else {
-      // Cells near a face: products outside the grid masked to zero. (These
-      // wavefronts run beside the interior ones; unrolled by 8 the level-1
-      // sweep of the 1025^2 cavity took 25.9 us against 27.4 by 4, FP64
-      // 44.0 against 46.7; by 16 FP64 slowed to 53.9.)
-#pragma unroll 8
-      for (int e = lane; e < ne; e += LANES) {
-        const int2 it = item(e);
-        const int2 sh = shifts(e);
-        const bool ok = shifted_inside(i0, i1, i2, sh.x, nw) &
-                        shifted_inside(i0, i1, i2, sh.y, nw);
-        const T* xp = CONTIG ? x.p[0] : pick(x, (unsigned)sh.y >> 30);
-        const T a = B32 ? ld_b32(coef, ok ? cs + (unsigned)it.x : 0u)
-                        : coef[ok ? (unsigned)(c + it.x) : 0u];
-        const T b = B32 ? ld_b32(xp, ok ? cs + (unsigned)it.y : 0u)
-                        : xp[ok ? (unsigned)(c + it.y) : 0u];
-        acc += ok ? a * b : T(0);
+      // Cells near a face: products outside the grid masked to zero (their
+      // loads read element 0).
+      for (int e0 = lane; e0 < ne; e0 += UM * LANES) {
+        T pa[UM], pb[UM];
+        bool pk[UM];
+#pragma unroll
+        for (int u = 0; u < UM; ++u) {
+          const int e = min(e0 + u * LANES, ne - 1);
+          const int2 it = item(e);
+          const int2 sh = shifts(e);
+          const bool ok = (e0 + u * LANES < ne) &
+                          shifted_inside(i0, i1, i2, sh.x, nw) &
+                          shifted_inside(i0, i1, i2, sh.y, nw);
+          const T* xp = CONTIG ? x.p[0] : pick(x, (unsigned)sh.y >> 30);
+          pa[u] = B32 ? ld_b32(coef, ok ? cs + (unsigned)it.x : 0u)
+                      : coef[ok ? (unsigned)(c + it.x) : 0u];
+          pb[u] = B32 ? ld_b32(xp, ok ? cs + (unsigned)it.y : 0u)
+                      : xp[ok ? (unsigned)(c + it.y) : 0u];
+          pk[u] = ok;
+        }
+#pragma unroll
+        for (int u = 0; u < UM; ++u) acc += pk[u] ? pa[u] * pb[u] : T(0);
       }
     }
 #pragma unroll
