@@ -8,8 +8,11 @@ fastest for the rest of the process. Each timed batch is queued behind a
 short device-side spin so that the host has enqueued every launch before the
 first one starts: the events then bracket back-to-back kernels and measure
 device time, not the host's launch rate (which would hide the differences
-between candidates on small grids). Set ODIL_AUTOTUNE=0 to skip tuning and
-use the first candidate (the kernel's size-based default).
+between candidates on small grids). `pick_in_situ` re-times a kernel's
+candidates through a larger workload that contains it (a whole multigrid
+cycle), where the other kernels' traffic between its launches changes what
+the caches hold, and replaces the isolated choice. Set ODIL_AUTOTUNE=0 to
+skip tuning and use the first candidate (the kernel's size-based default).
 
 ODIL_AUTOTUNE_FILE names a JSON table of choices (key and choice as their
 Python repr): keys found there are not timed again, and every new choice is
@@ -20,6 +23,7 @@ the timing).
 
 import json
 import os
+import time
 
 import torch
 
@@ -38,10 +42,13 @@ def cached(key):
 _SPIN_CYCLES = 4_000_000  # about 2 ms of device spin ahead of each batch
 
 
-def _hold_device():
+def _hold_device(seconds=None):
+    """Queue a device-side spin of about 2 ms, or of `seconds`."""
     spin = getattr(torch.cuda, '_sleep', None)
     if spin is not None:
-        spin(_SPIN_CYCLES)
+        cycles = _SPIN_CYCLES if seconds is None else \
+            max(_SPIN_CYCLES, int(seconds * _SPIN_CYCLES / 2e-3))
+        spin(cycles)
 
 
 def _file_table():
@@ -109,6 +116,63 @@ def pick(key, candidates, run, reps=10):
     _CACHE[key] = best
     _to_file(key, best)
     return best
+
+
+def pick_in_situ(key, candidates, install, run, reps=4, batches=3):
+    """Fastest candidate for `key` timed inside a larger workload.
+
+    `install(c)` makes candidate c the kernel's launch shape and `run()`
+    executes a workload that contains that kernel among others (e.g. one
+    multigrid cycle, in which the other levels evict this level's data from
+    the caches between its launches, as they do in the solver). The kernel
+    alone, relaunched on the same buffers, runs from the caches and can
+    rank the candidates differently. Each candidate is timed over `batches`
+    batches of `reps` workloads (the fastest batch counts); the choice
+    replaces any choice timed in isolation under `key`, in this process and
+    in the ODIL_AUTOTUNE_FILE table (where it is marked so that a second
+    process installs it without timing). Returns the choice, or None when
+    tuning is off. Not callable under HIP graph capture."""
+    candidates = list(candidates)
+    if not enabled() or not candidates:
+        return None
+    mark = ('in_situ', ) + tuple(key)
+    done = _CACHE.get(mark)
+    if done is None:
+        done = _from_file(mark, candidates)
+    if done is not None:
+        install(done)
+        _CACHE[key] = _CACHE[mark] = done
+        return done
+    if torch.cuda.is_current_stream_capturing():
+        raise RuntimeError('autotune.pick_in_situ called during graph capture')
+    s = torch.cuda.current_stream()
+    times = {}
+    for c in candidates:
+        install(c)
+        run()  # warm-up
+        # The spin ahead of each batch outlasts the host's enqueueing of the
+        # whole batch (twice the measured time of one workload's launches).
+        t0 = time.perf_counter()
+        run()
+        host = time.perf_counter() - t0
+        best = float('inf')
+        for _ in range(batches):
+            _hold_device(2 * reps * host)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            for _ in range(reps):
+                run()
+            e1.record(s)
+            e1.synchronize()
+            best = min(best, e0.elapsed_time(e1) / reps)
+        times[c] = best
+    choice = min(candidates, key=lambda c: times[c])
+    install(choice)
+    _CACHE[key] = _CACHE[mark] = choice
+    _to_file(key, choice)
+    _to_file(mark, choice)
+    return choice
 
 
 def table():

@@ -304,6 +304,21 @@ class BlockStencil:
         return jacobi_zero(f if f_src is None else f_src, self.diagonal(), y,
                            omega, fw=None if f_src is None else f)
 
+    def tune_key(self, contig):
+        """Autotune key of the symmetric sweep's launch shape."""
+        return ('apply_sym', self.nw, len(self.entries), self._max_items,
+                str(self.dtype), contig)
+
+    def sweep_candidates(self, contig, per=(1, 2, 4, 8)):
+        """Launch shapes (lanes, per) of the symmetric sweep: threads per
+        cell with `per` tiles per workgroup; with contiguous fields also the
+        16-byte vector form, two cells per thread and the LDS tiles."""
+        cand = [(L, p) for L in (0, 1, 2, 4, 8, 16) for p in per]
+        if contig:
+            cand += [(-1, 1), (-2, 1)]
+            cand += [(TILED, t) for t in self.tile_shapes()]
+        return cand
+
     def tune(self, x, f, y, omega=0.5):
         """Pick the threads per cell of the symmetric sweep by timing the
         Jacobi sweep on these buffers (cached per operator shape)."""
@@ -311,17 +326,13 @@ class BlockStencil:
             return self
         from odil_amd import autotune
         contig = self._contig(x)
-        key = ('apply_sym', self.nw, len(self.entries), self._max_items,
-               str(self.dtype), contig)
 
         def run(c):
             self.lanes, self.per = c
             self._call_apply(x, y, f, 2, omega)
-        cand = [(L, p) for L in (0, 1, 2, 4, 8, 16) for p in (1, 2, 4, 8)]
-        if contig:
-            cand += [(-1, 1), (-2, 1)]
-            cand += [(TILED, t) for t in self.tile_shapes()]
-        self.lanes, self.per = autotune.pick(key, cand, run)
+        self.lanes, self.per = autotune.pick(self.tune_key(contig),
+                                             self.sweep_candidates(contig),
+                                             run)
         return self
 
     def apply_t(self, x, y=None, accumulate=False):
@@ -1050,6 +1061,32 @@ class MultigridOp:
             if A.sym:
                 b = buf[level]
                 A.tune(b['u'], b['f'], b['v'])
+
+    def tune_in_cycle(self, run):
+        """Re-pick the sweep launch shape of every stored symmetric level
+        (finest first) by timing `run()`, one application of the cycle as
+        the solver makes it, with each candidate of that level in turn:
+        between a level's launches the other levels' traffic evicts its
+        coefficients from the caches, which a sweep timed alone on its own
+        buffers does not see (on the 1025^2 cavity the FP64 513^2 level
+        times fastest with two cells per thread alone and slower than one
+        cell per thread inside the cycle). Shapes are kernel arguments:
+        call before the cycle is captured into a graph."""
+        from odil_amd import autotune
+        buf = self._buffers()
+        for level in range(len(self.AA) - 1):
+            A = self._op(level)
+            if not isinstance(A, BlockStencil) or not A.sym:
+                continue
+            contig = A._contig(buf[level]['u'])
+            cand = A.sweep_candidates(contig, per=(1, ))
+            if (A.lanes, A.per) not in cand:
+                cand.append((A.lanes, A.per))
+
+            def install(c, A=A):
+                A.lanes, A.per = c
+            autotune.pick_in_situ(A.tune_key(contig), cand, install, run)
+        return self
 
     def _invert(self, level):
         """Dense FP64 solution operator of the coarsest level.

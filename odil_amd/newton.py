@@ -241,6 +241,7 @@ class NewtonMultigrid:
         self.sc = DeviceScalars(8, self.device)
         self.N = None
         self._graph = {}
+        self._cycle_tuned = set()
         self._ready = False
         self.timers = {}
         self.last_linear = None
@@ -364,10 +365,27 @@ class NewtonMultigrid:
         v.dot(b, b, s.p(4))
         self._pcg_iters(iters)
 
+    def _tune_cycle(self):
+        """Launch shapes of the stored levels timed inside the cycle this
+        solver applies (once per hierarchy object, before its solve is
+        captured; later hierarchies of the same shapes reuse the choice)."""
+        if id(self.mg) in self._cycle_tuned:
+            return
+        if self.linsolver == 'pcg':
+            run = lambda: self._precond(self.r)
+        else:
+            run = lambda: self.mg.vcycle(self.x_w, self.b_w,
+                                         omega=self.omega,
+                                         ndirect=self.ndirect, pre=self.pre,
+                                         post=self.post)
+        self.mg.tune_in_cycle(run)
+        self._cycle_tuned.add(id(self.mg))
+
     def _run_chunk(self, iters, fresh=True, finish=True):
         if not fresh:
             self._pcg_iters(iters)
             return
+        self._tune_cycle()
         key = (iters, finish, id(self.mg))
         if not self.use_graph:
             self._solve_body(iters, finish)

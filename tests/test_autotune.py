@@ -36,3 +36,33 @@ def test_stale_entry_is_ignored_and_new_choices_written(tmp_path,
     autotune._to_file(('other', 1), 4)
     table = json.loads(path.read_text())
     assert table == {repr(key): '(16, 16)', repr(('other', 1)): '4'}
+
+
+def test_in_situ_choice_is_installed_and_overrides(tmp_path, monkeypatch):
+    path = tmp_path / 'tune.json'
+    key = ('apply_sym', (513, 513), 72, 51, 'torch.float64', True)
+    mark = ('in_situ', ) + key
+    # The isolated choice (-2, 1) and the in-situ one (1, 1) both stored:
+    # the in-situ one is installed without timing and replaces the other.
+    path.write_text(json.dumps({repr(key): repr((-2, 1)),
+                                repr(mark): repr((1, 1))}))
+    _fresh(monkeypatch, path)
+    installed = []
+
+    def run():
+        raise AssertionError('a stored in-situ key must not be timed')
+    cand = [(1, 1), (2, 1), (-2, 1)]
+    assert autotune.pick_in_situ(key, cand, installed.append, run) == (1, 1)
+    assert installed == [(1, 1)]
+    assert autotune.cached(key) == (1, 1)
+    assert autotune.pick(key, cand, lambda c: None) == (1, 1)
+
+
+def test_in_situ_off_with_autotune_off(monkeypatch):
+    monkeypatch.setenv('ODIL_AUTOTUNE', '0')
+    monkeypatch.setattr(autotune, '_CACHE', {})
+
+    def fail(*a):
+        raise AssertionError('tuning is off')
+    assert autotune.pick_in_situ(('k', 1), [(1, 1), (2, 1)], fail,
+                                 fail) is None
