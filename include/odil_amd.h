@@ -562,6 +562,16 @@ int odil_axpy2_ratio(int dtype, int nvar, long long N, void* const* x,
                      const void* const* p, void* const* r,
                      const void* const* q, const double* num,
                      const double* den, void* stream);
+/* odil_axpy2_ratio, then y_k = omega r_k / d_k with the updated r (the
+ * arithmetic of odil_jacobi_zero, same dtype for every vector) in the same
+ * pass: the conjugate-gradient updates and the first sweep from zero of the
+ * preconditioner cycle applied to r next. No reference counterpart (the
+ * reference solves with plain V-cycles, util.py:588-599). */
+int odil_axpy2_jacobi_zero(int dtype, int nvar, long long N, void* const* x,
+                           const void* const* p, void* const* r,
+                           const void* const* q, const double* num,
+                           const double* den, const void* const* d,
+                           void* const* y, double omega, void* stream);
 /* Flexible-CG pass after the preconditioner, one read of each vector:
  * z = zw converted to dtype (the Krylov precision of r and zo);
  * *rz_prev = *rz; *rz = r.z; *rzo = r.zo with zo the previous z (sums in

@@ -70,11 +70,14 @@ __global__ void __launch_bounds__(256)
 
 // x_k += (num / den) p_k and r_k -= (num / den) q_k in one pass: the two
 // conjugate-gradient updates sharing one step length (den == 0 -> none).
-template <class T>
+// JZ: also y_k = omega r_k / dg_k with the updated r (k_jacobi_zero's
+// arithmetic), the first sweep of the preconditioner cycle that follows.
+template <class T, bool JZ>
 __global__ void __launch_bounds__(256)
     k_axpy2_ratio(VP<T> x, CVP<T> p, VP<T> r, CVP<T> q, int nvar, size_t N,
                   const double* __restrict__ num,
-                  const double* __restrict__ den) {
+                  const double* __restrict__ den, CVP<T> dg, VP<T> y,
+                  T omega) {
   const double d = *den;
   const T a = d != 0.0 ? T(*num / d) : T(0);
   const T an = d != 0.0 ? T(-*num / d) : T(0);
@@ -83,7 +86,9 @@ __global__ void __launch_bounds__(256)
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += stride) {
       x.p[k][i] += a * p.p[k][i];
-      r.p[k][i] += an * q.p[k][i];
+      const T rv = r.p[k][i] + an * q.p[k][i];
+      r.p[k][i] = rv;
+      if (JZ) y.p[k][i] = omega * rv / dg.p[k][i];
     }
 }
 
@@ -381,20 +386,52 @@ int odil_axpy2_ratio(int dtype, int nvar, long long N, void* const* x,
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (dtype == kF64)
-    hipLaunchKernelGGL(k_axpy2_ratio<double>, dim3(grid_for(N)), dim3(256), 0,
-                       s, mkv<double>(x, nvar), mkc<double>(p, nvar),
-                       mkv<double>(r, nvar), mkc<double>(q, nvar), nvar,
-                       (size_t)N, num, den);
+    hipLaunchKernelGGL((k_axpy2_ratio<double, false>), dim3(grid_for(N)),
+                       dim3(256), 0, s, mkv<double>(x, nvar),
+                       mkc<double>(p, nvar), mkv<double>(r, nvar),
+                       mkc<double>(q, nvar), nvar, (size_t)N, num, den,
+                       CVP<double>{}, VP<double>{}, 0.0);
   else if (dtype == kF32)
-    hipLaunchKernelGGL(k_axpy2_ratio<float>, dim3(grid_for(N)), dim3(256), 0,
-                       s, mkv<float>(x, nvar), mkc<float>(p, nvar),
-                       mkv<float>(r, nvar), mkc<float>(q, nvar), nvar,
-                       (size_t)N, num, den);
+    hipLaunchKernelGGL((k_axpy2_ratio<float, false>), dim3(grid_for(N)),
+                       dim3(256), 0, s, mkv<float>(x, nvar),
+                       mkc<float>(p, nvar), mkv<float>(r, nvar),
+                       mkc<float>(q, nvar), nvar, (size_t)N, num, den,
+                       CVP<float>{}, VP<float>{}, 0.0f);
   else {
     set_error("axpy2_ratio: bad dtype");
     return kErrArg;
   }
   return check_launch("axpy2_ratio");
+}
+
+int odil_axpy2_jacobi_zero(int dtype, int nvar, long long N, void* const* x,
+                           const void* const* p, void* const* r,
+                           const void* const* q, const double* num,
+                           const double* den, const void* const* d,
+                           void* const* y, double omega, void* stream) {
+  if (nvar < 1 || nvar > kMaxVec || N < 0 || !num || !den || !d || !y) {
+    set_error("axpy2_jacobi_zero: bad arguments");
+    return kErrArg;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == kF64)
+    hipLaunchKernelGGL((k_axpy2_ratio<double, true>), dim3(grid_for(N)),
+                       dim3(256), 0, s, mkv<double>(x, nvar),
+                       mkc<double>(p, nvar), mkv<double>(r, nvar),
+                       mkc<double>(q, nvar), nvar, (size_t)N, num, den,
+                       mkc<double>(d, nvar), mkv<double>(y, nvar), omega);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL((k_axpy2_ratio<float, true>), dim3(grid_for(N)),
+                       dim3(256), 0, s, mkv<float>(x, nvar),
+                       mkc<float>(p, nvar), mkv<float>(r, nvar),
+                       mkc<float>(q, nvar), nvar, (size_t)N, num, den,
+                       mkc<float>(d, nvar), mkv<float>(y, nvar),
+                       (float)omega);
+  else {
+    set_error("axpy2_jacobi_zero: bad dtype");
+    return kErrArg;
+  }
+  return check_launch("axpy2_jacobi_zero");
 }
 
 int odil_fcg_z(int dtype_w, int dtype, int nvar, long long N,

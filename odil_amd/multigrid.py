@@ -1223,20 +1223,36 @@ class MultigridOp:
         return self._buf
 
     def vcycle(self, u, f, omega=0.6, ndirect=5, pre=2, post=2, zero=False,
-               f_src=None):
+               f_src=None, first_done=False):
         """V-cycle with the same arithmetic as step() but on preallocated
         level buffers (no allocation, capturable in a HIP graph). `u` is
         updated in place; `f` is read only. zero=True: the cycle starts from
         u = 0 whatever u holds (the first pre-smoothing sweep is then one
         pass, y = omega f / d). f_src (with zero=True): the right-hand side
-        in another precision, converted into `f` by that first pass."""
+        in another precision, converted into `f` by that first pass.
+        first_done (with zero=True, pre > 0, no f_src): the caller has
+        already stored that first pass in the buffer named by
+        first_sweep()."""
         buf = self._buffers()
         if f_src is not None and not (zero and pre > 0):
             for a, b in zip(f, f_src):
                 a.copy_(b)
             f_src = None
-        self._vc(0, u, f, buf[0]['v'], omega, ndirect, pre, post, zero, f_src)
+        if first_done and not (zero and pre > 0 and f_src is None
+                               and not self._is_direct(0, ndirect)):
+            raise ValueError('first_done needs zero=True, pre > 0, no f_src '
+                             'and a finest level that is not solved directly')
+        self._vc(0, u, f, buf[0]['v'], omega, ndirect, pre, post, zero, f_src,
+                 first_done=first_done)
         return u
+
+    def first_sweep(self, omega):
+        """(diagonal, output fields, damping) of the finest level's first
+        sweep from zero in vcycle(zero=True): y = damping f / diagonal into
+        the output fields, which vcycle(first_done=True) then takes as
+        done."""
+        return (self._op(0).diagonal(), self._buffers()[0]['v'],
+                self.level_omega(0, omega))
 
     def _is_direct(self, level, ndirect):
         return level + 1 >= self.nlevels or min(self.nnw[level]) <= ndirect

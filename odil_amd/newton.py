@@ -100,6 +100,14 @@ class VectorOps:
                                           ptrs(x), ptrs(p), ptrs(r), ptrs(q),
                                           num, den, stream()), 'axpy2_ratio')
 
+    def axpy2_jacobi_zero(self, x, p, r, q, num, den, d, y, omega):
+        """axpy2, then y = omega r / d with the updated r, in one pass."""
+        check(_lib.lib().odil_axpy2_jacobi_zero(
+            c_int(_lib.dtype_tag(x[0].dtype)), c_int(len(x)),
+            _ll(x[0].numel()), ptrs(x), ptrs(p), ptrs(r), ptrs(q), num, den,
+            ptrs(d), ptrs(y), c_double(omega), stream()),
+            'axpy2_jacobi_zero')
+
     def fcg_z(self, zw, r, zo, rz, rz_prev, rzo):
         """z = zw in FP64; rz_prev = rz, rz = r.z, rzo = r.zo; zo = z."""
         check(_lib.lib().odil_fcg_z(c_int(_lib.dtype_tag(zw[0].dtype)),
@@ -145,6 +153,9 @@ class FusedNormal:
         it and stores its conversion into f."""
         return jacobi_zero(f if f_src is None else f_src, self.diag, y, omega,
                            fw=None if f_src is None else f)
+
+    def diagonal(self):
+        return self.diag
 
 
 class NewtonMultigrid:
@@ -276,13 +287,15 @@ class NewtonMultigrid:
         self._ready = True
 
     # -- linear solve on fixed buffers (capturable) ----------------------
-    def _precond(self, r):
+    def _precond(self, r, first_done=False):
         """M r: one V-cycle from zero in the working precision (an FP32
         cycle converts an FP64 r in its first sweep). Returns the result in
-        the working precision (self.z or self.x_w)."""
+        the working precision (self.z or self.x_w). first_done: the finest
+        level's first sweep is already in its buffer (see _fused_sweep)."""
         if self.wdtype == r[0].dtype:
             self.mg.vcycle(self.z, r, omega=self.omega, ndirect=self.ndirect,
-                           pre=self.pre, post=self.post, zero=True)
+                           pre=self.pre, post=self.post, zero=True,
+                           first_done=first_done)
             return self.z
         self.mg.vcycle(self.x_w, self.b_w, omega=self.omega,
                        ndirect=self.ndirect, pre=self.pre, post=self.post,
@@ -302,15 +315,32 @@ class NewtonMultigrid:
         else:
             self._N64.apply(p, q)
 
+    def _fused_sweep(self):
+        """True when the residual update of each iteration also makes the
+        preconditioner cycle's first sweep from zero (y = omega r / d on
+        the finest level): r in the working precision, a pre-sweep and a
+        finest level that is smoothed, not solved directly."""
+        return (self.wdtype == self.kdtype and self.pre > 0
+                and not self.mg._is_direct(0, self.ndirect))
+
     def _pcg_iters(self, iters):
         # Scalars: s0 = r.z, s1 = previous r.z, s2 = p.N p, s3 = r.z_old,
         # s4 = |b|^2, s6 = |r|^2. zo holds the latest preconditioned
         # residual z in FP64 (fcg_z moves the new z there after using the
         # old one for s3).
         v, s = self.vec, self.sc
+        fused = self._fused_sweep()
         for _ in range(iters):
             self._apply_N(self.p, self.q)
             v.dot(self.p, self.q, s.p(2))
+            if fused:
+                d, y0, om = self.mg.first_sweep(self.omega)
+                v.axpy2_jacobi_zero(self.xk, self.p, self.r, self.q, s.p(0),
+                                    s.p(2), d, y0, om)
+                zw = self._precond(self.r, first_done=True)
+                v.fcg_z(zw, self.r, self.zo, s.p(0), s.p(1), s.p(3))
+                v.xpay(self.p, self.zo, s.p(0), s.p(1), sub=s.p(3))
+                continue
             v.axpy2(self.xk, self.p, self.r, self.q, s.p(0), s.p(2))
             zw = self._precond(self.r)
             v.fcg_z(zw, self.r, self.zo, s.p(0), s.p(1), s.p(3))

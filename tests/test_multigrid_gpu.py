@@ -542,6 +542,60 @@ def test_restrict_smooth_matches_two_passes(dev, nw, dtype):
         assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_axpy2_jacobi_zero_matches_two_passes(dev, dtype):
+    """The conjugate-gradient update fused with the next cycle's first
+    sweep (odil_axpy2_jacobi_zero) equals odil_axpy2_ratio followed by the
+    zero-start sweep, bit for bit; a zero step length leaves x and r as
+    they are."""
+    from odil_amd.multigrid import jacobi_zero
+    from odil_amd.newton import DeviceScalars, VectorOps
+    nw = (33, 47)
+    x, p, r, q = (_vec(dev, dtype, 3, nw, k) for k in (21, 22, 23, 24))
+    d = [a.abs() + 0.5 for a in _vec(dev, dtype, 3, nw, 25)]
+    sc = DeviceScalars(2, dev)
+    v = VectorOps(dev)
+    for num, den in ((0.37, 1.9), (0.37, 0.0)):
+        sc.t[0], sc.t[1] = num, den
+        x1, r1 = [a.clone() for a in x], [a.clone() for a in r]
+        v.axpy2(x1, p, r1, q, sc.p(0), sc.p(1))
+        y1 = jacobi_zero(r1, d, [torch.empty_like(a) for a in r1], 0.6)
+        x2, r2 = [a.clone() for a in x], [a.clone() for a in r]
+        y2 = [torch.full_like(a, -1.) for a in r]
+        v.axpy2_jacobi_zero(x2, p, r2, q, sc.p(0), sc.p(1), d, y2, 0.6)
+        for a, b in zip(x1 + r1 + y1, x2 + r2 + y2):
+            assert torch.equal(a, b)
+        if den == 0:
+            for a, b in zip(x1 + r1, x + r):
+                assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_fcg_fused_first_sweep_same_iterates(dev, precision, monkeypatch):
+    """Newton steps whose FCG iterations make the cycle's first sweep in
+    the residual update reach the same state, bit for bit, as with the
+    update and the sweep as two passes."""
+    from odil_amd.newton import NewtonMultigrid
+    from odil_amd.operators import Cavity
+    out = []
+    for fused in (True, False):
+        if not fused:
+            monkeypatch.setattr(NewtonMultigrid, '_fused_sweep',
+                                lambda self: False)
+        nm = NewtonMultigrid(Cavity((65, 65), 0.01), nlvl=4, maxiter=6,
+                             smooth_pre=1, smooth_post=1,
+                             coarse_smooth=(1, 1), precision=precision,
+                             use_graph=False)
+        assert nm._fused_sweep() == fused
+        fields = [torch.zeros(65, 65, dtype=torch.float64, device=dev)
+                  for _ in range(3)]
+        for _ in range(3):
+            nm.step(fields)
+        out.append(fields)
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
+
+
 # -- sparse-matrix Multigrid (multigrid.py:72-339) ---------------------------
 def _poisson_dirichlet17():
     n = 17
