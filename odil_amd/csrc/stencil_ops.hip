@@ -899,39 +899,48 @@ __global__ void __launch_bounds__(256)
                      (I1 > 0 && I1 < nwh.n1 - 1) &&
                      (DIM < 3 || (I2 > 0 && I2 < nwh.n2 - 1));
   const int c0 = 2 * I0, c1 = 2 * I1, c2 = 2 * I2;
+  // Boundary nodes of the injecting restriction (rmode 0) take the centre
+  // value: weight 1 there and 0 elsewhere in the same branch-free sum.
+  const bool inj = !inner && rmode == 0;
   T acc[NV];
-  if (!inner && rmode == 0) {
-    const unsigned c = ((unsigned)c0 * nw.n1 + c1) * nw.n2 + c2;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = fine.p[k][c];
-  } else {
+  for (int k = 0; k < NV; ++k) acc[k] = 0;
+  constexpr int R2 = DIM == 3 ? 1 : 0;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0;
-    constexpr int R2 = DIM == 3 ? 1 : 0;
+  for (int r0 = -1; r0 <= 1; ++r0)
 #pragma unroll
-    for (int r0 = -1; r0 <= 1; ++r0)
+    for (int r1 = -1; r1 <= 1; ++r1)
 #pragma unroll
-      for (int r1 = -1; r1 <= 1; ++r1)
+      for (int r2 = -R2; r2 <= R2; ++r2) {
+        // Addresses from the bounds alone, so that the fine-grid loads
+        // issue together with the weight loads instead of after them.
+        const double wr = rw[(r0 + 1) * 9 + (r1 + 1) * 3 + (r2 + 1)];
+        const bool centre = r0 == 0 && r1 == 0 && r2 == 0;
+        const double w = inj ? (centre ? 1.0 : 0.0) : wr;
+        const bool in = inside(c0 + r0, c1 + r1, c2 + r2, nw);
+        const unsigned q =
+            in ? ((unsigned)(c0 + r0) * nw.n1 + (c1 + r1)) * nw.n2 +
+                     (c2 + r2)
+               : 0u;
+        const bool ok = in & (w != 0.0);
 #pragma unroll
-        for (int r2 = -R2; r2 <= R2; ++r2) {
-          const double w = rw[(r0 + 1) * 9 + (r1 + 1) * 3 + (r2 + 1)];
-          const bool ok = (w != 0.0) & inside(c0 + r0, c1 + r1, c2 + r2, nw);
-          const unsigned q =
-              ok ? ((unsigned)(c0 + r0) * nw.n1 + (c1 + r1)) * nw.n2 +
-                       (c2 + r2)
-                 : 0u;
-#pragma unroll
-          for (int k = 0; k < NV; ++k) {
-            const T v = fine.p[k][q];
-            acc[k] += ok ? T(w) * v : T(0);
-          }
+        for (int k = 0; k < NV; ++k) {
+          const T v = fine.p[k][q];
+          acc[k] += ok ? T(w) * v : T(0);
         }
-  }
+      }
+  // The sweep's diagonal, loaded unconditionally (without a sweep, from
+  // the coarse output at the same index) so that its loads overlap the
+  // restriction's; a conditional load would be waited for on its own.
+  const bool smooth = sm.p[0] != nullptr;
+  T dv[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) dv[k] = (smooth ? d.p[k] : coarse.p[k])[I];
 #pragma unroll
   for (int k = 0; k < NV; ++k) coarse.p[k][I] = acc[k];
-  if (sm.p[0] != nullptr) {
+  if (smooth) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k) sm.p[k][I] = omega * acc[k] / d.p[k][I];
+    for (int k = 0; k < NV; ++k) sm.p[k][I] = omega * acc[k] / dv[k];
   }
 }
 
