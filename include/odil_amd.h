@@ -117,8 +117,8 @@ int odil_cavity_nblocks(int kind, int dtype, int nx, int ny);
  * (3 x 65536 doubles per device, held until the process ends) and their
  * fixed-order sums go to ssq[0..2] (device, the values odil_reduce_partials
  * gives for odil_cavity_loss_grad's partials). Grids that need more than
- * 65536 partials per equation (FP64 above about 2048 x 2048 cells) return
- * ODIL_ERR_CAPACITY. Calls that overlap on one device race on the array. */
+ * 65536 partials per equation (square grids above 2022 x 2022 cells in FP64,
+ * 2822 x 2822 in FP32) return ODIL_ERR_CAPACITY. Calls that overlap on one device race on the array. */
 int odil_cavity_loss_grad_noalloc(int dtype, int nx, int ny, double dx,
                                   double dy, double mu, int bc, int rhiechow,
                                   const void* const* fields,

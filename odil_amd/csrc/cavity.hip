@@ -1027,8 +1027,9 @@ extern "C" {
 // the per-wavefront partials go to the library's device array and their
 // fixed-order sums to ssq[0..2] (device; sum of f_e^2 per equation, the same
 // values odil_reduce_partials gives for odil_cavity_loss_grad's partials).
-// Grids needing more than 65536 partials per equation (FP64 above about
-// 2048 x 2048 cells) are rejected with ODIL_ERR_CAPACITY. Calls that overlap
+// Grids needing more than 65536 partials per equation (square grids above
+// 2022 x 2022 cells in FP64, 2822 x 2822 in FP32) are rejected with
+// ODIL_ERR_CAPACITY. Calls that overlap
 // on one device race on the shared array.
 int odil_cavity_loss_grad_noalloc(int dtype, int nx, int ny, double dx,
                                   double dy, double mu, int bc, int rhiechow,
