@@ -79,8 +79,9 @@ int odil_reduce_partials(const double* partial, int nrows, int nb,
  *                    shifts[2 s] (2D); returns the number of entries
  *                    (either array may be NULL to query the count).
  *  odil_X_nblocks    host call: partial-sum count of kind 0 (residual) or
- *                    1 (loss_grad / jtf) for the grid: one partial per
- *                    wavefront of the launch.
+ *                    1 (loss_grad / jtf) for the grid, and for the cavity
+ *                    kind 2 (normal_apply_dot): one partial per wavefront
+ *                    of the launch.
  *  odil_X_normal_apply  matrix-free N = J^T J with J recomputed from
  *                    `state`: mode 0 y = N x, 1 y = b - N x,
  *                    2 y = x + omega (b - N x) / d (weighted Jacobi),
@@ -158,6 +159,17 @@ int odil_cavity_normal_apply(int dtype, int nx, int ny, double dx, double dy,
                              const void* const* b, const void* const* d,
                              void* const* y, double omega, int mode,
                              void* stream);
+/* Mode 0 of odil_cavity_normal_apply, y = N x, and *dot = sum of x * y over
+ * every cell and field (FP64, device scalar; partial holds
+ * odil_cavity_nblocks(2, dtype, nx, ny) doubles, summed in a fixed order):
+ * the product and the p.Np of a conjugate-gradient iteration in one launch.
+ * No reference counterpart (its Krylov-accelerated solves are library
+ * calls, pyamg with accel='cg', linsolver.py:77-88). */
+int odil_cavity_normal_apply_dot(int dtype, int nx, int ny, double dx,
+                                 double dy, double mu, int bc, int rhiechow,
+                                 const void* const* state,
+                                 const void* const* x, void* const* y,
+                                 double* partial, double* dot, void* stream);
 int odil_cavity_jtf(int dtype, int nx, int ny, double dx, double dy,
                     double mu, int bc, int rhiechow, const void* const* fields,
                     void* const* out, double scale, double* partial,

@@ -514,7 +514,8 @@ struct CavityOp {
       const FieldPtrs<T, 3>& st_in, const FieldPtrs<T, 3>& x_in,
       const FieldPtrs<T, 3>& b_in, const FieldPtrs<T, 3>& d_in,
       const OutPtrs<T, 3>& y, int n0, int n1, T omega, int mode,
-      const Params& P, int x0, int y0, T* smem, int v0, int v1) {
+      const Params& P, int x0, int y0, T* smem, int v0, int v1, bool dot,
+      double& dacc) {
     constexpr int A0 = TX + 4, A1 = TY + 4, NA = A0 * A1;
     constexpr int E0 = TX + 2, E1 = TY + 2, NE1 = E0 * E1;
     constexpr int NPT = TX * TY / 256;
@@ -603,6 +604,7 @@ struct CavityOp {
         else
           out = X[k * NA + s] + omega * (bv[p][k] - g[k]) / dv[p][k];
         y.f[k][c] = out;
+        if (dot) dacc += (double)X[k * NA + s] * (double)out;
       }
     }
   }
@@ -925,10 +927,12 @@ int odil_cavity_extras_jac_entries(int* ent, int* shifts) {
   return E::NJ;
 }
 
-// Number of per-block partials written by the residual (kind 0) and
-// loss-grad (kind 1) kernels of a 2D operator for the given dtype.
+// Number of per-block partials written by the residual (kind 0),
+// loss-grad (kind 1) and normal-operator product with dot (kind 2) kernels
+// for the given dtype.
 int odil_cavity_nblocks(int kind, int dtype, int nx, int ny) {
   if (kind == 0) return residual_nblocks(nx, ny, CavityOp<float>::R);
+  if (kind == 2) return normal_nblocks<CavityOp<float>>(dtype == kF32, nx, ny);
   return loss_grad_nblocks<CavityOp<float>>(dtype == kF32, nx, ny);
 }
 
@@ -949,6 +953,31 @@ int odil_cavity_normal_apply(int dtype, int nx, int ny, double dx, double dy,
   return dispatch_normal_apply<CavityOp<double>, CavityOp<float>>(
       dtype, state, x, b, d, y, nx, ny, omega, mode, P,
       static_cast<hipStream_t>(stream));
+}
+
+// y = N x (mode 0 of odil_cavity_normal_apply) and *dot = sum over the
+// cells and fields of x * y (FP64; per-wavefront partials, fixed-order sum).
+int odil_cavity_normal_apply_dot(int dtype, int nx, int ny, double dx,
+                                 double dy, double mu, int bc, int rhiechow,
+                                 const void* const* state,
+                                 const void* const* x, void* const* y,
+                                 double* partial, double* dot,
+                                 void* stream) {
+  if (nx < 3 || ny < 3 || !state || !x || !y || !partial || !dot) {
+    set_error("cavity_normal_apply_dot: need nx, ny >= 3, state, x, y, "
+              "partial and dot");
+    return kErrArg;
+  }
+  CavityParams P = cavity_params(dx, dy, mu, bc, rhiechow);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int nb = 0;
+  const int st = dispatch_normal_apply<CavityOp<double>, CavityOp<float>>(
+      dtype, state, x, nullptr, nullptr, y, nx, ny, 0.0, 0, P, s, partial,
+      &nb);
+  if (st != kOk) return st;
+  hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(256), 0, s, partial,
+                     nb, dot);
+  return check_launch("normal_apply_dot");
 }
 
 // out = scale * J^T f(state) plus per-block sums of f^2 (as loss_grad).

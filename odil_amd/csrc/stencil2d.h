@@ -867,6 +867,17 @@ inline int loss_grad_nblocks(bool fp32, int n0, int n1) {
   return fp32 ? loss_grad_nblocks_t<Op, kGradTx32, kGradTy32>(n0, n1)
               : loss_grad_nblocks_t<Op, kGradTx64, kGradTy64>(n0, n1);
 }
+// Partial count of the normal-operator launch with a dot product (the thin
+// split of operators with interior tiles, mode 0).
+template <class Op>
+inline int normal_nblocks(bool fp32, int n0, int n1) {
+  const ThinSplit sp =
+      fp32 ? thin_split(n0, n1, Op::R + Op::RJ, kNormTx32, kNormTy32,
+                        256 / Op::NS)
+           : thin_split(n0, n1, Op::R + Op::RJ, kNormTx64, kNormTy64,
+                        256 / Op::NS);
+  return (sp.nint + sp.nfb) * kWavesPerBlock;
+}
 
 // Normal operator N = J^T J (modes 0, 1, 2 of normal_apply_tile) at frame
 // cells (within H = R + RJ of a face) on the operator's generic path, for
@@ -881,7 +892,8 @@ __device__ inline void frame_normal(
     const FieldPtrs<T, Op::NF>& st_in, const FieldPtrs<T, Op::NF>& x_in,
     const FieldPtrs<T, Op::NF>& b_in, const FieldPtrs<T, Op::NF>& d_in,
     const OutPtrs<T, Op::NF>& y, int n0, int n1, T omega, int mode,
-    const typename Op::Params& P, const ThinSplit& sp, int fb, T* o) {
+    const typename Op::Params& P, const ThinSplit& sp, int fb, T* o,
+    bool dot, double& dacc) {
   constexpr int NF = Op::NF, NE = Op::NE, NS = Op::NS;
   constexpr int H = Op::R + Op::RJ, PER = 256 / NS;
   const int t = threadIdx.x;
@@ -953,6 +965,7 @@ __device__ inline void frame_normal(
       else
         r = x_in.f[k][c] + omega * (b_in.f[k][c] - g) / d_in.f[k][c];
       y.f[k][c] = r;
+      if (dot) dacc += (double)x_in.f[k][c] * (double)r;
     }
   }
 }
@@ -960,28 +973,38 @@ __device__ inline void frame_normal(
 // k_normal_apply (modes 0, 1, 2) over a thin split: the first nfb blocks in
 // dispatch order run frame_normal, the others the operator's
 // interior_normal_tile on tiles of interior cells (the last tile of a row
-// or column partial).
+// or column partial). With `dotp` (mode 0) each wavefront also writes its
+// FP64 sum of x * y over its cells to dotp[b * 4 + w] (fixed order).
 template <class T, class Op, int TX, int TY, int STG>
 __global__ void __launch_bounds__(256)
     k_normal_apply_split(FieldPtrs<T, Op::NF> st_in,
                          FieldPtrs<T, Op::NF> x_in, FieldPtrs<T, Op::NF> b_in,
                          FieldPtrs<T, Op::NF> d_in, OutPtrs<T, Op::NF> y,
                          int n0, int n1, T omega, int mode,
-                         typename Op::Params P, ThinSplit sp) {
+                         typename Op::Params P, ThinSplit sp,
+                         double* __restrict__ dotp) {
   extern __shared__ unsigned char smem_raw[];
   T* smem = reinterpret_cast<T*>(smem_raw);
   constexpr int H = Op::R + Op::RJ;
   const unsigned b = blockIdx.x;
+  const bool dot = dotp != nullptr;
+  double dacc = 0;
   if ((int)b >= sp.nfb) {
     const unsigned t = xcd_block(b - (unsigned)sp.nfb, (unsigned)sp.nint);
     const int x0 = H + (int)(t / (unsigned)sp.ntx) * TX;
     const int y0 = H + (int)(t % (unsigned)sp.ntx) * TY;
     Op::template interior_normal_tile<TX, TY>(st_in, x_in, b_in, d_in, y,
                                               n0, n1, omega, mode, P, x0, y0,
-                                              smem, n0 - H, n1 - H);
+                                              smem, n0 - H, n1 - H, dot,
+                                              dacc);
   } else {
     frame_normal<T, Op>(st_in, x_in, b_in, d_in, y, n0, n1, omega, mode, P,
-                        sp, (int)b, smem);
+                        sp, (int)b, smem, dot, dacc);
+  }
+  if (dot) {
+    double v[1] = {dacc};
+    wave_partials<1>(v, dotp, (size_t)gridDim.x * kWavesPerBlock,
+                     (size_t)b * kWavesPerBlock + (threadIdx.x >> 6));
   }
 }
 
@@ -1071,9 +1094,15 @@ int launch_normal_apply_t(const void* const* state, const void* const* x,
                           const void* const* b, const void* const* d,
                           void* const* y, int n0, int n1, double omega,
                           int mode, const typename Op::Params& P,
-                          hipStream_t stream) {
+                          hipStream_t stream, double* dotp = nullptr,
+                          int* nblocks = nullptr) {
   if (mode < 0 || mode > 3) {
     set_error("normal_apply: mode must be 0..3");
+    return kErrArg;
+  }
+  if (dotp && (mode != 0 || !has_interior_tiles<Op>::value)) {
+    set_error("normal_apply: the dot product needs mode 0 and an operator "
+              "with interior tiles");
     return kErrArg;
   }
   if ((mode != 3 && !x) || ((mode == 1 || mode == 2) && !b) ||
@@ -1105,9 +1134,10 @@ int launch_normal_apply_t(const void* const* state, const void* const* x,
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)smem2);
+      if (nblocks) *nblocks = (sp.nfb + sp.nint) * kWavesPerBlock;
       hipLaunchKernelGGL(kern, dim3(sp.nfb + sp.nint), dim3(256), smem2,
                          stream, S, X, B, D, Y, n0, n1, (T)omega, mode, P,
-                         sp);
+                         sp, dotp);
       return check_launch("normal_apply");
     }
   }
@@ -1125,15 +1155,16 @@ int dispatch_normal_apply(int dtype, const void* const* state,
                           const void* const* x, const void* const* b,
                           const void* const* d, void* const* y, int n0,
                           int n1, double omega, int mode,
-                          const typename Op64::Params& P, hipStream_t s) {
+                          const typename Op64::Params& P, hipStream_t s,
+                          double* dotp = nullptr, int* nblocks = nullptr) {
   if (dtype == kF64)
     return launch_normal_apply_t<double, Op64, kNormTx64, kNormTy64,
                                  kNormStage64>(
-        state, x, b, d, y, n0, n1, omega, mode, P, s);
+        state, x, b, d, y, n0, n1, omega, mode, P, s, dotp, nblocks);
   if (dtype == kF32)
     return launch_normal_apply_t<float, Op32, kNormTx32, kNormTy32,
                                  kNormStage32>(
-        state, x, b, d, y, n0, n1, omega, mode, P, s);
+        state, x, b, d, y, n0, n1, omega, mode, P, s, dotp, nblocks);
   set_error("normal_apply: bad dtype %d", dtype);
   return kErrArg;
 }

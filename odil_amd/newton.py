@@ -251,6 +251,7 @@ class NewtonMultigrid:
         self.vec = VectorOps(self.device)
         self.sc = DeviceScalars(8, self.device)
         self.N = None
+        self._ndot = None
         self._graph = {}
         self._cycle_tuned = set()
         self._ready = False
@@ -283,6 +284,13 @@ class NewtonMultigrid:
             # and the FP64 linear residual of the refinement passes.
             self.xk = self.x if kd == f64 else self._vecs(kd)
             self.r64 = self._vecs(f64) if self.refine else None
+            # Partials of the product launch that also forms p.Np.
+            self._ndot = None
+            has = getattr(self.op, 'has_normal_dot', None)
+            if self.matrix_free and has is not None and has():
+                self._ndot = torch.empty(
+                    self.op.nblocks(2, _lib.dtype_tag(kd)),
+                    dtype=f64, device=self.device)
         self._fields = fields
         self._ready = True
 
@@ -302,18 +310,29 @@ class NewtonMultigrid:
                        zero=True, f_src=r)
         return self.x_w
 
-    def _apply_N(self, p, q):
+    def _apply_N(self, p, q, dot=None):
         """q = N p on the finest level in the Krylov precision (an FP32
-        solve applies N matrix-free from the FP32 copy of the state)."""
+        solve applies N matrix-free from the FP32 copy of the state). With
+        `dot` (device scalar pointer), a matrix-free operator that fuses it
+        also writes p.q there in the same launch; returns whether it did."""
+        fuse = dot is not None and self._ndot is not None
         if self.kdtype == torch.float32:
             if self.matrix_free:
+                if fuse:
+                    self.op.normal_apply_dot(self.state_w, p, q, self._ndot,
+                                             dot)
+                    return True
                 self.op.normal_apply(self.state_w, p, q, mode=0)
             else:
                 self.N.apply(p, q)
         elif self.matrix_free:
+            if fuse:
+                self.op.normal_apply_dot(self._fields, p, q, self._ndot, dot)
+                return True
             self.op.normal_apply(self._fields, p, q, mode=0)
         else:
             self._N64.apply(p, q)
+        return False
 
     def _fused_sweep(self):
         """True when the residual update of each iteration also makes the
@@ -331,8 +350,8 @@ class NewtonMultigrid:
         v, s = self.vec, self.sc
         fused = self._fused_sweep()
         for _ in range(iters):
-            self._apply_N(self.p, self.q)
-            v.dot(self.p, self.q, s.p(2))
+            if not self._apply_N(self.p, self.q, dot=s.p(2)):
+                v.dot(self.p, self.q, s.p(2))
             if fused:
                 d, y0, om = self.mg.first_sweep(self.omega)
                 v.axpy2_jacobi_zero(self.xk, self.p, self.r, self.q, s.p(0),

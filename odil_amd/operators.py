@@ -119,6 +119,22 @@ class Operator:
         check(st, self.prefix + '_normal_apply')
         return y
 
+    def has_normal_dot(self):
+        """Whether the library fuses p.Np into the product (normal_apply_dot)."""
+        return hasattr(_lib.lib(), 'odil_{}_normal_apply_dot'.format(
+            self.prefix))
+
+    def normal_apply_dot(self, state, x, y, partial, dot):
+        """y = N x (mode 0 of normal_apply) and the FP64 sum of x * y into
+        the device scalar at `dot` (a pointer), in one launch; `partial` is
+        a float64 device tensor of nblocks(2, dtype) values."""
+        dtag = self._check_fields(state)
+        st = self._fn('normal_apply_dot')(
+            c_int(dtag), *self._shape_args(), *self._args_for(dtag),
+            ptrs(state), ptrs(x), ptrs(y), ptr(partial), dot, _lib.stream())
+        check(st, self.prefix + '_normal_apply_dot')
+        return y
+
     def jtf(self, state, out, scale, partial=None):
         """out = scale * J^T f(state); returns per-equation sums of f^2
         (float64 device tensor)."""

@@ -164,6 +164,38 @@ def _ops_for_normal(dev, dtype):
 
 
 @pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_normal_apply_dot_matches_product_and_dot(dev, dtype):
+    """odil_cavity_normal_apply_dot: y equals mode 0 of normal_apply bit for
+    bit and the dot equals the FP64 sum of x * y up to summation order, on
+    grids with interior tiles and on one without (every cell a frame
+    cell); the partials start as NaN, so every slot must be written."""
+    from odil_amd import _lib
+    from odil_amd.newton import DeviceScalars
+    from odil_amd.operators import Cavity
+    rng = np.random.default_rng(11)
+    for shape, mu, bc in (((37, 70), 0.01, 0), ((70, 100), 0.005, 1),
+                          ((6, 9), 0.02, 2), ((129, 65), 0.01, 0)):
+        op = Cavity(shape, mu, bc=bc)
+        assert op.has_normal_dot()
+        mk = lambda: [torch.tensor(rng.uniform(-1, 1, shape), dtype=dtype,
+                                   device=dev) for _ in range(3)]
+        st, x = mk(), mk()
+        y1 = [torch.empty_like(a) for a in x]
+        op.normal_apply(st, x, y1, mode=0)
+        part = torch.full((op.nblocks(2, _lib.dtype_tag(dtype)), ),
+                          float('nan'), dtype=torch.float64, device=dev)
+        sc = DeviceScalars(1, dev)
+        y2 = [torch.full_like(a, float('nan')) for a in x]
+        op.normal_apply_dot(st, x, y2, part, sc.p(0))
+        for a, b in zip(y1, y2):
+            assert torch.equal(a, b), shape
+        prod = [a.double() * b.double() for a, b in zip(x, y1)]
+        want = sum(q.sum() for q in prod).item()
+        mag = sum(q.abs().sum() for q in prod).item()
+        assert abs(sc.t[0].item() - want) <= 1e-13 * mag, shape
+
+
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
 def test_normal_apply_matches_stored_normal(dev, dtype):
     from odil_amd.multigrid import BlockStencil, normal_operator
     tol = 1e-12 if dtype == torch.float64 else 3e-5
