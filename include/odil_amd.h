@@ -593,6 +593,14 @@ int odil_fcg_z(int dtype_w, int dtype, int nvar, long long N,
                const void* const* zw, const void* const* r, void* const* zo,
                double* partial, double* rz, double* rz_prev, double* rzo,
                void* stream);
+/* The sums of odil_fcg_z without the conversion and the copy (z, r, zo of
+ * one dtype, none written): *rz_prev = *rz; *rz = r.z; *rzo = r.zo, the
+ * same values odil_fcg_z gives, for a caller that keeps z and the previous
+ * z in two buffers used in turn. */
+int odil_fcg_dots(int dtype, int nvar, long long N, const void* const* z,
+                  const void* const* r, const void* const* zo,
+                  double* partial, double* rz, double* rz_prev, double* rzo,
+                  void* stream);
 /* p_k = z_k + ((num - sub) / den) p_k (sub may be NULL; den == 0: p = z). */
 int odil_xpay_ratio(int dtype, int nvar, long long N, void* const* p,
                     const void* const* z, const double* num,

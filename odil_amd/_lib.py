@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, 'libodil_amd.so')
 
 F32, F64 = 0, 1
 # ABI of include/odil_amd.h these bindings call (odil_abi_version).
-ABI_VERSION = 6
+ABI_VERSION = 7
 _lib = None
 
 c_int = ctypes.c_int

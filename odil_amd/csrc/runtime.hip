@@ -72,6 +72,6 @@ int odil_reduce_partials(const double* partial, int nrows, int nb,
   return check_launch("reduce_partials");
 }
 
-int odil_abi_version(void) { return 6; }
+int odil_abi_version(void) { return 7; }
 
 }  // extern "C"

@@ -96,7 +96,9 @@ __global__ void __launch_bounds__(256)
 // Krylov precision T, per-block FP64 partial sums of r.z (partial[b]) and
 // r.zo (partial[nb + b]) with zo the previous z, then zo = z. Same grid and
 // summation order as k_dot_partial.
-template <class Tw, class T>
+// COPY = false: the sums only (the caller keeps z and the previous z in
+// two buffers used in turn).
+template <class Tw, class T, bool COPY>
 __global__ void __launch_bounds__(256)
     k_fcg_z(CVP<Tw> zw, CVP<T> r, VP<T> zo, int nvar, size_t N,
             double* __restrict__ partial) {
@@ -113,7 +115,7 @@ __global__ void __launch_bounds__(256)
       const double rv = (double)b[i];
       s += rv * (double)z;
       t += rv * (double)c[i];
-      c[i] = z;
+      if (COPY) c[i] = z;
     }
   }
   s = block_sum(s, red);
@@ -445,7 +447,8 @@ int odil_fcg_z(int dtype_w, int dtype, int nvar, long long N,
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
 #define ODIL_FCG_Z(TW, T)                                                    \
-  hipLaunchKernelGGL((k_fcg_z<TW, T>), dim3(kDotBlocks), dim3(256), 0, s,    \
+  hipLaunchKernelGGL((k_fcg_z<TW, T, true>), dim3(kDotBlocks), dim3(256), 0, \
+                     s,                                                      \
                      mkc<TW>(zw, nvar), mkc<T>(r, nvar), mkv<T>(zo, nvar),   \
                      nvar, (size_t)N, partial)
   if (dtype_w == kF64 && dtype == kF64)
@@ -462,6 +465,37 @@ int odil_fcg_z(int dtype_w, int dtype, int nvar, long long N,
   hipLaunchKernelGGL(k_fcg_final, dim3(1), dim3(256), 0, s, partial,
                      kDotBlocks, rz, rz_prev, rzo);
   return check_launch("fcg_z");
+}
+
+int odil_fcg_dots(int dtype, int nvar, long long N, const void* const* z,
+                  const void* const* r, const void* const* zo,
+                  double* partial, double* rz, double* rz_prev, double* rzo,
+                  void* stream) {
+  if (nvar < 1 || nvar > kMaxVec || N < 0 || !z || !r || !zo || !partial ||
+      !rz || !rz_prev || !rzo) {
+    set_error("fcg_dots: bad arguments");
+    return kErrArg;
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // zo is only read (COPY = false).
+  void* const* zo_rw = const_cast<void* const*>(zo);
+  if (dtype == kF64)
+    hipLaunchKernelGGL((k_fcg_z<double, double, false>), dim3(kDotBlocks),
+                       dim3(256), 0, s, mkc<double>(z, nvar),
+                       mkc<double>(r, nvar), mkv<double>(zo_rw, nvar), nvar,
+                       (size_t)N, partial);
+  else if (dtype == kF32)
+    hipLaunchKernelGGL((k_fcg_z<float, float, false>), dim3(kDotBlocks),
+                       dim3(256), 0, s, mkc<float>(z, nvar),
+                       mkc<float>(r, nvar), mkv<float>(zo_rw, nvar), nvar,
+                       (size_t)N, partial);
+  else {
+    set_error("fcg_dots: bad dtype %d", dtype);
+    return kErrArg;
+  }
+  hipLaunchKernelGGL(k_fcg_final, dim3(1), dim3(256), 0, s, partial,
+                     kDotBlocks, rz, rz_prev, rzo);
+  return check_launch("fcg_dots");
 }
 
 // p = z + ((num - sub) / den) p; `sub` may be NULL (conjugate gradients:

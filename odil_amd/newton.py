@@ -117,6 +117,14 @@ class VectorOps:
                                     c_vp(self.partial2.data_ptr()), rz,
                                     rz_prev, rzo, stream()), 'fcg_z')
 
+    def fcg_dots(self, z, r, zo, rz, rz_prev, rzo):
+        """rz_prev = rz, rz = r.z, rzo = r.zo (fcg_z's sums, no copy)."""
+        check(_lib.lib().odil_fcg_dots(c_int(_lib.dtype_tag(z[0].dtype)),
+                                       c_int(len(z)), _ll(z[0].numel()),
+                                       ptrs(z), ptrs(r), ptrs(zo),
+                                       c_vp(self.partial2.data_ptr()), rz,
+                                       rz_prev, rzo, stream()), 'fcg_dots')
+
     def xpay(self, p, z, num, den, sub=None):
         """p = z + ((num - sub) / den) p with device scalars."""
         check(_lib.lib().odil_xpay_ratio(c_int(_lib.dtype_tag(p[0].dtype)),
@@ -254,6 +262,7 @@ class NewtonMultigrid:
         self._ndot = None
         self._graph = {}
         self._cycle_tuned = set()
+        self._zi = 0
         self._ready = False
         self.timers = {}
         self.last_linear = None
@@ -295,16 +304,18 @@ class NewtonMultigrid:
         self._ready = True
 
     # -- linear solve on fixed buffers (capturable) ----------------------
-    def _precond(self, r, first_done=False):
+    def _precond(self, r, first_done=False, out=None):
         """M r: one V-cycle from zero in the working precision (an FP32
         cycle converts an FP64 r in its first sweep). Returns the result in
-        the working precision (self.z or self.x_w). first_done: the finest
-        level's first sweep is already in its buffer (see _fused_sweep)."""
+        the working precision (`out`, default self.z, or self.x_w).
+        first_done: the finest level's first sweep is already in its buffer
+        (see _fused_sweep)."""
         if self.wdtype == r[0].dtype:
-            self.mg.vcycle(self.z, r, omega=self.omega, ndirect=self.ndirect,
+            u = self.z if out is None else out
+            self.mg.vcycle(u, r, omega=self.omega, ndirect=self.ndirect,
                            pre=self.pre, post=self.post, zero=True,
                            first_done=first_done)
-            return self.z
+            return u
         self.mg.vcycle(self.x_w, self.b_w, omega=self.omega,
                        ndirect=self.ndirect, pre=self.pre, post=self.post,
                        zero=True, f_src=r)
@@ -342,13 +353,24 @@ class NewtonMultigrid:
         return (self.wdtype == self.kdtype and self.pre > 0
                 and not self.mg._is_direct(0, self.ndirect))
 
+    def _pingpong(self):
+        """True when the preconditioned residuals go to z and zo in turn
+        (no copy of the new z into zo): z already in the Krylov precision
+        and a fixed iteration budget, so that every solve (and every replay
+        of its captured graph) runs the same buffer sequence from
+        _fcg_pass on."""
+        return self.rtol <= 0 and self.wdtype == self.kdtype
+
     def _pcg_iters(self, iters):
         # Scalars: s0 = r.z, s1 = previous r.z, s2 = p.N p, s3 = r.z_old,
         # s4 = |b|^2, s6 = |r|^2. zo holds the latest preconditioned
-        # residual z in FP64 (fcg_z moves the new z there after using the
-        # old one for s3).
+        # residual z in the Krylov precision (fcg_z moves the new z there
+        # after using the old one for s3); with _pingpong the latest z is
+        # in (z, zo)[self._zi] and the next goes to the other buffer.
         v, s = self.vec, self.sc
         fused = self._fused_sweep()
+        pp = self._pingpong()
+        bufs = (self.z, self.zo)
         for _ in range(iters):
             if not self._apply_N(self.p, self.q, dot=s.p(2)):
                 v.dot(self.p, self.q, s.p(2))
@@ -356,12 +378,16 @@ class NewtonMultigrid:
                 d, y0, om = self.mg.first_sweep(self.omega)
                 v.axpy2_jacobi_zero(self.xk, self.p, self.r, self.q, s.p(0),
                                     s.p(2), d, y0, om)
-                zw = self._precond(self.r, first_done=True)
-                v.fcg_z(zw, self.r, self.zo, s.p(0), s.p(1), s.p(3))
-                v.xpay(self.p, self.zo, s.p(0), s.p(1), sub=s.p(3))
+            else:
+                v.axpy2(self.xk, self.p, self.r, self.q, s.p(0), s.p(2))
+            if pp:
+                new, old = bufs[1 - self._zi], bufs[self._zi]
+                self._precond(self.r, first_done=fused, out=new)
+                v.fcg_dots(new, self.r, old, s.p(0), s.p(1), s.p(3))
+                v.xpay(self.p, new, s.p(0), s.p(1), sub=s.p(3))
+                self._zi = 1 - self._zi
                 continue
-            v.axpy2(self.xk, self.p, self.r, self.q, s.p(0), s.p(2))
-            zw = self._precond(self.r)
+            zw = self._precond(self.r, first_done=fused)
             v.fcg_z(zw, self.r, self.zo, s.p(0), s.p(1), s.p(3))
             v.xpay(self.p, self.zo, s.p(0), s.p(1), sub=s.p(3))
         v.dot(self.r, self.r, s.p(6))
@@ -407,9 +433,16 @@ class NewtonMultigrid:
         for a in self.xk:
             a.zero_()
         zw = self._precond(self.r)
-        # s0 = r.z and zo = z (s1, s3 are not used before being set again).
-        v.fcg_z(zw, self.r, self.zo, s.p(0), s.p(1), s.p(3))
-        for a, c in zip(self.p, self.zo):
+        # s0 = r.z and zo = z (s1, s3 are not used before being set again);
+        # with _pingpong z stays where it is (zw is self.z).
+        if self._pingpong():
+            v.fcg_dots(zw, self.r, self.zo, s.p(0), s.p(1), s.p(3))
+            self._zi = 0
+            src = zw
+        else:
+            v.fcg_z(zw, self.r, self.zo, s.p(0), s.p(1), s.p(3))
+            src = self.zo
+        for a, c in zip(self.p, src):
             a.copy_(c)
         v.dot(b, b, s.p(4))
         self._pcg_iters(iters)

@@ -570,23 +570,45 @@ def test_axpy2_jacobi_zero_matches_two_passes(dev, dtype):
                 assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize('dtype', [torch.float64, torch.float32])
+def test_fcg_dots_match_fcg_z(dev, dtype):
+    """odil_fcg_dots gives odil_fcg_z's three scalars bit for bit and writes
+    no vector."""
+    from odil_amd.newton import DeviceScalars, VectorOps
+    nw = (33, 47)
+    z, r, zo = (_vec(dev, dtype, 3, nw, k) for k in (31, 32, 33))
+    v = VectorOps(dev)
+    a, b = DeviceScalars(3, dev), DeviceScalars(3, dev)
+    a.t[0] = b.t[0] = 2.5
+    z1, r1, zo1 = ([x.clone() for x in y] for y in (z, r, zo))
+    v.fcg_dots(z1, r1, zo1, b.p(0), b.p(1), b.p(2))
+    zo2 = [x.clone() for x in zo]
+    v.fcg_z(z, r, zo2, a.p(0), a.p(1), a.p(2))
+    assert torch.equal(a.t, b.t)
+    for x, y in zip(z1 + r1 + zo1, z + r + zo):
+        assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize('feature', ['_fused_sweep', '_pingpong'])
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
-def test_fcg_fused_first_sweep_same_iterates(dev, precision, monkeypatch):
+def test_fcg_fused_first_sweep_same_iterates(dev, precision, feature,
+                                             monkeypatch):
     """Newton steps whose FCG iterations make the cycle's first sweep in
-    the residual update reach the same state, bit for bit, as with the
-    update and the sweep as two passes."""
+    the residual update (_fused_sweep), or keep the new and the previous
+    preconditioned residual in two buffers used in turn instead of copying
+    (_pingpong), reach the same state, bit for bit, as without."""
     from odil_amd.newton import NewtonMultigrid
     from odil_amd.operators import Cavity
     out = []
     for fused in (True, False):
         if not fused:
-            monkeypatch.setattr(NewtonMultigrid, '_fused_sweep',
+            monkeypatch.setattr(NewtonMultigrid, feature,
                                 lambda self: False)
-        nm = NewtonMultigrid(Cavity((65, 65), 0.01), nlvl=4, maxiter=6,
+        nm = NewtonMultigrid(Cavity((65, 65), 0.01), nlvl=4, maxiter=7,
                              smooth_pre=1, smooth_post=1,
                              coarse_smooth=(1, 1), precision=precision,
                              use_graph=False)
-        assert nm._fused_sweep() == fused
+        assert getattr(nm, feature)() == fused
         fields = [torch.zeros(65, 65, dtype=torch.float64, device=dev)
                   for _ in range(3)]
         for _ in range(3):
