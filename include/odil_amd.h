@@ -79,9 +79,9 @@ int odil_reduce_partials(const double* partial, int nrows, int nb,
  *                    shifts[2 s] (2D); returns the number of entries
  *                    (either array may be NULL to query the count).
  *  odil_X_nblocks    host call: partial-sum count of kind 0 (residual) or
- *                    1 (loss_grad / jtf) for the grid, and for the cavity
- *                    kind 2 (normal_apply_dot): one partial per wavefront
- *                    of the launch.
+ *                    1 (loss_grad / jtf) for the grid: one partial per
+ *                    wavefront of the launch; for the cavity also kind 2
+ *                    (normal_apply_dot): one per workgroup.
  *  odil_X_normal_apply  matrix-free N = J^T J with J recomputed from
  *                    `state`: mode 0 y = N x, 1 y = b - N x,
  *                    2 y = x + omega (b - N x) / d (weighted Jacobi),

@@ -868,7 +868,7 @@ inline int loss_grad_nblocks(bool fp32, int n0, int n1) {
               : loss_grad_nblocks_t<Op, kGradTx64, kGradTy64>(n0, n1);
 }
 // Partial count of the normal-operator launch with a dot product (the thin
-// split of operators with interior tiles, mode 0).
+// split of operators with interior tiles, mode 0): one per workgroup.
 template <class Op>
 inline int normal_nblocks(bool fp32, int n0, int n1) {
   const ThinSplit sp =
@@ -876,7 +876,7 @@ inline int normal_nblocks(bool fp32, int n0, int n1) {
                         256 / Op::NS)
            : thin_split(n0, n1, Op::R + Op::RJ, kNormTx64, kNormTy64,
                         256 / Op::NS);
-  return (sp.nint + sp.nfb) * kWavesPerBlock;
+  return sp.nint + sp.nfb;
 }
 
 // Normal operator N = J^T J (modes 0, 1, 2 of normal_apply_tile) at frame
@@ -973,8 +973,9 @@ __device__ inline void frame_normal(
 // k_normal_apply (modes 0, 1, 2) over a thin split: the first nfb blocks in
 // dispatch order run frame_normal, the others the operator's
 // interior_normal_tile on tiles of interior cells (the last tile of a row
-// or column partial). With `dotp` (mode 0) each wavefront also writes its
-// FP64 sum of x * y over its cells to dotp[b * 4 + w] (fixed order).
+// or column partial). With `dotp` (mode 0) each workgroup also writes its
+// FP64 sum of x * y over its cells to dotp[b] (fixed order; one partial per
+// workgroup keeps the final sum short).
 template <class T, class Op, int TX, int TY, int STG>
 __global__ void __launch_bounds__(256)
     k_normal_apply_split(FieldPtrs<T, Op::NF> st_in,
@@ -1002,9 +1003,9 @@ __global__ void __launch_bounds__(256)
                         sp, (int)b, smem, dot, dacc);
   }
   if (dot) {
-    double v[1] = {dacc};
-    wave_partials<1>(v, dotp, (size_t)gridDim.x * kWavesPerBlock,
-                     (size_t)b * kWavesPerBlock + (threadIdx.x >> 6));
+    __shared__ double red[kWavesPerBlock];
+    const double v = block_sum(dacc, red);
+    if (threadIdx.x == 0) dotp[b] = v;
   }
 }
 
@@ -1134,7 +1135,7 @@ int launch_normal_apply_t(const void* const* state, const void* const* x,
       (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)smem2);
-      if (nblocks) *nblocks = (sp.nfb + sp.nint) * kWavesPerBlock;
+      if (nblocks) *nblocks = sp.nfb + sp.nint;
       hipLaunchKernelGGL(kern, dim3(sp.nfb + sp.nint), dim3(256), smem2,
                          stream, S, X, B, D, Y, n0, n1, (T)omega, mode, P,
                          sp, dotp);
